@@ -1,0 +1,115 @@
+/*
+ * ocl_hip.h -- C ABI of the MI355X (gfx950) batch drop-cutter / push-cutter.
+ *
+ * This is the drop-in boundary behind OpenCAMLib's C++ operation classes:
+ * the host classes in opencamlib_amd/host (ocl::BatchDropCutter,
+ * ocl::BatchPushCutter, ocl::STLSurf, the MillingCutter family) call these
+ * entry points, and so does the Python binding.  Plain pointers and sizes
+ * only; no C++ or torch types.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative OCL_E_* code on
+ *     failure; ocl_hip_last_error() returns the message (thread-local).
+ *   - Host-pointer entry points copy in and out and synchronise the stream.
+ *     *_device entry points take device pointers and are asynchronous on the
+ *     given stream (NULL = the default stream).
+ *   - A mesh handle owns its device memory.  Handles may be used from several
+ *     threads; calls on one handle are serialised by the caller (the reference
+ *     Operation objects are not re-entrant either).
+ *   - Triangle indices returned are positions in the input triangle array.
+ */
+#ifndef OCL_HIP_H
+#define OCL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OCL_OK 0
+#define OCL_E_ARG (-1)
+#define OCL_E_HIP (-2)
+#define OCL_E_NOMEM (-3)
+#define OCL_E_UNSUPPORTED (-4)
+
+/* cutter kinds (src/cutters: cylcutter, ballcutter, bullcutter, conecutter, compositecutter) */
+#define OCL_CUTTER_CYL 0
+#define OCL_CUTTER_BALL 1
+#define OCL_CUTTER_BULL 2
+#define OCL_CUTTER_CONE 3
+#define OCL_CUTTER_COMPOSITE 4
+
+/* precision modes of the drop/push kernels */
+#define OCL_PREC_EXACT64 0 /* reference-order FP64 on every pair */
+#define OCL_PREC_FP64 1    /* FP64 selection kernel + FP64 exact refine */
+#define OCL_PREC_FP32 2    /* FP32 selection kernel only (no refine) */
+#define OCL_PREC_MIXED 3   /* FP32 selection + FP64 refine + FP64 escalation */
+
+/* One simple cutter: CylCutter(d, L), BallCutter(d, L), BullCutter(d, r, L),
+ * ConeCutter(d, angle, L) (cylcutter.cpp:39, ballcutter.cpp:38,
+ * bullcutter.cpp:38, conecutter.cpp:39). */
+typedef struct {
+    int32_t type;
+    double diameter;
+    double length;
+    double corner_radius; /* Bull */
+    double angle;         /* Cone half-angle, radians */
+} ocl_cutter_simple;
+
+/* Cutter descriptor.  For OCL_CUTTER_COMPOSITE the sub-cutters with their
+ * radius limit, height limit and z offset follow CompositeCutter::addCutter
+ * (compositecutter.cpp:43-53); `length` is the composite length. */
+typedef struct {
+    ocl_cutter_simple c;
+    int32_t nsub;
+    ocl_cutter_simple sub[4];
+    double radius_limit[4];
+    double height_limit[4];
+    double zoffset[4];
+} ocl_cutter_desc;
+
+typedef struct ocl_mesh ocl_mesh;
+
+/* Counters of one call (stats may be NULL). */
+typedef struct {
+    uint64_t points;        /* CL points (drop) or fibers (push) */
+    uint64_t pairs;         /* (point, triangle) pairs passing the overlap test */
+    uint64_t escalated;     /* points re-evaluated in exact FP64 (mixed mode) */
+    uint64_t brent_iters;   /* Bull edge Brent iterations (FP64) */
+    uint64_t intervals;     /* push: output intervals */
+    double kernel_ms;       /* device time of the compute kernels */
+} ocl_stats;
+
+/* Mesh: ntri triangles, 9 doubles each (p0, p1, p2).  bucket = the KD-tree
+ * bucket size of Operation::setBucketSize (default 1), which fixes the
+ * reference's triangle evaluation order (kdtree.hpp:141-210).
+ * Replaces BatchDropCutter::setSTL (batchdropcutter.cpp:54-63) and
+ * BatchPushCutter::setSTL (batchpushcutter.cpp:55-70). */
+int ocl_hip_mesh_create(const double* tri_xyz, size_t ntri, unsigned bucket, ocl_mesh** out);
+int ocl_hip_mesh_destroy(ocl_mesh* mesh);
+size_t ocl_hip_mesh_size(const ocl_mesh* mesh);
+
+/* Batch drop-cutter: replaces BatchDropCutter::run / dropCutter5
+ * (batchdropcutter.cpp:187-231).  cl_xyz: n x (x, y, z0).  Outputs (any may
+ * be NULL except z_out): lifted z, CC point xyz, CCType, index of the contact
+ * triangle (UINT32_MAX if none), contact feature code. */
+int ocl_hip_drop_batch(ocl_mesh* mesh, const ocl_cutter_desc* cutter, const double* cl_xyz,
+                       size_t n, int precision, double* z_out, double* cc_xyz_out,
+                       uint8_t* cc_type_out, uint32_t* tri_out, uint8_t* feat_out,
+                       ocl_stats* stats, void* stream);
+
+/* Same as ocl_hip_drop_batch with every array in device memory (asynchronous). */
+int ocl_hip_drop_device(ocl_mesh* mesh, const ocl_cutter_desc* cutter, const double* d_cl_xyz,
+                        size_t n, int precision, double* d_z_out, double* d_cc_xyz_out,
+                        uint8_t* d_cc_type_out, uint32_t* d_tri_out, uint8_t* d_feat_out,
+                        ocl_stats* stats, void* stream);
+
+/* Thread-local message of the last failure. */
+const char* ocl_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,158 @@
+"""ctypes binding of the C ABI in include/ocl_hip.h (libocl_hip.so).
+
+This is the thin layer the Python package, the tests and the bench use to
+reach the HIP kernels.  It fails loudly when the native library is missing:
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libocl_hip.so")
+
+CUTTER_TYPES = {"cyl": 0, "ball": 1, "bull": 2, "cone": 3, "composite": 4}
+PREC = {"exact64": 0, "fp64": 1, "fp32": 2, "mixed": 3}
+
+# CCType names (src/geo/ccpoint.hpp:33-39)
+CCTYPE_NAMES = ["NONE", "VERTEX", "VERTEX_CYL", "EDGE", "EDGE_HORIZ", "EDGE_SHAFT", "EDGE_HORIZ_CYL",
+                "EDGE_HORIZ_TOR", "EDGE_BALL", "EDGE_POS", "EDGE_NEG", "EDGE_CYL", "EDGE_CONE",
+                "EDGE_CONE_BASE", "FACET", "FACET_TIP", "FACET_CYL", "ERROR"]
+
+
+class CutterSimple(C.Structure):
+    _fields_ = [("type", C.c_int32), ("diameter", C.c_double), ("length", C.c_double),
+                ("corner_radius", C.c_double), ("angle", C.c_double)]
+
+
+class CutterDesc(C.Structure):
+    _fields_ = [("c", CutterSimple), ("nsub", C.c_int32), ("sub", CutterSimple * 4),
+                ("radius_limit", C.c_double * 4), ("height_limit", C.c_double * 4),
+                ("zoffset", C.c_double * 4)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("points", C.c_uint64), ("pairs", C.c_uint64), ("escalated", C.c_uint64),
+                ("brent_iters", C.c_uint64), ("intervals", C.c_uint64), ("kernel_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"native library missing: {LIB_PATH}; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, sz = C.c_void_p, C.POINTER(C.c_double), C.c_size_t
+        L.ocl_hip_mesh_create.argtypes = [dp, sz, C.c_uint, C.POINTER(C.c_void_p)]
+        L.ocl_hip_mesh_create.restype = C.c_int
+        L.ocl_hip_mesh_destroy.argtypes = [vp]
+        L.ocl_hip_mesh_destroy.restype = C.c_int
+        L.ocl_hip_mesh_size.argtypes = [vp]
+        L.ocl_hip_mesh_size.restype = sz
+        L.ocl_hip_drop_batch.argtypes = [vp, C.POINTER(CutterDesc), dp, sz, C.c_int, dp, dp,
+                                         C.POINTER(C.c_uint8), C.POINTER(C.c_uint32), C.POINTER(C.c_uint8),
+                                         C.POINTER(Stats), vp]
+        L.ocl_hip_drop_batch.restype = C.c_int
+        L.ocl_hip_drop_device.argtypes = [vp, C.POINTER(CutterDesc), vp, sz, C.c_int, vp, vp, vp, vp, vp,
+                                          C.POINTER(Stats), vp]
+        L.ocl_hip_drop_device.restype = C.c_int
+        L.ocl_hip_last_error.argtypes = []
+        L.ocl_hip_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+class OclError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != 0:
+        raise OclError(f"ocl_hip error {rc}: {lib().ocl_hip_last_error().decode()}")
+
+
+def _simple(spec) -> CutterSimple:
+    s = CutterSimple()
+    s.type = CUTTER_TYPES[spec["kind"]]
+    s.diameter = float(spec.get("d", 0.0))
+    s.length = float(spec.get("L", 0.0))
+    s.corner_radius = float(spec.get("r", 0.0))
+    s.angle = float(spec.get("angle", 0.0))
+    return s
+
+
+def cutter_desc(spec: dict) -> CutterDesc:
+    """Descriptor from a cutter spec dict: {"kind": "cyl"|"ball"|"bull"|"cone", "d",
+    "L", "r" (bull corner radius), "angle" (cone half-angle, rad)} or
+    {"kind": "composite", "L", "subs": [{"cutter": spec, "r", "h", "zoff"}, ...]}."""
+    d = CutterDesc()
+    if spec["kind"] == "composite":
+        d.c.type = CUTTER_TYPES["composite"]
+        d.c.length = float(spec["L"])
+        d.nsub = len(spec["subs"])
+        for i, s in enumerate(spec["subs"]):
+            d.sub[i] = _simple(s["cutter"])
+            d.radius_limit[i] = float(s["r"])
+            d.height_limit[i] = float(s["h"])
+            d.zoffset[i] = float(s["zoff"])
+    else:
+        d.c = _simple(spec)
+    return d
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class DeviceMesh:
+    """Owns an ocl_mesh handle (device copy + box hierarchy in KD order)."""
+
+    def __init__(self, tris, bucket: int = 1):
+        t = np.ascontiguousarray(np.asarray(tris, dtype=np.float64).reshape(-1, 9))
+        h = C.c_void_p()
+        check(lib().ocl_hip_mesh_create(_ptr(t, C.c_double), t.shape[0], bucket, C.byref(h)))
+        self.h = h
+        self.ntri = t.shape[0]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ocl_hip_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def drop(self, spec: dict, cl_xyz, precision="exact64", want_stats=True):
+        cl = np.ascontiguousarray(np.asarray(cl_xyz, dtype=np.float64).reshape(-1, 3))
+        n = cl.shape[0]
+        z = np.empty(n)
+        cc = np.empty((n, 3))
+        ty = np.empty(n, dtype=np.uint8)
+        tri = np.empty(n, dtype=np.uint32)
+        ft = np.empty(n, dtype=np.uint8)
+        st = Stats()
+        desc = cutter_desc(spec)
+        check(lib().ocl_hip_drop_batch(self.h, C.byref(desc), _ptr(cl, C.c_double), n, PREC[precision],
+                                       _ptr(z, C.c_double), _ptr(cc, C.c_double), _ptr(ty, C.c_uint8),
+                                       _ptr(tri, C.c_uint32), _ptr(ft, C.c_uint8),
+                                       C.byref(st) if want_stats else None, None))
+        return dict(z=z, cc=cc, type=ty, tri=tri, feat=ft, stats=st.as_dict())
+
+    def drop_device(self, spec: dict, d_cl: int, n: int, precision, d_z: int, d_cc=0, d_type=0, d_tri=0,
+                    d_feat=0, stream=None, stats=False):
+        """Asynchronous drop on device pointers (ints, e.g. torch data_ptr())."""
+        st = Stats()
+        desc = cutter_desc(spec)
+        check(lib().ocl_hip_drop_device(self.h, C.byref(desc), d_cl, n, PREC[precision], d_z,
+                                        d_cc or None, d_type or None, d_tri or None, d_feat or None,
+                                        C.byref(st) if stats else None, stream))
+        return st.as_dict() if stats else None

@@ -1,0 +1,124 @@
+/*
+ * ocl_oracle.h -- FP64 CPU restatement of OpenCAMLib's batch drop-cutter and
+ * batch push-cutter, used ONLY as test infrastructure: the parity checker of
+ * the HIP path and the timed CPU baseline.  Nothing in the product library
+ * links or calls this code.
+ *
+ * Every routine follows the reference operation order so that results match
+ * the reference's FP64 arithmetic (x86-64 SSE2, no FMA contraction).  The
+ * per-function comments cite the reference file:line restated.
+ *
+ * Pinning (what checks this restatement independently of its own formulas):
+ *   - closed forms: plane facet contacts, vertex contacts, Cyl/Ball edge
+ *     contacts against lines (tests/test_oracle_closed_form.py);
+ *   - brute force: drop z of every cutter against a triangle equals the
+ *     supremum of p.z - height(|p - cl|_xy) over densely sampled triangle
+ *     points, refined by golden-section search (tests/test_oracle_bruteforce.py);
+ *   - push vs drop duality: push interval endpoints equal the fiber
+ *     positions where the drop height crosses the fiber z;
+ *   - KD search vs exhaustive overlap scan (candidate sets equal);
+ *   - interval union vs an independent sort-and-merge union.
+ * Parity unpinned (no independent check reproduces these reference quirks):
+ *   - ConeCutter rim facet contact, which subtracts `length` instead of
+ *     `center_height` (conecutter.cpp:92), and the hyperbola edge branch
+ *     using square(length) (conecutter.cpp:122);
+ *   - CompositeCutter edge windows, which validate only the highest of the
+ *     three edge contacts per sub-cutter (compositecutter.cpp:175-194).
+ */
+#ifndef OCL_ORACLE_H
+#define OCL_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* CCType numbering of src/geo/ccpoint.hpp:33-39 */
+enum oc_cctype {
+    OC_NONE = 0, OC_VERTEX, OC_VERTEX_CYL, OC_EDGE, OC_EDGE_HORIZ, OC_EDGE_SHAFT,
+    OC_EDGE_HORIZ_CYL, OC_EDGE_HORIZ_TOR, OC_EDGE_BALL, OC_EDGE_POS, OC_EDGE_NEG,
+    OC_EDGE_CYL, OC_EDGE_CONE, OC_EDGE_CONE_BASE, OC_FACET, OC_FACET_TIP,
+    OC_FACET_CYL, OC_ERROR
+};
+
+enum oc_cutter_kind { OC_CYL = 0, OC_BALL = 1, OC_BULL = 2, OC_CONE = 3, OC_COMPOSITE = 4 };
+
+#define OC_MAX_SUB 4
+
+typedef struct oc_cutter {
+    int kind;
+    double diameter, radius, length;
+    double xy_normal_length, normal_length, center_height;
+    double radius1, radius2; /* bull */
+    double angle;            /* cone half-angle */
+    /* composite */
+    int nsub;
+    struct oc_cutter* sub[OC_MAX_SUB];
+    double radiusvec[OC_MAX_SUB], heightvec[OC_MAX_SUB], zoffset[OC_MAX_SUB];
+} oc_cutter;
+
+typedef struct oc_mesh oc_mesh;
+
+/* cutters (constructors of cylcutter.cpp:39-46, ballcutter.cpp:38-45,
+   bullcutter.cpp:38-47, conecutter.cpp:39-49) */
+void oc_cutter_cyl(oc_cutter* c, double d, double l);
+void oc_cutter_ball(oc_cutter* c, double d, double l);
+void oc_cutter_bull(oc_cutter* c, double d, double r, double l);
+void oc_cutter_cone(oc_cutter* c, double d, double angle, double l);
+void oc_cutter_composite(oc_cutter* c);
+/* CompositeCutter::addCutter (compositecutter.cpp:43-53); sub is borrowed */
+void oc_composite_add(oc_cutter* c, oc_cutter* sub, double r, double h, double zoff);
+
+/* mesh from 9 doubles per triangle; triangle normal/bbox as triangle.cpp:39-75 */
+oc_mesh* oc_mesh_create(const double* xyz9, long ntri);
+void oc_mesh_free(oc_mesh* m);
+long oc_mesh_size(const oc_mesh* m);
+
+/* KD-tree leaf order (rank) of kdtree.hpp:117-268.  dimset 0 = XY (drop),
+   1 = YZ (X-fibers), 2 = XZ (Y-fibers).  rank_out[tri] = position. */
+int oc_kd_rank(const oc_mesh* m, int dimset, unsigned bucket, int* rank_out);
+
+/* KD search (kdtree.hpp:123-135, 273-304) for a drop query; returns the number
+   of triangles written (in search order) or -needed if cap is too small. */
+long oc_kd_search_drop(oc_mesh* m, const oc_cutter* c, double x, double y, double z,
+                       unsigned bucket, int* out, long cap);
+
+/* Batch drop-cutter (batchdropcutter.cpp:187-231).
+   search_mode 0: reference KD search then overlaps/below filter;
+   search_mode 1: exhaustive overlap scan sorted by KD rank (same result);
+   search_mode 2: every triangle in input order, like dropCutter1 (no filter).
+   Outputs: z, cc xyz, cc type, index of the triangle and feature code of the
+   last lift (-1 if none).  ncalls = dropCutter() calls made. */
+int oc_batch_drop(oc_mesh* m, const oc_cutter* c, const double* cl_xyz, long n,
+                  int search_mode, int nthreads, unsigned bucket,
+                  double* z_out, double* cc_out, int* cctype_out,
+                  int* tri_out, int* feat_out, long* ncalls_out);
+
+/* single-triangle MillingCutter::dropCutter (millingcutter.cpp:386-409) on a
+   CL point starting at z_io; returns 1 if lifted. */
+int oc_drop_triangle(const oc_cutter* c, const double* tri9, double x, double y,
+                     double* z_io, double* cc_out, int* cctype_out, int* feat_out);
+
+/* Batch push-cutter (batchpushcutter.cpp:136-188).  dir 0 = X fibers,
+   1 = Y fibers.  fib = 6 doubles per fiber (p1, p2).
+   search_mode 0: reference KD superset (no filter);
+   search_mode 2: KD search then exact bbox overlap in the two search dims.
+   Output: offsets[nf+1]; for interval k: lohi[2k], lohi[2k+1];
+   cc[8k..8k+7] = lower cc xyz, type, upper cc xyz, type.
+   Returns total intervals, or -(needed) when cap is too small. */
+long oc_batch_push(oc_mesh* m, const oc_cutter* c, int dir, const double* fib, long nf,
+                   int search_mode, int nthreads, unsigned bucket,
+                   long cap, long* offsets, double* lohi, double* cc, long* ncalls_out);
+
+/* single (fiber, triangle) MillingCutter::pushCutter (millingcutter.cpp:377-382).
+   Returns 1 if the interval is non-empty. out = lower, upper, lower cc xyz,
+   type, upper cc xyz, type. */
+int oc_push_triangle(const oc_cutter* c, const double* fib6, const double* tri9, double* out10);
+
+/* Fiber::addInterval (fiber.cpp:58-93) applied to a list of per-pair
+   intervals in order (10 doubles each, as oc_push_triangle).  Returns count. */
+long oc_fiber_union(const double* ints10, long n, long cap, double* out10);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
