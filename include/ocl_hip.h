@@ -76,7 +76,8 @@ typedef struct ocl_mesh ocl_mesh;
 typedef struct {
     uint64_t points;        /* CL points (drop) or fibers (push) */
     uint64_t pairs;         /* (point, triangle) pairs passing the overlap test */
-    uint64_t escalated;     /* points re-evaluated in exact FP64 (mixed mode) */
+    uint64_t escalated;     /* points the first selection pass could not decide */
+    uint64_t escalated_exact; /* of those, points finished by the exact FP64 pass */
     uint64_t brent_iters;   /* Bull edge Brent iterations (FP64) */
     uint64_t intervals;     /* push: output intervals */
     double kernel_ms;       /* device time of the compute kernels */

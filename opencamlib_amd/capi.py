@@ -36,6 +36,7 @@ class CutterDesc(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("points", C.c_uint64), ("pairs", C.c_uint64), ("escalated", C.c_uint64),
+                ("escalated_exact", C.c_uint64),
                 ("brent_iters", C.c_uint64), ("intervals", C.c_uint64), ("kernel_ms", C.c_double)]
 
     def as_dict(self):

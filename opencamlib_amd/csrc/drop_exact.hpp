@@ -39,6 +39,14 @@ OCL_HD bool lane_lift(DropLane& L, double zin, V3 cc, int type) {  // clpoint.cp
     return false;
 }
 
+// vertex k of a triangle without dynamic indexing of a local array (which
+// would be placed in scratch memory on the GPU)
+OCL_HD double pick3(int k, double a, double b, double c) { return k == 0 ? a : (k == 1 ? b : c); }
+OCL_HD V3 tri_vtx(const TriX& t, int k) {
+    return v3(pick3(k, t.p[0].x, t.p[1].x, t.p[2].x), pick3(k, t.p[0].y, t.p[1].y, t.p[2].y),
+              pick3(k, t.p[0].z, t.p[1].z, t.p[2].z));
+}
+
 OCL_HD V3 up_normal(const TriX& t) { return (t.n.z < 0) ? vscale(-1.0, t.n) : t.n; }
 OCL_HD bool tri_contains(V3 p, const TriX& t) { return in_triangle_xy(p, t.p[0], t.p[1], t.p[2]); }
 
@@ -161,7 +169,7 @@ OCL_HD bool vertex_drop(const CutterDev& cd, DropLane& L, const TriX& t) {
     bool r = false;
     const int f0 = L.cur_feat;
     for (int k = 0; k < 3; ++k) {
-        const V3 p = t.p[k];
+        const V3 p = tri_vtx(t, k);
         const double q = vxynorm(vsub(v3(L.x, L.y, L.z), p));
         if (q <= cd.c.radius) {
             L.cur_feat = f0 + FT_VTX0 + k;
@@ -378,7 +386,7 @@ OCL_HD bool edge_simple(const CutterP& c, DropLane& L, const TriX& t, int& iters
     bool r = false;
     const int f0 = L.cur_feat;
     for (int n = 0; n < 3; n++) {
-        const V3 p1 = t.p[n], p2 = t.p[(n + 1) % 3];
+        const V3 p1 = tri_vtx(t, n), p2 = tri_vtx(t, n == 2 ? 0 : n + 1);
         if (!near0(p1.x - p2.x) || !near0(p1.y - p2.y)) {
             const double d = xy_line_distance(v3(L.x, L.y, L.z), p1, p2);
             if (d <= c.radius) {
@@ -408,15 +416,66 @@ OCL_HD bool edge_composite(const CutterDev& cd, DropLane& L, const TriX& t, int&
     return r;
 }
 
+// Exact FP64 contact of ONE feature of a triangle (a selection-pass winner),
+// computed as the reference computes that feature inside dropCutter.  The
+// lane must start below every candidate (z = -inf).  Not for composites.
+OCL_HD void exact_feature(const CutterP& c, DropLane& L, const TriX& t, int feat, int& iters) {
+    const int kf = feat & 15;
+    L.cur_feat = feat;
+    iters = 0;
+    if (kf == FT_FACET || kf == FT_FACET_RIM) {
+        if (c.kind != K_CONE) {
+            facet_generic(c, L, t);
+            return;
+        }
+        // conecutter.cpp:70-105, one of the two contacts
+        V3 n = up_normal(t);
+        if (near0(n.z)) return;
+        if (near0(n.x) && near0(n.y)) {
+            if (kf != FT_FACET) return;
+            const V3 cc = v3(L.x, L.y, t.p[0].z);
+            if (tri_contains(cc, t)) lane_lift(L, cc.z, cc, CC_FACET_TIP);
+            return;
+        }
+        const double a = n.x, b = n.y, cz = n.z;
+        const double d = -vdot(n, t.p[0]);
+        n = vxyunit(n);
+        if (kf == FT_FACET) {
+            V3 tip = v3(L.x, L.y, 0.0);
+            tip.z = (1.0 / cz) * (-d - a * tip.x - b * tip.y);
+            if (tri_contains(tip, t)) lane_lift(L, tip.z, tip, CC_FACET_TIP);
+        } else {
+            V3 rim = vsub(v3(L.x, L.y, L.z), vscale(c.radius, n));
+            rim.z = (1.0 / cz) * (-d - a * rim.x - b * rim.y);
+            if (tri_contains(rim, t)) lane_lift(L, rim.z - c.length, rim, CC_FACET_CYL);
+        }
+        return;
+    }
+    if (kf < FT_EDGE0) {
+        const V3 p = tri_vtx(t, kf - FT_VTX0);
+        const double q = vxynorm(vsub(v3(L.x, L.y, L.z), p));
+        if (q <= c.radius) lane_lift(L, p.z - prof_height(c, q), p, CC_VERTEX);
+        return;
+    }
+    const int n = kf - FT_EDGE0;
+    const V3 p1 = tri_vtx(t, n), p2 = tri_vtx(t, n == 2 ? 0 : n + 1);
+    if (!near0(p1.x - p2.x) || !near0(p1.y - p2.y)) {
+        const double d = xy_line_distance(v3(L.x, L.y, L.z), p1, p2);
+        if (d <= c.radius) single_edge_drop(c, L, p1, p2, d, iters);
+    }
+}
+
 // millingcutter.cpp:422-433 (XY box of the cutter vs triangle bbox)
 OCL_HD bool cutter_overlaps(double radius, double x, double y, const TriX& t) {
     double mnx = t.p[0].x, mxx = t.p[0].x, mny = t.p[0].y, mxy = t.p[0].y;
-    for (int k = 1; k < 3; ++k) {
-        if (t.p[k].x > mxx) mxx = t.p[k].x;
-        if (t.p[k].x < mnx) mnx = t.p[k].x;
-        if (t.p[k].y > mxy) mxy = t.p[k].y;
-        if (t.p[k].y < mny) mny = t.p[k].y;
-    }
+    if (t.p[1].x > mxx) mxx = t.p[1].x;
+    if (t.p[1].x < mnx) mnx = t.p[1].x;
+    if (t.p[1].y > mxy) mxy = t.p[1].y;
+    if (t.p[1].y < mny) mny = t.p[1].y;
+    if (t.p[2].x > mxx) mxx = t.p[2].x;
+    if (t.p[2].x < mnx) mnx = t.p[2].x;
+    if (t.p[2].y > mxy) mxy = t.p[2].y;
+    if (t.p[2].y < mny) mny = t.p[2].y;
     if (mxx < x - radius) return false;
     if (mnx > x + radius) return false;
     if (mxy < y - radius) return false;

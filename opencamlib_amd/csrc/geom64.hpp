@@ -7,6 +7,7 @@
 #pragma once
 
 #include <cmath>
+#include <cstdint>
 
 #if defined(__HIPCC__)
 #define OCL_HD __host__ __device__ inline
@@ -15,6 +16,12 @@
 #endif
 
 namespace ocl_amd {
+
+#if !defined(__HIPCC__)
+using std::signbit;
+using std::sqrt;
+using std::fabs;
+#endif
 
 struct V3 {
     double x, y, z;

@@ -24,9 +24,9 @@ constexpr int kFan = 16;        // children per box above level 0
 constexpr int kMaxLevels = 8;   // 16^8 triangles is far beyond HBM
 
 struct BoxTree {
-    const float4* box[kMaxLevels];  // (minx, miny, maxx, maxy), outward rounded
-    uint32_t cnt[kMaxLevels];
-    int top;                        // index of the highest level (cnt[top] <= kFan)
+    const float4* box;     // all levels, (minx, miny, maxx, maxy), outward rounded
+    const uint32_t* lvl;   // lvl[2l] = box count of level l, lvl[2l+1] = its offset in box
+    int top;               // index of the highest level (count <= kFan)
 };
 
 struct MeshView {

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -13,6 +14,7 @@
 #include "../../include/ocl_hip.h"
 #include "cutter.hpp"
 #include "drop_exact.hpp"
+#include "drop_fast.hpp"
 #include "kd_rank.hpp"
 #include "mesh.hpp"
 #include "walk.hpp"
@@ -63,12 +65,19 @@ struct ocl_mesh {
     ocl_amd::DevBuf tri;     // kRec doubles per triangle, rank order
     ocl_amd::DevBuf order;   // rank -> input index
     ocl_amd::DevBuf boxes;   // all hierarchy levels
+    ocl_amd::DevBuf levels;  // (count, offset) per level
     uint32_t cnt[ocl_amd::kMaxLevels] = {};
     size_t box_off[ocl_amd::kMaxLevels] = {};
     int top = 0;
     // per-call workspace
     ocl_amd::DevBuf keys, perm, hist, cursor, cl, out_z, out_cc, out_ty, out_tri, out_ft, counters, fastrec;
-    ocl_amd::DevBuf esc_list;
+    ocl_amd::DevBuf esc_list, esc_list2, esc_flag, blocksum, qcount, cnt2, cnt3;
+    ocl_amd::DevBuf origins;     // double2 box centre per level-0 box
+    double zmid = 0, zhalf = 0, xyb = 0;
+    // cached selection records (rebuilt when the cutter or precision changes)
+    ocl_amd::DevBuf rec32, rec64;
+    bool rec32_ok = false, rec64_ok = false;
+    ocl_amd::CutterDev rec32_cd{}, rec64_cd{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     ocl_amd::MeshView view() const {
@@ -76,10 +85,8 @@ struct ocl_mesh {
         v.tri = tri.as<const double>();
         v.order = order.as<const uint32_t>();
         v.ntri = (uint32_t)ntri;
-        for (int l = 0; l < ocl_amd::kMaxLevels; ++l) {
-            v.tree.box[l] = boxes.as<const float4>() ? boxes.as<const float4>() + box_off[l] : nullptr;
-            v.tree.cnt[l] = cnt[l];
-        }
+        v.tree.box = boxes.as<const float4>();
+        v.tree.lvl = levels.as<const uint32_t>();
         v.tree.top = top;
         return v;
     }
@@ -137,6 +144,14 @@ __global__ void k_box_up(const float4* __restrict__ in, uint32_t nin, float4* __
         r.w = fmaxf(r.w, c.w);
     }
     out[b] = r;
+}
+
+// Level-0 box centres (local frame origins of the selection records).
+__global__ void k_box_origin(const float4* __restrict__ box, uint32_t nbox, double2* __restrict__ org) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    const float4 q = box[b];
+    org[b] = make_double2(0.5 * ((double)q.x + (double)q.z), 0.5 * ((double)q.y + (double)q.w));
 }
 
 // ------------------------------------------------------------ CL binning
@@ -255,12 +270,14 @@ __device__ inline DropLane fresh_lane(double x, double y, double z) {
 // Exact FP64 drop over a list of points (identity list or escalation list).
 // One lane per point; the wavefront walks the hierarchy with the union box.
 __global__ void __launch_bounds__(256) k_drop_exact(MeshView m, CutterDev cd, const double* __restrict__ cl,
-                                                    const uint32_t* __restrict__ list, uint32_t n, DropOut o,
+                                                    const uint32_t* __restrict__ list,
+                                                    const uint32_t* __restrict__ count, uint32_t n, DropOut o,
                                                     Counters* cnt) {
+    const uint32_t n_eff = count ? *count : n;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t wave_first = gid & ~63u;
-    if (wave_first >= n) return;
-    const bool act = gid < n;
+    if (wave_first >= n_eff) return;
+    const bool act = gid < n_eff;
     const uint32_t src = act ? list[gid] : 0;
     const double x = act ? cl[3 * (size_t)src] : 0.0;
     const double y = act ? cl[3 * (size_t)src + 1] : 0.0;
@@ -293,6 +310,468 @@ __global__ void __launch_bounds__(256) k_drop_exact(MeshView m, CutterDev cd, co
     }
 }
 
+
+// ----------------------------------------------------- selection kernels
+// Per-triangle selection records for one cutter, in T, box-local frames.
+template <class T>
+__global__ void k_sel_prep(const double* __restrict__ tri, uint32_t ntri, const BoxOrigin* __restrict__ org,
+                           CutterDev cd, SelErr er, T* __restrict__ rec) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntri) return;
+    const TriX t = load_tri(tri, i);
+    const BoxOrigin o = org[i / kLeaf];
+    build_record<T>(t, cd, o.ox, o.oy, o.oz, er, rec + (size_t)kFastRec * i);
+}
+
+// Box origins: xy centre of the level-0 box, mid z of its triangles.
+__global__ void k_box_frames(const double* __restrict__ tri, uint32_t ntri, const float4* __restrict__ box,
+                             uint32_t nbox, BoxOrigin* __restrict__ org) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    const float4 q = box[b];
+    double z0 = INFINITY, z1 = -INFINITY;
+    const uint32_t e = min(ntri, (b + 1) * kLeaf);
+    for (uint32_t i = b * kLeaf; i < e; ++i) {
+        const double* r = tri + (size_t)kRec * i;
+        for (int k = 0; k < 3; ++k) {
+            z0 = fmin(z0, r[3 * k + 2]);
+            z1 = fmax(z1, r[3 * k + 2]);
+        }
+    }
+    BoxOrigin o;
+    o.ox = 0.5 * ((double)q.x + (double)q.z);
+    o.oy = 0.5 * ((double)q.y + (double)q.w);
+    o.oz = 0.5 * (z0 + z1);
+    o.pad = 0.0;
+    org[b] = o;
+}
+
+template <class T>
+struct WaveState {
+    T zb, zt[4], za;
+};
+
+// Selection over a list of points.  Writes final results for lanes that
+// need no escalation and flags the others (esc[pos] = 1).  refine = 0 is the
+// FP32-only mode: the selection result is written as is.
+template <class T, int K>
+__global__ void __launch_bounds__(256) k_drop_select(MeshView m, const T* __restrict__ rec,
+                                                     const BoxOrigin* __restrict__ org, SelParams<T> P, CutterDev cd,
+                                                     const double* __restrict__ cl, const uint32_t* __restrict__ list,
+                                                     const uint32_t* __restrict__ count, uint32_t n, DropOut o,
+                                                     uint32_t* __restrict__ next, uint32_t* __restrict__ next_n,
+                                                     int refine, Counters* cnt) {
+    typedef FM<T> F;
+    const uint32_t n_eff = count ? *count : n;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t wave_first = gid & ~63u;
+    if (wave_first >= n_eff) return;
+    const bool act = gid < n_eff;
+    const uint32_t src = act ? list[gid] : 0;
+    const double x = act ? cl[3 * (size_t)src] : 0.0;
+    const double y = act ? cl[3 * (size_t)src + 1] : 0.0;
+    const double z0 = act ? cl[3 * (size_t)src + 2] : 0.0;
+    const double R = cd.c.radius;
+    const float qx0 = uniform_f(__double2float_rd(wave_min(act ? x : INFINITY) - R));
+    const float qy0 = uniform_f(__double2float_rd(wave_min(act ? y : INFINITY) - R));
+    const float qx1 = uniform_f(__double2float_ru(wave_max(act ? x : -INFINITY) + R));
+    const float qy1 = uniform_f(__double2float_ru(wave_max(act ? y : -INFINITY) + R));
+    Sel<T> S;
+    WaveState<T> W;
+    double ozw = 0.0;
+    bool framed = false;
+    sel_init(S, (T)0);
+    W.zb = (T)0;
+    for (int i = 0; i < 4; ++i) W.zt[i] = -F::inf();
+    W.za = -F::inf();
+    walk_boxes(m.tree, qx0, qy0, qx1, qy1, [&](uint32_t b) {
+        const BoxOrigin bo = org[b];
+        if (!framed) {
+            framed = true;
+            ozw = bo.oz;
+            W.zb = (T)(z0 - ozw);
+        }
+        const T dz = (T)(bo.oz - ozw);
+        const T xl = (T)(x - bo.ox), yl = (T)(y - bo.oy);
+        const T lx0 = (T)((double)qx0 - bo.ox), lx1 = (T)((double)qx1 - bo.ox);
+        const T ly0 = (T)((double)qy0 - bo.oy), ly1 = (T)((double)qy1 - bo.oy);
+        // enter the box frame
+        const uint32_t bt_in = S.bt;
+        S.zb = W.zb - dz;
+        for (int i = 0; i < 4; ++i) S.zt[i] = -F::inf();
+        S.za = -F::inf();
+        const uint32_t e = min(m.ntri, (b + 1) * kLeaf);
+        for (uint32_t k = b * kLeaf; k < e; ++k) {
+            const uint32_t ku = uniform_u(k);
+            const T* r = rec + (size_t)kFastRec * ku;
+            // one vector of the triangle box, tested without short-circuits
+            const T bx0 = r[R_BX0], bx1 = r[R_BX1], by0 = r[R_BY0], by1 = r[R_BY1];
+            const bool out = (bx1 < lx0) | (bx0 > lx1) | (by1 < ly0) | (by0 > ly1);
+            if (out) continue;
+            const bool on = act & !((bx1 < xl - P.R) | (bx0 > xl + P.R) | (by1 < yl - P.R) | (by0 > yl + P.R));
+            if (!__any(on)) continue;
+            sel_triangle<T, K>(r, ku, xl, yl, P, S, on);
+        }
+        // leave the box frame
+        if (S.bt != bt_in) W.zb = S.zb + dz;
+        for (int i = 0; i < 4; ++i) W.zt[i] = F::max_(W.zt[i], S.zt[i] + dz);
+        W.za = F::max_(W.za, S.za + dz);
+    });
+    // decision in the wave frame
+    Sel<T> D = S;
+    D.zb = W.zb;
+    for (int i = 0; i < 4; ++i) D.zt[i] = W.zt[i];
+    D.za = W.za;
+    const T z0w = (T)(z0 - ozw);
+    bool up = refine && framed && sel_escalate(D, z0w, P);
+    DropLane L = fresh_lane(x, y, z0);
+    unsigned long long brent = 0;
+    if (act && !up && D.bt != 0xffffffffu) {
+        if (refine) {
+            const TriX t = load_tri(m.tri, D.bt);
+            DropLane X = fresh_lane(x, y, -INFINITY);
+            X.cur_tri = D.bt;
+            int it = 0;
+            exact_feature(cd.c, X, t, D.bf, it);
+            brent += it;
+            const double zsel = ozw + (double)D.zb;
+            if (X.type == CC_NONE || feat_slot(X.feat) != feat_slot(D.bf) || !(X.z > z0) ||
+                fabs(X.z - zsel) > 4.0 * ((double)D.eb + (double)P.E)) {
+                up = true;
+            } else {
+                L.z = X.z;
+                L.cc = X.cc;
+                L.type = X.type;
+                L.tri = D.bt;
+                L.feat = X.feat;
+            }
+        } else {
+            L.z = ozw + (double)D.zb;
+            L.cc = v3(NAN, NAN, NAN);
+            const int sl = feat_slot(D.bf);
+            L.type = (sl == 0) ? CC_VERTEX : (sl == 1) ? CC_EDGE : (K == K_CONE) ? ((sl == 2) ? CC_FACET_TIP : CC_FACET_CYL) : CC_FACET;
+            L.tri = D.bt;
+            L.feat = D.bf;
+        }
+    }
+    if (!refine) up = false;
+    const bool push = act && up;
+    const unsigned long long bal = __ballot(push);
+    if (bal) {
+        uint32_t base = 0;
+        if ((threadIdx.x & 63) == (uint32_t)(__ffsll((long long)bal) - 1)) base = atomicAdd(next_n, (uint32_t)__popcll(bal));
+        base = __shfl(base, __ffsll((long long)bal) - 1, 64);
+        if (push) next[base + __popcll(bal & ((1ull << (threadIdx.x & 63)) - 1ull))] = src;
+    }
+    if (act && !up) write_drop(o, m.order, src, L);
+    if (cnt) {
+        wave_add(&cnt->pairs, act ? S.pairs : 0);
+        wave_add(&cnt->escalated, (act && up) ? 1 : 0);
+        wave_add(&cnt->brent, brent);
+    }
+}
+
+
+// ------------------------------------------------- cooperative tiers
+// One wavefront per point (points of an escalation queue are scattered, so
+// a lane-per-point wave would walk a huge union box).  The wave walks the
+// hierarchy with the point's own cutter box, gathers the overlapping
+// triangles in rank order into batches of 64 (one per lane), evaluates the
+// batch in parallel and folds it into the point's state in rank order.
+constexpr int kCoopWaves = 4;  // wavefronts per 256-thread block
+
+__device__ inline uint32_t lane_id() { return threadIdx.x & 63u; }
+
+// Gather driver: calls batch(list, cnt) with cnt <= 64 candidate triangle
+// indices in rank order.  Candidates are triangles whose XY box overlaps
+// [x-R, x+R] x [y-R, y+R] (the reference overlaps() test, exact FP64).
+template <class Batch>
+__device__ inline void coop_gather(const MeshView& m, uint32_t* qbuf, double x, double y, double R, Batch&& batch) {
+    const uint32_t lane = lane_id();
+    const float qx0 = __double2float_rd(x - R), qy0 = __double2float_rd(y - R);
+    const float qx1 = __double2float_ru(x + R), qy1 = __double2float_ru(y + R);
+    uint32_t qn = 0;
+    walk_boxes(m.tree, qx0, qy0, qx1, qy1, [&](uint32_t b) {
+        const uint32_t k = b * kLeaf + lane;
+        bool cand = false;
+        if (lane < (uint32_t)kLeaf && k < m.ntri) {
+            const TriX t = load_tri(m.tri, k);
+            cand = cutter_overlaps(R, x, y, t);
+        }
+        const unsigned long long bal = __ballot(cand);
+        const uint32_t pos = qn + __popcll(bal & ((1ull << lane) - 1ull));
+        if (cand) qbuf[pos] = k;
+        qn += __popcll(bal);
+        __builtin_amdgcn_wave_barrier();
+        if (qn >= 64) {
+            batch(qbuf, 64u);
+            const uint32_t rest = qn - 64;
+            const uint32_t moved = (lane < rest) ? qbuf[64 + lane] : 0u;
+            __builtin_amdgcn_wave_barrier();
+            if (lane < rest) qbuf[lane] = moved;
+            __builtin_amdgcn_wave_barrier();
+            qn = rest;
+        }
+    });
+    if (qn > 0) batch(qbuf, qn);
+}
+
+// Exact FP64 tier: reproduces the reference's sequential dropCutter fold.
+// Per triangle the lane computes, from z = -inf, the facet (cone: tip)
+// contact, the cone rim contact and the best vertex/edge contact; the fold
+// then applies them in rank order exactly as MillingCutter::dropCutter does
+// against the running z (a lifting facet skips the triangle's vertices and
+// edges; the cone rim is tried only when the tip did not lift).
+__global__ void __launch_bounds__(256) k_drop_coop_exact(MeshView m, CutterDev cd, const double* __restrict__ cl,
+                                                         const uint32_t* __restrict__ list,
+                                                         const uint32_t* __restrict__ count, DropOut o,
+                                                         Counters* cnt) {
+    __shared__ uint32_t qmem[kCoopWaves][80];
+    const uint32_t lane = lane_id();
+    const uint32_t wv = threadIdx.x >> 6;
+    uint32_t* qbuf = qmem[wv];
+    const uint32_t n_eff = *count;
+    const uint32_t nwaves = gridDim.x * kCoopWaves;
+    unsigned long long brent = 0;
+    for (uint32_t w = blockIdx.x * kCoopWaves + wv; w < n_eff; w += nwaves) {
+        const uint32_t src = uniform_u(list[w]);
+        const double x = cl[3 * (size_t)src], y = cl[3 * (size_t)src + 1], z0 = cl[3 * (size_t)src + 2];
+        const CutterP& c = cd.c;
+        double zr = z0;
+        uint32_t win_k = 0xffffffffu;
+        int win_part = -1;
+        V3 win_cc = v3(0, 0, 0);
+        int win_type = CC_NONE, win_feat = FT_NONE;
+        coop_gather(m, qbuf, x, y, c.radius, [&](uint32_t* q, uint32_t nb) {
+            const bool has = lane < nb;
+            const uint32_t k = has ? q[lane] : 0u;
+            DropLane A = fresh_lane(x, y, -INFINITY), B = fresh_lane(x, y, -INFINITY), V = fresh_lane(x, y, -INFINITY);
+            double maxz = -INFINITY;
+            if (has) {
+                const TriX t = load_tri(m.tri, k);
+                maxz = t.maxz;
+                A.cur_tri = B.cur_tri = V.cur_tri = k;
+                int it = 0;
+                exact_feature(c, A, t, FT_FACET, it);
+                if (c.kind == K_CONE) exact_feature(c, B, t, FT_FACET_RIM, it);
+                V.cur_feat = 0;
+                vertex_drop(cd, V, t);
+                if (V.z < t.maxz) {
+                    int it2 = 0;
+                    edge_simple(c, V, t, it2);
+                    brent += it2;
+                }
+            }
+            // rank-ordered fold (uniform across the wave)
+            for (uint32_t i = 0; i < nb; ++i) {
+                const double zmax = __shfl(maxz, i, 64);
+                if (!(zr < zmax)) continue;  // below()
+                const double z1 = __shfl(A.z, i, 64), z2 = __shfl(B.z, i, 64), z3 = __shfl(V.z, i, 64);
+                const bool v1 = __shfl(A.type, i, 64) != CC_NONE, v2 = __shfl(B.type, i, 64) != CC_NONE;
+                const bool v3_ = __shfl(V.type, i, 64) != CC_NONE;
+                // millingcutter.cpp:386-409 / conecutter.cpp:70-105
+                int part = -1;
+                if (v1 && z1 > zr) part = 0;
+                else if (v2 && z2 > zr) part = 1;
+                else if (v3_ && z3 > zr) part = 2;
+                if (part >= 0) {
+                    zr = (part == 0) ? z1 : (part == 1) ? z2 : z3;
+                    win_k = __shfl(k, i, 64);
+                    win_part = part;
+                    // select by value (a reference select would spill the lanes to scratch)
+                    const double cx = (part == 0) ? A.cc.x : (part == 1) ? B.cc.x : V.cc.x;
+                    const double cy = (part == 0) ? A.cc.y : (part == 1) ? B.cc.y : V.cc.y;
+                    const double cz = (part == 0) ? A.cc.z : (part == 1) ? B.cc.z : V.cc.z;
+                    const int ty = (part == 0) ? A.type : (part == 1) ? B.type : V.type;
+                    const int ft = (part == 0) ? A.feat : (part == 1) ? B.feat : V.feat;
+                    win_cc = v3(__shfl(cx, i, 64), __shfl(cy, i, 64), __shfl(cz, i, 64));
+                    win_type = __shfl(ty, i, 64);
+                    win_feat = __shfl(ft, i, 64);
+                }
+            }
+        });
+        if (lane == 0) {
+            DropLane L = fresh_lane(x, y, z0);
+            if (win_part >= 0) {
+                L.z = zr;
+                L.cc = win_cc;
+                L.type = win_type;
+                L.tri = win_k;
+                L.feat = win_feat;
+            }
+            write_drop(o, m.order, src, L);
+        }
+    }
+    if (cnt) wave_add(&cnt->brent, brent);
+}
+
+// FP64 selection tier for Cyl/Ball/Bull: per-triangle selection summaries
+// (from z = -inf) reduced in rank order; undecided points go to the exact
+// tier queue, decided ones are refined by the exact path.
+template <int K>
+__global__ void __launch_bounds__(256) k_drop_coop_select(MeshView m, const double* __restrict__ rec,
+                                                          const BoxOrigin* __restrict__ org, SelParams<double> P,
+                                                          CutterDev cd, const double* __restrict__ cl,
+                                                          const uint32_t* __restrict__ list,
+                                                          const uint32_t* __restrict__ count, DropOut o,
+                                                          uint32_t* __restrict__ next, uint32_t* __restrict__ next_n,
+                                                          Counters* cnt) {
+    __shared__ uint32_t qmem[kCoopWaves][80];
+    const uint32_t lane = lane_id();
+    const uint32_t wv = threadIdx.x >> 6;
+    uint32_t* qbuf = qmem[wv];
+    const uint32_t n_eff = *count;
+    const uint32_t nwaves = gridDim.x * kCoopWaves;
+    unsigned long long brent = 0, nesc = 0;
+    for (uint32_t w = blockIdx.x * kCoopWaves + wv; w < n_eff; w += nwaves) {
+        const uint32_t src = uniform_u(list[w]);
+        const double x = cl[3 * (size_t)src], y = cl[3 * (size_t)src + 1], z0 = cl[3 * (size_t)src + 2];
+        // wave state in the global z frame
+        double zb = z0, eb = 0.0, zt[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY}, za = -INFINITY;
+        uint32_t bt = 0xffffffffu;
+        int bf = FT_NONE;
+        bool bamb = false;
+        coop_gather(m, qbuf, x, y, cd.c.radius, [&](uint32_t* q, uint32_t nb) {
+            const bool has = lane < nb;
+            const uint32_t k = has ? q[lane] : 0u;
+            Sel<double> S;
+            sel_init<double>(S, -INFINITY);
+            double oz = 0.0;
+            if (has) {
+                const BoxOrigin bo = org[k / kLeaf];
+                oz = bo.oz;
+                sel_triangle<double, K>(rec + (size_t)kFastRec * k, k, x - bo.ox, y - bo.oy, P, S, true);
+            }
+            const bool lifted = has && S.bt != 0xffffffffu;
+            const double zl = lifted ? S.zb + oz : -INFINITY;
+            for (int i = 0; i < 4; ++i) zt[i] = fmax(zt[i], wave_max(has ? S.zt[i] + oz : -INFINITY));
+            za = fmax(za, wave_max(has ? S.za + oz : -INFINITY));
+            // first lane (lowest rank) holding the batch maximum
+            const double zmaxb = wave_max(zl);
+            const unsigned long long hit = __ballot(lifted && zl == zmaxb);
+            if (hit && zmaxb > zb) {
+                const uint32_t i = (uint32_t)__ffsll((long long)hit) - 1u;
+                zb = zmaxb;
+                eb = __shfl(S.eb, i, 64);
+                bt = __shfl(k, i, 64);
+                bf = __shfl(S.bf, i, 64);
+                bamb = __shfl((int)S.bamb, i, 64) != 0;
+            }
+        });
+        Sel<double> D;
+        sel_init(D, zb);
+        D.eb = eb;
+        D.bt = bt;
+        D.bf = bf;
+        D.bamb = bamb;
+        for (int i = 0; i < 4; ++i) D.zt[i] = zt[i];
+        D.za = za;
+        bool up = sel_escalate(D, z0, P);
+        DropLane L = fresh_lane(x, y, z0);
+        if (!up && bt != 0xffffffffu) {
+            // exact refine of the selected feature (lane 0's result is used)
+            const TriX t = load_tri(m.tri, bt);
+            DropLane X = fresh_lane(x, y, -INFINITY);
+            X.cur_tri = bt;
+            int it = 0;
+            exact_feature(cd.c, X, t, bf, it);
+            brent += (lane == 0) ? it : 0;
+            if (X.type == CC_NONE || feat_slot(X.feat) != feat_slot(bf) || !(X.z > z0) ||
+                fabs(X.z - zb) > 4.0 * (eb + (double)P.E))
+                up = true;
+            else
+                L = X;
+        }
+        if (up) {
+            if (lane == 0) next[atomicAdd(next_n, 1u)] = src;
+            nesc += (lane == 0);
+        } else if (lane == 0) {
+            write_drop(o, m.order, src, L);
+        }
+    }
+    if (cnt) {
+        wave_add(&cnt->brent, brent);
+        wave_add(&cnt->escalated, nesc);
+    }
+}
+
+// ------------------------------------------------------ stable compaction
+// list_out = [list_in[i] for i < n_eff if flag[i]] in order; *count_out = size.
+constexpr int kCompactItems = 4096;  // per block of 256 threads
+
+__global__ void __launch_bounds__(256) k_flag_count(const uint8_t* __restrict__ flag, const uint32_t* __restrict__ count,
+                                                    uint32_t n, uint32_t* __restrict__ blocksum) {
+    const uint32_t n_eff = count ? *count : n;
+    __shared__ uint32_t red[256];
+    const uint32_t base = blockIdx.x * kCompactItems;
+    uint32_t c = 0;
+    for (int j = 0; j < kCompactItems / 256; ++j) {
+        const uint32_t i = base + j * 256 + threadIdx.x;
+        if (i < n_eff) c += flag[i];
+    }
+    red[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) blocksum[blockIdx.x] = red[0];
+}
+
+// exclusive scan of nb block sums (one block); total -> *count_out
+__global__ void __launch_bounds__(1024) k_scan_blocks(uint32_t* __restrict__ blocksum, uint32_t nb,
+                                                      uint32_t* __restrict__ count_out) {
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nb + 1023) / 1024;
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < per; ++k) {
+        const uint32_t i = t * per + k;
+        if (i < nb) s += blocksum[i];
+    }
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024; o <<= 1) {
+        const uint32_t v = (t >= o) ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - s;
+    for (uint32_t k = 0; k < per; ++k) {
+        const uint32_t i = t * per + k;
+        if (i < nb) {
+            const uint32_t v = blocksum[i];
+            blocksum[i] = run;
+            run += v;
+        }
+    }
+    if (t == 1023) *count_out = part[1023];
+}
+
+__global__ void __launch_bounds__(256) k_compact(const uint8_t* __restrict__ flag, const uint32_t* __restrict__ count,
+                                                 uint32_t n, const uint32_t* __restrict__ list_in,
+                                                 const uint32_t* __restrict__ blockoff, uint32_t* __restrict__ list_out) {
+    const uint32_t n_eff = count ? *count : n;
+    __shared__ uint32_t wsum[4];
+    uint32_t run = blockoff[blockIdx.x];
+    const uint32_t base = blockIdx.x * kCompactItems;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int j = 0; j < kCompactItems / 256; ++j) {
+        const uint32_t i = base + j * 256 + threadIdx.x;
+        const bool f = (i < n_eff) && flag[i];
+        const unsigned long long bal = __ballot(f);
+        const uint32_t below = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wv] = __popcll(bal);
+        __syncthreads();
+        uint32_t off = run;
+        for (int w = 0; w < wv; ++w) off += wsum[w];
+        if (f) list_out[off + below] = list_in[i];
+        run += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+    }
+}
+
 // Empty mesh: every CL point keeps its z and has no contact.
 __global__ void k_drop_none(const double* __restrict__ cl, uint32_t n, DropOut o) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -304,10 +783,106 @@ __global__ void k_drop_none(const double* __restrict__ cl, uint32_t n, DropOut o
 
 using namespace ocl_amd;
 
-static int launch_drop_fast(ocl_mesh* m, const MeshView& mv, const CutterDev& cd, const double* d_cl, uint32_t n,
-                            int precision, const DropOut& o, Counters* cnt, hipStream_t st);
+// ---------------------------------------------------------- fast launcher
+template <class T>
+static SelErr make_err(const CutterDev& cd, double scale_e) {
+    SelErr er;
+    er.u = FM<T>::eps() * scale_e;
+    er.R = cd.c.radius;
+    er.zmag = 4.0 * cd.c.radius + 10.0;
+    return er;
+}
 
-// ------------------------------------------------------------- C ABI impl
+template <class T>
+static SelParams<T> make_params(const CutterDev& cd, const SelErr& er) {
+    return sel_params<T>(cd, er);
+}
+
+template <class T>
+static int ensure_records(ocl_mesh* m, const CutterDev& cd, DevBuf& buf, bool& ok, CutterDev& key, const SelErr& er,
+                          hipStream_t st) {
+    if (ok && std::memcmp(&key, &cd, sizeof(cd)) == 0) return OCL_OK;
+    int rc = buf.ensure(sizeof(T) * kFastRec * m->ntri);
+    if (rc) return rc;
+    const uint32_t n = (uint32_t)m->ntri;
+    k_sel_prep<T><<<(n + 255) / 256, 256, 0, st>>>(m->tri.as<double>(), n, m->origins.as<BoxOrigin>(), cd, er,
+                                                   buf.as<T>());
+    HIP_TRY(hipGetLastError());
+    key = cd;
+    ok = true;
+    return OCL_OK;
+}
+
+template <class T>
+static int launch_select(ocl_mesh* m, const MeshView& mv, const T* rec, const SelParams<T>& P, const CutterDev& cd,
+                         const double* d_cl, const uint32_t* list, uint32_t n, const DropOut& o, uint32_t* next,
+                         uint32_t* next_n, int refine, Counters* cnt, hipStream_t st) {
+    const uint32_t nb = (n + 255) / 256;
+    const BoxOrigin* org = m->origins.as<BoxOrigin>();
+    switch (cd.c.kind) {
+    case K_CYL: k_drop_select<T, K_CYL><<<nb, 256, 0, st>>>(mv, rec, org, P, cd, d_cl, list, nullptr, n, o, next, next_n, refine, cnt); break;
+    case K_BALL: k_drop_select<T, K_BALL><<<nb, 256, 0, st>>>(mv, rec, org, P, cd, d_cl, list, nullptr, n, o, next, next_n, refine, cnt); break;
+    case K_BULL: k_drop_select<T, K_BULL><<<nb, 256, 0, st>>>(mv, rec, org, P, cd, d_cl, list, nullptr, n, o, next, next_n, refine, cnt); break;
+    case K_CONE: k_drop_select<T, K_CONE><<<nb, 256, 0, st>>>(mv, rec, org, P, cd, d_cl, list, nullptr, n, o, next, next_n, refine, cnt); break;
+    default: return fail(OCL_E_UNSUPPORTED, "selection pass has no composite specialisation");
+    }
+    HIP_TRY(hipGetLastError());
+    return OCL_OK;
+}
+
+constexpr uint32_t kCoopBlocks = 2048;  // persistent grid of the cooperative tiers
+
+static double env_scale(const char* name, double def) {
+    const char* v = std::getenv(name);
+    return v ? std::atof(v) : def;
+}
+
+static int launch_drop_fast(ocl_mesh* m, const MeshView& mv, const CutterDev& cd, const double* d_cl, uint32_t n,
+                            int precision, const DropOut& o, Counters* cnt, hipStream_t st) {
+    int rc;
+    const uint32_t* perm = m->perm.as<uint32_t>();
+    if ((rc = m->esc_list.ensure(4ull * n + 64)) || (rc = m->esc_list2.ensure(4ull * n + 64))) return rc;
+    uint32_t* q1 = m->esc_list.as<uint32_t>();
+    uint32_t* q2 = m->esc_list2.as<uint32_t>();
+    uint32_t* qcount = m->qcount.as<uint32_t>();
+    HIP_TRY(hipMemsetAsync(qcount, 0, 16, st));
+    const double se = env_scale("OCL_SEL_E_SCALE", 1.0);
+    const SelErr er64 = make_err<double>(cd, se);
+    const SelParams<double> P64 = make_params<double>(cd, er64);
+    if ((rc = ensure_records<double>(m, cd, m->rec64, m->rec64_ok, m->rec64_cd, er64, st))) return rc;
+    const BoxOrigin* org = m->origins.as<BoxOrigin>();
+    uint32_t* exact_list = q1;
+    uint32_t* exact_n = qcount;
+    if (precision == OCL_PREC_FP32 || precision == OCL_PREC_MIXED) {
+        const SelErr er32 = make_err<float>(cd, se);
+        const SelParams<float> P32 = make_params<float>(cd, er32);
+        if ((rc = ensure_records<float>(m, cd, m->rec32, m->rec32_ok, m->rec32_cd, er32, st))) return rc;
+        const int refine = precision == OCL_PREC_MIXED;
+        if ((rc = launch_select<float>(m, mv, m->rec32.as<float>(), P32, cd, d_cl, perm, n, o, q1, qcount, refine, cnt, st)))
+            return rc;
+        if (!refine) return OCL_OK;
+        if (cd.c.kind != K_CONE) {
+            // FP64 selection of the undecided points, one wavefront per point
+            Counters* c2 = m->cnt2.as<Counters>();
+            const double* r64 = m->rec64.as<double>();
+            switch (cd.c.kind) {
+            case K_CYL: k_drop_coop_select<K_CYL><<<kCoopBlocks, 256, 0, st>>>(mv, r64, org, P64, cd, d_cl, q1, qcount, o, q2, qcount + 1, c2); break;
+            case K_BALL: k_drop_coop_select<K_BALL><<<kCoopBlocks, 256, 0, st>>>(mv, r64, org, P64, cd, d_cl, q1, qcount, o, q2, qcount + 1, c2); break;
+            default: k_drop_coop_select<K_BULL><<<kCoopBlocks, 256, 0, st>>>(mv, r64, org, P64, cd, d_cl, q1, qcount, o, q2, qcount + 1, c2); break;
+            }
+            HIP_TRY(hipGetLastError());
+            exact_list = q2;
+            exact_n = qcount + 1;
+        }
+    } else {
+        if ((rc = launch_select<double>(m, mv, m->rec64.as<double>(), P64, cd, d_cl, perm, n, o, q1, qcount, 1, cnt, st)))
+            return rc;
+    }
+    k_drop_coop_exact<<<kCoopBlocks, 256, 0, st>>>(mv, cd, d_cl, exact_list, exact_n, o, m->cnt3.as<Counters>());
+    HIP_TRY(hipGetLastError());
+    return OCL_OK;
+}
+
 static CutterP simple_from(const ocl_cutter_simple& s, bool& ok) {
     ok = true;
     switch (s.type) {
@@ -418,9 +993,20 @@ int ocl_hip_mesh_create(const double* tri_xyz, size_t ntri, unsigned bucket, ocl
         ++lvl;
     }
     m->top = lvl;
-    if ((rc = m->boxes.ensure(sizeof(float4) * total))) {
+    if ((rc = m->boxes.ensure(sizeof(float4) * total)) || (rc = m->levels.ensure(sizeof(uint32_t) * 2 * kMaxLevels))) {
         delete m;
         return rc;
+    }
+    {
+        uint32_t lv[2 * kMaxLevels] = {};
+        for (int l = 0; l <= m->top; ++l) {
+            lv[2 * l] = m->cnt[l];
+            lv[2 * l + 1] = (uint32_t)m->box_off[l];
+        }
+        if (hipMemcpy(m->levels.p, lv, sizeof(lv), hipMemcpyHostToDevice) != hipSuccess) {
+            delete m;
+            return fail(OCL_E_HIP, "level table upload failed");
+        }
     }
     hipError_t e = hipMemcpy(staging.p, sorted.data(), sizeof(double) * 9 * ntri, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(m->order.p, order.data(), sizeof(uint32_t) * ntri, hipMemcpyHostToDevice);
@@ -434,6 +1020,11 @@ int ocl_hip_mesh_create(const double* tri_xyz, size_t ntri, unsigned bucket, ocl
     k_box_leaf<<<(m->cnt[0] + 255) / 256, 256>>>(m->tri.as<double>(), n32, boxes, m->cnt[0]);
     for (int l = 1; l <= m->top; ++l)
         k_box_up<<<(m->cnt[l] + 255) / 256, 256>>>(boxes + m->box_off[l - 1], m->cnt[l - 1], boxes + m->box_off[l], m->cnt[l]);
+    if ((rc = m->origins.ensure(sizeof(BoxOrigin) * m->cnt[0]))) {
+        delete m;
+        return rc;
+    }
+    k_box_frames<<<(m->cnt[0] + 255) / 256, 256>>>(m->tri.as<double>(), n32, boxes, m->cnt[0], m->origins.as<BoxOrigin>());
     e = hipDeviceSynchronize();
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) {
@@ -469,7 +1060,12 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
     DropOut o{d_z, d_cc, d_type, d_tri, d_feat};
     if ((rc = m->counters.ensure(sizeof(Counters)))) return rc;
     Counters* cnt = m->counters.as<Counters>();
+    if ((rc = m->cnt2.ensure(sizeof(Counters))) || (rc = m->cnt3.ensure(sizeof(Counters))) ||
+        (rc = m->qcount.ensure(16)))
+        return rc;
     HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(Counters), st));
+    HIP_TRY(hipMemsetAsync(m->cnt2.p, 0, sizeof(Counters), st));
+    HIP_TRY(hipMemsetAsync(m->cnt3.p, 0, sizeof(Counters), st));
     // bin CL points
     if ((rc = m->keys.ensure(4ull * n)) || (rc = m->perm.ensure(4ull * n)) || (rc = m->hist.ensure(4ull * 65536)))
         return rc;
@@ -492,9 +1088,9 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
         HIP_TRY(hipEventRecord(m->ev0, st));
     }
     const MeshView mv = m->view();
-    if (precision == OCL_PREC_EXACT64 || m->ntri == 0) {
+    if (precision == OCL_PREC_EXACT64 || m->ntri == 0 || cd.c.kind == K_COMPOSITE) {
         if (m->ntri > 0)
-            k_drop_exact<<<nb, 256, 0, st>>>(mv, cd, d_cl, m->perm.as<uint32_t>(), n32, o, cnt);
+            k_drop_exact<<<nb, 256, 0, st>>>(mv, cd, d_cl, m->perm.as<uint32_t>(), nullptr, n32, o, cnt);
         else
             k_drop_none<<<nb, 256, 0, st>>>(d_cl, n32, o);
     } else {
@@ -507,12 +1103,18 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
         HIP_TRY(hipEventSynchronize(m->ev1));
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, m->ev0, m->ev1));
-        Counters hc;
+        Counters hc, h2, h3;
         HIP_TRY(hipMemcpy(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&h2, m->cnt2.p, sizeof(h2), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&h3, m->cnt3.p, sizeof(h3), hipMemcpyDeviceToHost));
+        uint32_t qc[2] = {0, 0};
+        if (precision != OCL_PREC_EXACT64 && cd.c.kind != K_COMPOSITE && m->ntri > 0)
+            HIP_TRY(hipMemcpy(qc, m->qcount.p, 8, hipMemcpyDeviceToHost));
         stats->points = n;
         stats->pairs = hc.pairs;
-        stats->escalated = hc.escalated;
-        stats->brent_iters = hc.brent;
+        stats->escalated = qc[0];
+        stats->escalated_exact = (precision == OCL_PREC_MIXED && cd.c.kind != K_CONE) ? qc[1] : qc[0];
+        stats->brent_iters = hc.brent + h2.brent + h3.brent;
         stats->kernel_ms = ms;
     }
     return OCL_OK;
@@ -547,8 +1149,3 @@ int ocl_hip_drop_batch(ocl_mesh* m, const ocl_cutter_desc* desc, const double* c
 }
 
 }  // extern "C"
-
-static int launch_drop_fast(ocl_mesh*, const MeshView&, const CutterDev&, const double*, uint32_t, int,
-                            const DropOut&, Counters*, hipStream_t) {
-    return fail(OCL_E_UNSUPPORTED, "precision mode not built yet");
-}

@@ -1,0 +1,288 @@
+// Host build of the selection-pass arithmetic (drop_fast.hpp), for CPU tests
+// of the product's host-compilable logic: record construction, the per-pair
+// selection, the escalation decision and the exact FP64 refine.  It emulates
+// one GPU lane per CL point (the lane's walk visits the level-0 boxes that
+// overlap the point's own cutter box, in rank order).  Test support only:
+// the product path always runs the HIP kernels.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../../include/ocl_hip.h"
+#include "cutter.hpp"
+#include "drop_exact.hpp"
+#include "drop_fast.hpp"
+#include "kd_rank.hpp"
+
+using namespace ocl_amd;
+
+namespace {
+
+constexpr int kLeafH = 16;  // triangles per level-0 box (same as mesh.hpp kLeaf)
+
+bool cutter_of(const ocl_cutter_desc* d, CutterDev& cd) {
+    std::memset(&cd, 0, sizeof(cd));
+    auto simple = [](const ocl_cutter_simple& s, CutterP& c) {
+        switch (s.type) {
+        case OCL_CUTTER_CYL: c = CutterP::cyl(s.diameter, s.length); return true;
+        case OCL_CUTTER_BALL: c = CutterP::ball(s.diameter, s.length); return true;
+        case OCL_CUTTER_BULL: c = CutterP::bull(s.diameter, s.corner_radius, s.length); return true;
+        case OCL_CUTTER_CONE: c = CutterP::cone(s.diameter, s.angle, s.length); return true;
+        default: return false;
+        }
+    };
+    return simple(d->c, cd.c);
+}
+
+template <class T>
+SelParams<T> params(const CutterDev& cd, const SelErr& er) {
+    return sel_params<T>(cd, er);
+}
+
+template <class T, int K>
+void run_points(const std::vector<TriX>& tx, const std::vector<BoxOrigin>& org, const std::vector<double>& bbx,
+                const CutterDev& cd, const SelParams<T>& P, const double* cl, size_t n, const std::vector<T>& rec,
+                const uint32_t* order, double* z_out, int* type_out, int* esc_out, int* feat_out, int* tri_out) {
+    typedef FM<T> F;
+    const double R = cd.c.radius;
+    const size_t ntri = tx.size();
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long i = 0; i < (long)n; ++i) {
+        const double x = cl[3 * i], y = cl[3 * i + 1], z0 = cl[3 * i + 2];
+        Sel<T> S;
+        sel_init(S, (T)0);
+        T wzb = 0, wzt[4], wza = -F::inf();
+        for (int q = 0; q < 4; ++q) wzt[q] = -F::inf();
+        bool framed = false;
+        double ozw = 0;
+        const size_t nbox = (ntri + kLeafH - 1) / kLeafH;
+        for (size_t b = 0; b < nbox; ++b) {
+            const double* bb = &bbx[4 * b];
+            if (bb[1] < x - R || bb[0] > x + R || bb[3] < y - R || bb[2] > y + R) continue;
+            const BoxOrigin& bo = org[b];
+            if (!framed) {
+                framed = true;
+                ozw = bo.oz;
+                wzb = (T)(z0 - ozw);
+            }
+            const T dz = (T)(bo.oz - ozw);
+            const T xl = (T)(x - bo.ox), yl = (T)(y - bo.oy);
+            const uint32_t bt_in = S.bt;
+            S.zb = wzb - dz;
+            for (int q = 0; q < 4; ++q) S.zt[q] = -F::inf();
+            S.za = -F::inf();
+            const size_t e = std::min(ntri, (b + 1) * kLeafH);
+            for (size_t k = b * kLeafH; k < e; ++k) {
+                const T* r = &rec[(size_t)kFastRec * k];
+                if (!(r[R_BX1] < xl - P.R || r[R_BX0] > xl + P.R || r[R_BY1] < yl - P.R || r[R_BY0] > yl + P.R))
+                    sel_triangle<T, K>(r, (uint32_t)k, xl, yl, P, S, true);
+            }
+            if (S.bt != bt_in) wzb = S.zb + dz;
+            for (int q = 0; q < 4; ++q) wzt[q] = F::max_(wzt[q], S.zt[q] + dz);
+            wza = F::max_(wza, S.za + dz);
+        }
+        Sel<T> D = S;
+        D.zb = wzb;
+        for (int q = 0; q < 4; ++q) D.zt[q] = wzt[q];
+        D.za = wza;
+        bool up = framed && sel_escalate(D, (T)(z0 - ozw), P);
+        int why = 0;
+        if (framed) {
+            if (D.bamb) why |= 1;
+            if (D.oamb) why |= 2;
+            T other = -F::inf();
+            const int sl = feat_slot(D.bf);
+            for (int q = 0; q < 4; ++q)
+                if (D.bt == 0xffffffffu || q != sl) other = F::max_(other, D.zt[q]);
+            if (D.bt != 0xffffffffu) other = F::max_(other, (T)(z0 - ozw) + P.E);
+            if (other >= D.zb - D.eb) why |= 4;
+            if (D.za >= D.zb - D.eb) why |= 8;
+        }
+        double zr = z0;
+        int ty = CC_NONE, ft = FT_NONE, tr = -1;
+        if (D.bt != 0xffffffffu) {
+            const TriX& t = tx[D.bt];
+            DropLane X;
+            X.x = x; X.y = y; X.z = -INFINITY; X.cc = v3(0, 0, 0); X.type = CC_NONE;
+            X.tri = 0xffffffffu; X.feat = FT_NONE; X.cur_tri = D.bt; X.cur_feat = 0;
+            int it = 0;
+            exact_feature(cd.c, X, t, D.bf, it);
+            zr = ozw + (double)D.zb;
+            ty = X.type;
+            ft = D.bf;
+            tr = (int)order[D.bt];
+            if (!up && (X.type == CC_NONE || feat_slot(X.feat) != feat_slot(D.bf) || !(X.z > z0) ||
+                        fabs(X.z - zr) > 4.0 * ((double)D.eb + (double)P.E))) {
+                up = true;
+                why |= 16;
+            }
+            if (!up) zr = X.z;
+        }
+        z_out[i] = zr;
+        type_out[i] = ty;
+        esc_out[i] = up ? (1 | (why << 1)) : 0;
+        feat_out[i] = ft;
+        tri_out[i] = tr;
+    }
+}
+
+}  // namespace
+
+extern "C" int ocl_selcheck_drop(const double* tris, size_t ntri, unsigned bucket, const ocl_cutter_desc* desc,
+                                 const double* cl, size_t n, int use_float, double escale, double* z_out,
+                                 int* type_out, int* esc_out, int* feat_out, int* tri_out) {
+    CutterDev cd;
+    if (!cutter_of(desc, cd)) return OCL_E_UNSUPPORTED;
+    std::vector<double> bb(6 * ntri);
+    for (size_t i = 0; i < ntri; ++i)
+        for (int d = 0; d < 3; ++d) {
+            double lo = tris[9 * i + d], hi = lo;
+            for (int k = 1; k < 3; ++k) {
+                lo = std::fmin(lo, tris[9 * i + 3 * k + d]);
+                hi = std::fmax(hi, tris[9 * i + 3 * k + d]);
+            }
+            bb[6 * i + 2 * d] = lo;
+            bb[6 * i + 2 * d + 1] = hi;
+        }
+    static const int kXY[4] = {0, 1, 2, 3};
+    const std::vector<uint32_t> order = kd_rank_order(bb.data(), ntri, kXY, bucket);
+    std::vector<TriX> tx(ntri);
+    for (size_t r = 0; r < ntri; ++r) {
+        const double* q = tris + 9 * (size_t)order[r];
+        TriX& t = tx[r];
+        t.p[0] = v3(q[0], q[1], q[2]);
+        t.p[1] = v3(q[3], q[4], q[5]);
+        t.p[2] = v3(q[6], q[7], q[8]);
+        t.n = vunit(vcross(vsub(t.p[0], t.p[1]), vsub(t.p[0], t.p[2])));
+        t.maxz = std::fmax(std::fmax(q[2], q[5]), q[8]);
+    }
+    const size_t nbox = (ntri + kLeafH - 1) / kLeafH;
+    std::vector<BoxOrigin> org(nbox);
+    std::vector<double> bbx(4 * nbox);
+    for (size_t b = 0; b < nbox; ++b) {
+        double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY, z0 = INFINITY, z1 = -INFINITY;
+        for (size_t k = b * kLeafH; k < std::min(ntri, (b + 1) * kLeafH); ++k)
+            for (int j = 0; j < 3; ++j) {
+                x0 = std::fmin(x0, tx[k].p[j].x); x1 = std::fmax(x1, tx[k].p[j].x);
+                y0 = std::fmin(y0, tx[k].p[j].y); y1 = std::fmax(y1, tx[k].p[j].y);
+                z0 = std::fmin(z0, tx[k].p[j].z); z1 = std::fmax(z1, tx[k].p[j].z);
+            }
+        // same rounding as the device boxes (float, outward)
+        const float fx0 = std::nextafterf((float)x0, -INFINITY), fx1 = std::nextafterf((float)x1, INFINITY);
+        const float fy0 = std::nextafterf((float)y0, -INFINITY), fy1 = std::nextafterf((float)y1, INFINITY);
+        bbx[4 * b] = fx0; bbx[4 * b + 1] = fx1; bbx[4 * b + 2] = fy0; bbx[4 * b + 3] = fy1;
+        org[b].ox = 0.5 * ((double)fx0 + (double)fx1);
+        org[b].oy = 0.5 * ((double)fy0 + (double)fy1);
+        org[b].oz = 0.5 * (z0 + z1);
+        org[b].pad = 0;
+    }
+    auto go = [&](auto tag) {
+        typedef decltype(tag) T;
+        SelErr er;
+        er.u = FM<T>::eps() * escale;
+        er.R = cd.c.radius;
+        er.zmag = 4.0 * cd.c.radius + 10.0;
+        const SelParams<T> P = params<T>(cd, er);
+        std::vector<T> rec((size_t)kFastRec * ntri);
+        for (size_t k = 0; k < ntri; ++k) {
+            const BoxOrigin& o = org[k / kLeafH];
+            build_record<T>(tx[k], cd, o.ox, o.oy, o.oz, er, &rec[(size_t)kFastRec * k]);
+        }
+        switch (cd.c.kind) {
+        case K_CYL: run_points<T, K_CYL>(tx, org, bbx, cd, P, cl, n, rec, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
+        case K_BALL: run_points<T, K_BALL>(tx, org, bbx, cd, P, cl, n, rec, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
+        case K_BULL: run_points<T, K_BULL>(tx, org, bbx, cd, P, cl, n, rec, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
+        default: run_points<T, K_CONE>(tx, org, bbx, cd, P, cl, n, rec, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
+        }
+    };
+    if (use_float) go(0.0f);
+    else go(0.0);
+    return OCL_OK;
+}
+
+// Error calibration: for every (point, triangle) pair evaluates each contact
+// (facet, 3 vertices, 3 edges) from the float and the double record and
+// returns, per contact class, the largest |z_float - z_double| / bound seen
+// on valid, well-conditioned contacts (bound = the error bound the selection
+// uses for that contact).  ratio[0] facet, [1] vertex, [2] edge.
+extern "C" int ocl_selcheck_calibrate(const double* tris, size_t ntri, const ocl_cutter_desc* desc, const double* cl,
+                                      size_t n, double* ratio, double* worst_abs) {
+    CutterDev cd;
+    if (!cutter_of(desc, cd)) return OCL_E_UNSUPPORTED;
+    if (cd.c.kind != K_CYL && cd.c.kind != K_BALL) return OCL_E_UNSUPPORTED;
+    SelErr e32, e64;
+    e32.u = FM<float>::eps(); e32.R = cd.c.radius; e32.zmag = 4.0 * cd.c.radius + 10.0;
+    e64 = e32; e64.u = FM<double>::eps();
+    const SelParams<float> P = sel_params<float>(cd, e32);
+    std::vector<TriX> tx(ntri);
+    for (size_t r = 0; r < ntri; ++r) {
+        const double* q = tris + 9 * r;
+        tx[r].p[0] = v3(q[0], q[1], q[2]); tx[r].p[1] = v3(q[3], q[4], q[5]); tx[r].p[2] = v3(q[6], q[7], q[8]);
+        tx[r].n = vunit(vcross(vsub(tx[r].p[0], tx[r].p[1]), vsub(tx[r].p[0], tx[r].p[2])));
+        tx[r].maxz = std::fmax(std::fmax(q[2], q[5]), q[8]);
+    }
+    for (int c = 0; c < 3; ++c) { ratio[c] = 0; worst_abs[c] = 0; }
+    const double R = cd.c.radius;
+    for (size_t i = 0; i < n; ++i) {
+        const double x = cl[3 * i], y = cl[3 * i + 1];
+        for (size_t k = 0; k < ntri; ++k) {
+            const TriX& t = tx[k];
+            double bx0 = std::fmin(std::fmin(t.p[0].x, t.p[1].x), t.p[2].x), bx1 = std::fmax(std::fmax(t.p[0].x, t.p[1].x), t.p[2].x);
+            double by0 = std::fmin(std::fmin(t.p[0].y, t.p[1].y), t.p[2].y), by1 = std::fmax(std::fmax(t.p[0].y, t.p[1].y), t.p[2].y);
+            if (bx1 < x - R || bx0 > x + R || by1 < y - R || by0 > y + R) continue;
+            // box origin: snap to a 2R grid like a level-0 box centre would be
+            const double ox = std::floor(bx0 / 2.0) * 2.0 + 1.0, oy = std::floor(by0 / 2.0) * 2.0 + 1.0, oz = t.maxz;
+            float rf[kFastRec];
+            double rd[kFastRec];
+            build_record<float>(t, cd, ox, oy, oz, e32, rf);
+            build_record<double>(t, cd, ox, oy, oz, e64, rd);
+            const float xf = (float)(x - ox), yf = (float)(y - oy);
+            const double xd = x - ox, yd = y - oy;
+            // facet
+            {
+                const float u = fmaf(rf[R_U2], yf, fmaf(rf[R_U1], xf, rf[R_U0]));
+                const float v = fmaf(rf[R_V2], yf, fmaf(rf[R_V1], xf, rf[R_V0]));
+                const float mf = std::fmin(std::fmin(u, v), 1.0f - u - v);
+                const float zf = fmaf(rf[R_GY], yf, fmaf(rf[R_GX], xf, rf[R_F0]));
+                const double zd = fma(rd[R_GY], yd, fma(rd[R_GX], xd, rd[R_F0]));
+                if (mf > rf[R_EBF]) {
+                    const double err = std::fabs((double)zf - zd);
+                    ratio[0] = std::fmax(ratio[0], err / (double)rf[R_EZF]);
+                    worst_abs[0] = std::fmax(worst_abs[0], err);
+                }
+            }
+            for (int j = 0; j < 3; ++j) {
+                const float dx = rf[R_P0X + 3 * j] - xf, dy = rf[R_P0Y + 3 * j] - yf;
+                const float m = P.R2 - fmaf(dx, dx, dy * dy);
+                const double ddx = rd[R_P0X + 3 * j] - xd, ddy = rd[R_P0Y + 3 * j] - yd;
+                const double md = R * R - fma(ddx, ddx, ddy * ddy);
+                if (m > P.S2V) {
+                    const float zf = rf[R_P0Z + 3 * j] + (cd.c.kind == K_BALL ? std::sqrt(m) : 0.0f);
+                    const double zd = rd[R_P0Z + 3 * j] + (cd.c.kind == K_BALL ? std::sqrt(md) : 0.0);
+                    const double err = std::fabs((double)zf - zd);
+                    ratio[1] = std::fmax(ratio[1], err / (double)P.E);
+                    worst_abs[1] = std::fmax(worst_abs[1], err);
+                }
+                const float* ef = rf + R_E0 + kEdge * j;
+                const double* ed = rd + R_E0 + kEdge * j;
+                const float sd = fmaf(ef[E_EX], yf, fmaf(-ef[E_EY], xf, ef[E_C]));
+                const float me = fmaf(-sd, sd, P.R2);
+                const float u1 = fmaf(-ef[E_EY], yf, fmaf(-ef[E_EX], xf, ef[E_A]));
+                const double sdd = fma(ed[E_EX], yd, fma(-ed[E_EY], xd, ed[E_C]));
+                const double med = fma(-sdd, sdd, R * R);
+                const double u1d = fma(-ed[E_EY], yd, fma(-ed[E_EX], xd, ed[E_A]));
+                if (me > ef[E_S2E] && med > 0) {
+                    const float s = std::sqrt(me);
+                    const double sdb = std::sqrt(med);
+                    const float zf = fmaf(s, ef[E_K1], fmaf(-u1, ef[E_M], ef[E_Z1]));
+                    const double zd = fma(sdb, ed[E_K1], fma(-u1d, ed[E_M], ed[E_Z1]));
+                    const double err = std::fabs((double)zf - zd);
+                    ratio[2] = std::fmax(ratio[2], err / (double)ef[E_EZE]);
+                    worst_abs[2] = std::fmax(worst_abs[2], err);
+                }
+            }
+        }
+    }
+    return OCL_OK;
+}

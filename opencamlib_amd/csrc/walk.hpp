@@ -36,7 +36,8 @@ __device__ inline void walk_boxes(const BoxTree& T, float qx0, float qy0, float 
     int lvl = T.top;
     uint32_t node = 0;
     for (;;) {
-        if (node < T.cnt[lvl] && box_hits(T.box[lvl][node], qx0, qy0, qx1, qy1)) {
+        const uint32_t cnt = T.lvl[2 * lvl], off = T.lvl[2 * lvl + 1];
+        if (node < cnt && box_hits(T.box[off + node], qx0, qy0, qx1, qy1)) {
             if (lvl == 0) {
                 visit(uniform_u(node));
             } else {
@@ -46,11 +47,11 @@ __device__ inline void walk_boxes(const BoxTree& T, float qx0, float qy0, float 
             }
         }
         ++node;
-        while (lvl < T.top && (node % kFan == 0 || node >= T.cnt[lvl])) {
+        while (lvl < T.top && (node % kFan == 0 || node >= T.lvl[2 * lvl])) {
             node = (node + kFan - 1) / kFan;
             ++lvl;
         }
-        if (lvl == T.top && node >= T.cnt[lvl]) break;
+        if (lvl == T.top && node >= T.lvl[2 * lvl]) break;
     }
 }
 
