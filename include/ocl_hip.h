@@ -81,6 +81,8 @@ typedef struct {
     uint64_t brent_iters;   /* Bull edge Brent iterations (FP64) */
     uint64_t intervals;     /* push: output intervals */
     double kernel_ms;       /* device time of the compute kernels */
+    double select_ms;       /* device time of the first selection kernel (the
+                               whole compute for exact64 and composites) */
 } ocl_stats;
 
 /* Mesh: ntri triangles, 9 doubles each (p0, p1, p2).  bucket = the KD-tree
@@ -91,6 +93,15 @@ typedef struct {
 int ocl_hip_mesh_create(const double* tri_xyz, size_t ntri, unsigned bucket, ocl_mesh** out);
 int ocl_hip_mesh_destroy(ocl_mesh* mesh);
 size_t ocl_hip_mesh_size(const ocl_mesh* mesh);
+
+/* Host only (no device work): the reference KD-tree's leaf order for a
+ * triangle array, order_out[rank] = triangle index.  dimset 0 = XY (drop,
+ * KDTree::setXYDimensions), 1 = YZ (X fibers, setYZDimensions), 2 = XZ
+ * (Y fibers, setXZDimensions); kdtree.hpp:117-268.  It is the order in which
+ * KDTree::search returns overlapping triangles (kdtree.hpp:273-304), used for
+ * getTrianglesUnderCutter / getOverlapTriangles (batchdropcutter_py.hpp:46-55,
+ * batchpushcutter_py.hpp:59-84). */
+int ocl_hip_kd_order(const double* tri_xyz, size_t ntri, int dimset, unsigned bucket, uint32_t* order_out);
 
 /* Batch drop-cutter: replaces BatchDropCutter::run / dropCutter5
  * (batchdropcutter.cpp:187-231).  cl_xyz: n x (x, y, z0).  Outputs (any may

@@ -37,7 +37,8 @@ class CutterDesc(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("points", C.c_uint64), ("pairs", C.c_uint64), ("escalated", C.c_uint64),
                 ("escalated_exact", C.c_uint64),
-                ("brent_iters", C.c_uint64), ("intervals", C.c_uint64), ("kernel_ms", C.c_double)]
+                ("brent_iters", C.c_uint64), ("intervals", C.c_uint64), ("kernel_ms", C.c_double),
+                ("select_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -66,6 +67,8 @@ def lib():
         L.ocl_hip_drop_device.argtypes = [vp, C.POINTER(CutterDesc), vp, sz, C.c_int, vp, vp, vp, vp, vp,
                                           C.POINTER(Stats), vp]
         L.ocl_hip_drop_device.restype = C.c_int
+        L.ocl_hip_kd_order.argtypes = [dp, sz, C.c_int, C.c_uint, C.POINTER(C.c_uint32)]
+        L.ocl_hip_kd_order.restype = C.c_int
         L.ocl_hip_last_error.argtypes = []
         L.ocl_hip_last_error.restype = C.c_char_p
         _lib = L
@@ -114,6 +117,19 @@ def _ptr(a, t):
     return a.ctypes.data_as(C.POINTER(t))
 
 
+DIMSETS = {"xy": 0, "yz": 1, "xz": 2}
+
+
+def kd_order(tris, dimset="xy", bucket: int = 1) -> np.ndarray:
+    """order[rank] = triangle index in the reference KD-tree's leaf order
+    (host computation, no device needed)."""
+    t = np.ascontiguousarray(np.asarray(tris, dtype=np.float64).reshape(-1, 9))
+    out = np.empty(t.shape[0], dtype=np.uint32)
+    check(lib().ocl_hip_kd_order(_ptr(t, C.c_double), t.shape[0], DIMSETS[dimset], bucket,
+                                 _ptr(out, C.c_uint32)))
+    return out
+
+
 class DeviceMesh:
     """Owns an ocl_mesh handle (device copy + box hierarchy in KD order)."""
 
@@ -133,6 +149,8 @@ class DeviceMesh:
         self.close()
 
     def drop(self, spec: dict, cl_xyz, precision="exact64", want_stats=True):
+        """Batch drop of host CL points (n, 3).  Returns z, cc (n, 3), type
+        (CCType), tri (input index, -1 where nothing was touched), feat."""
         cl = np.ascontiguousarray(np.asarray(cl_xyz, dtype=np.float64).reshape(-1, 3))
         n = cl.shape[0]
         z = np.empty(n)
@@ -146,7 +164,9 @@ class DeviceMesh:
                                        _ptr(z, C.c_double), _ptr(cc, C.c_double), _ptr(ty, C.c_uint8),
                                        _ptr(tri, C.c_uint32), _ptr(ft, C.c_uint8),
                                        C.byref(st) if want_stats else None, None))
-        return dict(z=z, cc=cc, type=ty, tri=tri, feat=ft, stats=st.as_dict())
+        tri64 = tri.astype(np.int64)
+        tri64[tri == 0xFFFFFFFF] = -1
+        return dict(z=z, cc=cc, type=ty, tri=tri64, feat=ft, stats=st.as_dict())
 
     def drop_device(self, spec: dict, d_cl: int, n: int, precision, d_z: int, d_cc=0, d_type=0, d_tri=0,
                     d_feat=0, stream=None, stats=False):

@@ -78,7 +78,7 @@ struct ocl_mesh {
     ocl_amd::DevBuf rec32, rec64;
     bool rec32_ok = false, rec64_ok = false;
     ocl_amd::CutterDev rec32_cd{}, rec64_cd{};
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_s0 = nullptr, ev_s1 = nullptr;
 
     ocl_amd::MeshView view() const {
         ocl_amd::MeshView v;
@@ -838,7 +838,7 @@ static double env_scale(const char* name, double def) {
 }
 
 static int launch_drop_fast(ocl_mesh* m, const MeshView& mv, const CutterDev& cd, const double* d_cl, uint32_t n,
-                            int precision, const DropOut& o, Counters* cnt, hipStream_t st) {
+                            int precision, const DropOut& o, Counters* cnt, hipStream_t st, bool timed) {
     int rc;
     const uint32_t* perm = m->perm.as<uint32_t>();
     if ((rc = m->esc_list.ensure(4ull * n + 64)) || (rc = m->esc_list2.ensure(4ull * n + 64))) return rc;
@@ -858,8 +858,10 @@ static int launch_drop_fast(ocl_mesh* m, const MeshView& mv, const CutterDev& cd
         const SelParams<float> P32 = make_params<float>(cd, er32);
         if ((rc = ensure_records<float>(m, cd, m->rec32, m->rec32_ok, m->rec32_cd, er32, st))) return rc;
         const int refine = precision == OCL_PREC_MIXED;
+        if (timed) HIP_TRY(hipEventRecord(m->ev_s0, st));
         if ((rc = launch_select<float>(m, mv, m->rec32.as<float>(), P32, cd, d_cl, perm, n, o, q1, qcount, refine, cnt, st)))
             return rc;
+        if (timed) HIP_TRY(hipEventRecord(m->ev_s1, st));
         if (!refine) return OCL_OK;
         if (cd.c.kind != K_CONE) {
             // FP64 selection of the undecided points, one wavefront per point
@@ -875,8 +877,10 @@ static int launch_drop_fast(ocl_mesh* m, const MeshView& mv, const CutterDev& cd
             exact_n = qcount + 1;
         }
     } else {
+        if (timed) HIP_TRY(hipEventRecord(m->ev_s0, st));
         if ((rc = launch_select<double>(m, mv, m->rec64.as<double>(), P64, cd, d_cl, perm, n, o, q1, qcount, 1, cnt, st)))
             return rc;
+        if (timed) HIP_TRY(hipEventRecord(m->ev_s1, st));
     }
     k_drop_coop_exact<<<kCoopBlocks, 256, 0, st>>>(mv, cd, d_cl, exact_list, exact_n, o, m->cnt3.as<Counters>());
     HIP_TRY(hipGetLastError());
@@ -926,9 +930,41 @@ static int cutter_from_desc(const ocl_cutter_desc* d, CutterDev& cd) {
     return OCL_OK;
 }
 
+// bbox per triangle, [minx maxx miny maxy minz maxz] (bbox.cpp:64-90)
+static std::vector<double> tri_bboxes(const double* tri_xyz, size_t ntri) {
+    std::vector<double> bb(6 * ntri);
+    for (size_t i = 0; i < ntri; ++i) {
+        const double* q = tri_xyz + 9 * i;
+        for (int d = 0; d < 3; ++d) {
+            double lo = q[d], hi = q[d];
+            for (int k = 1; k < 3; ++k) {
+                if (q[3 * k + d] > hi) hi = q[3 * k + d];
+                if (q[3 * k + d] < lo) lo = q[3 * k + d];
+            }
+            bb[6 * i + 2 * d] = lo;
+            bb[6 * i + 2 * d + 1] = hi;
+        }
+    }
+    return bb;
+}
+
+static const int kDimsXY[4] = {0, 1, 2, 3};
+static const int kDimsYZ[4] = {2, 3, 4, 5};
+static const int kDimsXZ[4] = {0, 1, 4, 5};
+
 extern "C" {
 
 const char* ocl_hip_last_error(void) { return g_err.c_str(); }
+
+int ocl_hip_kd_order(const double* tri_xyz, size_t ntri, int dimset, unsigned bucket, uint32_t* order_out) {
+    if (ntri > 0 && (!tri_xyz || !order_out)) return fail(OCL_E_ARG, "NULL pointer");
+    if (dimset < 0 || dimset > 2) return fail(OCL_E_ARG, "dimset must be 0 (XY), 1 (YZ) or 2 (XZ)");
+    const int* dims = dimset == 0 ? kDimsXY : dimset == 1 ? kDimsYZ : kDimsXZ;
+    std::vector<double> bb = tri_bboxes(tri_xyz, ntri);
+    std::vector<uint32_t> order = kd_rank_order(bb.data(), ntri, dims, bucket);
+    std::copy(order.begin(), order.end(), order_out);
+    return OCL_OK;
+}
 
 size_t ocl_hip_mesh_size(const ocl_mesh* mesh) { return mesh ? mesh->ntri : 0; }
 
@@ -943,18 +979,8 @@ int ocl_hip_mesh_create(const double* tri_xyz, size_t ntri, unsigned bucket, ocl
     m->ntri = ntri;
     m->bucket = bucket < 1 ? 1 : bucket;
     // host: bbox per triangle and the reference's KD leaf order
-    std::vector<double> bb(6 * ntri);
+    std::vector<double> bb = tri_bboxes(tri_xyz, ntri);
     for (size_t i = 0; i < ntri; ++i) {
-        const double* q = tri_xyz + 9 * i;
-        for (int d = 0; d < 3; ++d) {
-            double lo = q[d], hi = q[d];
-            for (int k = 1; k < 3; ++k) {
-                if (q[3 * k + d] > hi) hi = q[3 * k + d];
-                if (q[3 * k + d] < lo) lo = q[3 * k + d];
-            }
-            bb[6 * i + 2 * d] = lo;
-            bb[6 * i + 2 * d + 1] = hi;
-        }
         for (int d = 0; d < 6; ++d) {
             const double v = bb[6 * i + d];
             if (i == 0) m->bbox[d] = v;
@@ -966,8 +992,7 @@ int ocl_hip_mesh_create(const double* tri_xyz, size_t ntri, unsigned bucket, ocl
         *out = m;
         return OCL_OK;
     }
-    static const int kXY[4] = {0, 1, 2, 3};
-    std::vector<uint32_t> order = kd_rank_order(bb.data(), ntri, kXY, m->bucket);
+    std::vector<uint32_t> order = kd_rank_order(bb.data(), ntri, kDimsXY, m->bucket);
     std::vector<double> sorted(9 * ntri);
     for (size_t r = 0; r < ntri; ++r) std::memcpy(&sorted[9 * r], tri_xyz + 9 * (size_t)order[r], 9 * sizeof(double));
     int rc;
@@ -1039,6 +1064,8 @@ int ocl_hip_mesh_destroy(ocl_mesh* mesh) {
     if (!mesh) return OCL_OK;
     if (mesh->ev0) (void)hipEventDestroy(mesh->ev0);
     if (mesh->ev1) (void)hipEventDestroy(mesh->ev1);
+    if (mesh->ev_s0) (void)hipEventDestroy(mesh->ev_s0);
+    if (mesh->ev_s1) (void)hipEventDestroy(mesh->ev_s1);
     delete mesh;
     return OCL_OK;
 }
@@ -1085,16 +1112,20 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
     if (stats) {
         if (!m->ev0) HIP_TRY(hipEventCreate(&m->ev0));
         if (!m->ev1) HIP_TRY(hipEventCreate(&m->ev1));
+        if (!m->ev_s0) HIP_TRY(hipEventCreate(&m->ev_s0));
+        if (!m->ev_s1) HIP_TRY(hipEventCreate(&m->ev_s1));
         HIP_TRY(hipEventRecord(m->ev0, st));
     }
     const MeshView mv = m->view();
+    bool fast = false;
     if (precision == OCL_PREC_EXACT64 || m->ntri == 0 || cd.c.kind == K_COMPOSITE) {
         if (m->ntri > 0)
             k_drop_exact<<<nb, 256, 0, st>>>(mv, cd, d_cl, m->perm.as<uint32_t>(), nullptr, n32, o, cnt);
         else
             k_drop_none<<<nb, 256, 0, st>>>(d_cl, n32, o);
     } else {
-        rc = launch_drop_fast(m, mv, cd, d_cl, n32, precision, o, cnt, st);
+        rc = launch_drop_fast(m, mv, cd, d_cl, n32, precision, o, cnt, st, stats != nullptr);
+        fast = true;
         if (rc) return rc;
     }
     HIP_TRY(hipGetLastError());
@@ -1116,6 +1147,11 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
         stats->escalated_exact = (precision == OCL_PREC_MIXED && cd.c.kind != K_CONE) ? qc[1] : qc[0];
         stats->brent_iters = hc.brent + h2.brent + h3.brent;
         stats->kernel_ms = ms;
+        stats->select_ms = ms;
+        if (fast) {
+            HIP_TRY(hipEventElapsedTime(&ms, m->ev_s0, m->ev_s1));
+            stats->select_ms = ms;
+        }
     }
     return OCL_OK;
 }

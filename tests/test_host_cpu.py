@@ -1,0 +1,122 @@
+"""CPU tests of the host side: the C ABI library loads and exports every
+symbol include/*.h declares, the host KD order equals the oracle's, and the
+selection arithmetic (compiled for the host from the same headers as the
+kernels) decides contacts like the oracle wherever it does not escalate."""
+import ctypes as C
+import glob
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from cutters import ALL
+from opencamlib_amd import capi, workloads as W
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    syms = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        syms |= set(re.findall(r"\b(ocl_hip_\w+)\s*\(", text))
+    return sorted(syms)
+
+
+def test_library_exports_every_declared_symbol():
+    syms = header_symbols()
+    assert "ocl_hip_drop_batch" in syms and "ocl_hip_mesh_create" in syms
+    lib = C.CDLL(capi.LIB_PATH)
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_error_paths_without_device_work():
+    L = capi.lib()
+    rc = L.ocl_hip_kd_order(None, 3, 0, 1, None)
+    assert rc == -1 and b"NULL" in L.ocl_hip_last_error()
+    t = np.zeros((1, 9))
+    out = np.zeros(1, dtype=np.uint32)
+    with pytest.raises(capi.OclError, match="dimset"):
+        capi.check(L.ocl_hip_kd_order(t.ctypes.data_as(C.POINTER(C.c_double)), 1, 7, 1,
+                                      out.ctypes.data_as(C.POINTER(C.c_uint32))))
+
+
+@pytest.mark.parametrize("dimset,odim", [("xy", 0), ("yz", 1), ("xz", 2)])
+@pytest.mark.parametrize("mesh", ["grid", "noisy", "spheres"])
+def test_kd_order_matches_oracle(dimset, odim, mesh):
+    if mesh == "grid":  # regular grid: every split ties on spread (min-dim rule)
+        tris = W.heightfield(16, 16.0, bumps=0, noise=0.0, seed=0)
+    elif mesh == "noisy":
+        tris = W.heightfield(20, 50.0, bumps=3, steps=3, seed=4)
+    else:
+        tris = W.spheres_scene(3, 2, 50.0, seed=9)
+    for bucket in (1, 4):
+        order = capi.kd_order(tris, dimset, bucket)
+        rank = O.Mesh(tris).kd_rank(odim, bucket)
+        inv = np.empty_like(rank)
+        inv[rank] = np.arange(len(rank))
+        assert np.array_equal(order, inv)
+
+
+class SelCheck:
+    """ctypes view of libocl_selcheck.so (host build of the selection pass)."""
+
+    def __init__(self):
+        path = os.path.join(ROOT, "opencamlib_amd", "lib", "libocl_selcheck.so")
+        self.L = C.CDLL(path)
+        dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+        self.L.ocl_selcheck_drop.argtypes = [dp, C.c_size_t, C.c_uint, C.POINTER(capi.CutterDesc), dp,
+                                             C.c_size_t, C.c_int, C.c_double, dp, ip, ip, ip, ip]
+        self.L.ocl_selcheck_drop.restype = C.c_int
+
+    def drop(self, tris, spec, cl, use_float):
+        t = np.ascontiguousarray(tris, dtype=np.float64)
+        cl = np.ascontiguousarray(cl, dtype=np.float64)
+        n = cl.shape[0]
+        z = np.empty(n)
+        ty, esc, ft, tr = (np.empty(n, dtype=np.int32) for _ in range(4))
+        d = capi.cutter_desc(spec)
+        P = lambda a, ct: a.ctypes.data_as(C.POINTER(ct))
+        rc = self.L.ocl_selcheck_drop(P(t, C.c_double), t.shape[0], 1, C.byref(d), P(cl, C.c_double), n,
+                                      int(use_float), 1.0, P(z, C.c_double), P(ty, C.c_int), P(esc, C.c_int),
+                                      P(ft, C.c_int), P(tr, C.c_int))
+        assert rc == 0
+        return dict(z=z, type=ty, esc=esc, feat=ft, tri=tr)
+
+
+@pytest.fixture(scope="module")
+def scene():
+    tris = W.heightfield(48, 60.0, bumps=4, steps=6, seed=21)
+    cl = W.subsample(W.cl_grid(300, 300, 0.2, z0=-10.0), 3000, seed=5)
+    return tris, cl
+
+
+@pytest.mark.parametrize("name", ["cyl", "ball", "bull", "cone"])
+@pytest.mark.parametrize("use_float", [1, 0])
+def test_selection_decides_like_oracle(scene, name, use_float):
+    """Points the selection pass does not escalate carry the oracle's triangle
+    and contact type, and a z within the FP32 (FP64) selection error."""
+    tris, cl = scene
+    spec = ALL[name]
+    ref = O.Mesh(tris).batch_drop(O.cutter_from_spec(spec), cl, search_mode=1)
+    got = SelCheck().drop(tris, spec, cl, use_float)
+    ok = (got["esc"] & 1) == 0
+    frac = 1.0 - ok.mean()
+    assert frac < 0.15, frac  # cone tips on grid lines tie exactly and escalate
+    assert np.array_equal(got["type"][ok], ref["type"][ok])
+    tol = 2e-4 if use_float else 1e-9
+    assert np.max(np.abs(got["z"][ok] - ref["z"][ok])) < tol
+    # A different triangle is only allowed where it realises the same contact
+    # height: a shared edge or vertex evaluated from the neighbouring triangle.
+    cut = O.cutter_from_spec(spec)
+    for i in np.nonzero(ok & (got["tri"] != ref["tri"]))[0]:
+        # start just below the height (a cone facet lift below the current z
+        # does not end the triangle's tests, conecutter.cpp:100-102)
+        alt = O.drop_triangle(cut, tris[got["tri"][i]], cl[i, 0], cl[i, 1], ref["z"][i] - 1e-7)
+        assert alt["type"] == ref["type"][i]
+        assert abs(alt["z"] - ref["z"][i]) < 1e-9
