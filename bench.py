@@ -88,6 +88,18 @@ def cpu_baseline(tris, spec, cl, n_kd, n_scan, seed, O, W):
     return dict(threads=threads, idx_kd=idx_kd, r_kd=r_kd, t_kd=t_kd, idx=idx, r=r_scan, t_scan=t_scan)
 
 
+def traffic_from_profile(cutter):
+    """HBM bytes per launch of the selection kernel, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"drop_select_{cutter}_c1_hbm_bytes.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("TRAFFIC_BYTES_PER_LAUNCH"):
+            return float(line.split()[-1])
+    return None
+
+
 def accuracy(got, ref, idx):
     dz = np.abs(got["z"][idx] - ref["z"])
     return {"max_dz_mm": float(dz.max()), "p99_dz_mm": float(np.percentile(dz, 99)),
@@ -105,6 +117,8 @@ def main():
     ap.add_argument("--cpu-scan-points", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64/FP32/exact device lines")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run 3 device copies of 256 MiB (reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design only",
@@ -117,6 +131,14 @@ def main():
     if not torch.cuda.is_available():
         sys.exit("bench.py needs a HIP device")
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        src = torch.ones(256 << 18, dtype=torch.float32, device="cuda")  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 x 268435456 bytes read + written")
+        del src, dst
     spec = SPECS[args.cutter]
     t0 = time.perf_counter()
     tris = W.config1_mesh()
@@ -158,7 +180,8 @@ def main():
                 "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
                 "peak_unpacked_fp32": PEAK_FP32_UNPACKED, "frac_unpacked": achieved / PEAK_FP32_UNPACKED,
                 "pairs_per_launch": int(pairs), "flop_per_pair": F_PAIR[args.cutter],
-                "kernel_ms": sel_ms, "traffic": None}
+                "kernel_ms": sel_ms, "traffic": traffic_from_profile(args.cutter),
+                "traffic_source": f"profiles/drop_select_{args.cutter}_c1_hbm_bytes.md"}
 
     out = {"metric": METRIC, "value": n / (ms / 1e3), "unit": "CL pts/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
