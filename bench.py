@@ -180,6 +180,8 @@ def main():
                 "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
                 "peak_unpacked_fp32": PEAK_FP32_UNPACKED, "frac_unpacked": achieved / PEAK_FP32_UNPACKED,
                 "pairs_per_launch": int(pairs), "flop_per_pair": F_PAIR[args.cutter],
+                "wave_evals": int(st["wave_evals"]),
+                "lane_utilisation": pairs / (64.0 * st["wave_evals"]) if st["wave_evals"] else None,
                 "kernel_ms": sel_ms, "traffic": traffic_from_profile(args.cutter),
                 "traffic_source": f"profiles/drop_select_{args.cutter}_c1_hbm_bytes.md"}
 

@@ -83,6 +83,9 @@ typedef struct {
     double kernel_ms;       /* device time of the compute kernels */
     double select_ms;       /* device time of the first selection kernel (the
                                whole compute for exact64 and composites) */
+    uint64_t wave_evals;    /* wavefront evaluations of the per-pair contact
+                               code in the first selection kernel (pairs /
+                               (64 wave_evals) = lane utilisation) */
 } ocl_stats;
 
 /* Mesh: ntri triangles, 9 doubles each (p0, p1, p2).  bucket = the KD-tree

@@ -56,20 +56,23 @@ enum FastRec : int {
     R_EBF = R_E0 + 3 * kEdge,  // facet barycentric margin
     R_EZF,                     // facet height error bound
     R_FLAGS,                   // see FL_* bits
-    kFastRec = 68
+    R_ZCAP,                    // cull cap: max z + largest error bound / slack of the contacts
+    R_EMAX,                    // largest error bound of the triangle's contacts
+    kFastRec = 72              // 288 B in FP32: 16-byte aligned records
 };
 // per-edge values: direction (EX,EY), line constants C and A, 1/length, slope
 // M, start height Z1, three cutter constants K1..K3, the s^2 band below which
 // the contact height is ill-conditioned (S2E), the height slack of an
 // ambiguous contact (EAE) and the height error bound of the contact (EZE).
 enum EdgeVal : int { E_EX = 0, E_EY, E_C, E_A, E_INVL, E_M, E_Z1, E_K1, E_K2, E_K3, E_S2E, E_EAE, E_EZE };
-static_assert(R_FLAGS < kFastRec, "record overflow");
+static_assert(R_EMAX < kFastRec, "record overflow");
 
 enum : int { FL_VERT = 1, FL_HORIZ = 2, FL_EDEG0 = 8, FL_EHOR0 = 64 };
 
 // Local frame of a level-0 box: xy centre and mid height of its triangles.
 struct BoxOrigin {
-    double ox, oy, oz, pad;
+    double ox, oy, oz;
+    float hx, hy;  // half extents of the level-0 box around (ox, oy), rounded up
 };
 
 // Cutter constants of the selection pass (T precision)
@@ -86,6 +89,10 @@ struct SelParams {
     T EPOS;         // edge: position error of cc_u and u1 (segment band EPOS/L)
     T SQDM;         // sqrt(2 DM): worst-case error of s = sqrt(m) near m = 0
     T E;            // z error bound of a well-conditioned contact
+    T HPAD;         // rounding allowance of the profile lower bound (culling)
+    T CUT;          // cull margin: 2 x the largest contact error bound of the mesh + HPAD
+    int cull;       // culling by profile bound enabled (not for cones: their
+                    // facet-first rule makes the result order dependent)
 };
 
 // Error model shared by the host launcher and the record builder.  Lane
@@ -231,6 +238,10 @@ OCL_HD void build_record(const TriX& t, const CutterDev& cd, double ox, double o
     // facet height error bound (steep facets amplify the position rounding)
     const double ezf = 4 * u * (fabs(F0) + fabs(F0B) + 2.0 * (fabs(GX) + fabs(GY)) * xyb) + Eabs;
     out[R_EZF] = (T)ezf;
+    // largest error bound and ambiguity slack over the contacts (culling);
+    // vertex contacts use the cutter-wide constants of sel_params
+    double emax = fmax(ezf, fmax(Eabs, 2e-9));
+    double slk = fmax(emax, fmax(er.slack(1.0, Eabs), 2.0 * Eabs));
     // ---- vertices (heights relative to the cutter-specific base)
     for (int k = 0; k < 3; ++k) {
         out[R_P0X + 3 * k] = (T)p[k].x;
@@ -305,9 +316,14 @@ OCL_HD void build_record(const TriX& t, const CutterDev& cd, double ox, double o
         e[E_EZE] = (T)ez;
         e[E_S2E] = (T)er.s2band(kslope, ez);
         e[E_EAE] = (T)er.slack(kslope, ez);
+        emax = fmax(emax, ez);
+        slk = fmax(slk, fmax(ez, er.slack(kslope, ez)));
     }
     out[R_FLAGS] = (T)flags;
-    for (int q = R_FLAGS + 1; q < kFastRec; ++q) out[q] = (T)0;
+    // rounded up: cull tests compare against this cap
+    out[R_ZCAP] = (T)((bz1 + Eabs + slk) * (1.0 + 4.0 * u) + 4.0 * u);
+    out[R_EMAX] = (T)(emax * (1.0 + 4.0 * u));
+    for (int q = R_EMAX + 1; q < kFastRec; ++q) out[q] = (T)0;
 }
 
 // Selection constants for one cutter (host side, FP64 -> T).
@@ -337,7 +353,80 @@ OCL_HD SelParams<T> sel_params(const CutterDev& cd, const SelErr& er) {
     // converged Newton solution by up to ~1e-9; ties closer than that are
     // decided by the exact path.
     if (c.kind == K_BULL) P.E = (T)fmax((double)P.E, 2e-9);
+    P.HPAD = (T)(16.0 * er.u * R);
+    P.CUT = (T)INFINITY;  // set by the launcher once the records' largest error is known
+    P.cull = (c.kind == K_CYL || c.kind == K_BALL || c.kind == K_BULL) ? 1 : 0;
     return P;
+}
+
+// ------------------------------------------------------------ culling
+// Every contact of a triangle whose XY box lies at distance >= d from the
+// CL point has height <= maxz - h(d) (h = the cutter profile height, which
+// grows with the distance from the axis).  With the record's R_ZCAP (max z
+// plus the largest error bound and ambiguity slack of its contacts), a lane
+// whose final height is known to reach thr gains nothing from a triangle
+// with R_ZCAP - h(d) < thr - CUT: none of its contacts can lift, tie or
+// enter the ambiguity window of the winner.
+
+// squared XY distance from (x, y) to the box [x0, x1] x [y0, y1]
+template <class T>
+OCL_HD T box_d2(T x, T y, T x0, T x1, T y0, T y1) {
+    typedef FM<T> F;
+    const T dx = F::max_(F::max_(x0 - x, x - x1), (T)0);
+    const T dy = F::max_(F::max_(y0 - y, y - y1), (T)0);
+    return F::fma_(dx, dx, dy * dy);
+}
+
+// profile height at XY distance sqrt(d2), rounded down (0 for the cylinder)
+template <class T, int K>
+OCL_HD T prof_lower(T d2, const SelParams<T>& P) {
+    typedef FM<T> F;
+    T h = (T)0;
+    if (K == K_BALL) {
+        h = P.R - F::sqrt_(F::max_(P.R2 - d2, (T)0));
+    } else if (K == K_BULL) {
+        const T w = F::max_(F::sqrt_(d2) - P.r1, (T)0);
+        h = P.r2 - F::sqrt_(F::max_(P.r22 - w * w, (T)0));
+    }
+    return F::max_(h - P.HPAD, (T)0);
+}
+
+// Vertex j contact height (cutter base at the vertex, see sel_triangle) and
+// m = R^2 - q^2 (>= 0: the vertex is under the cutter).
+template <class T, int K>
+OCL_HD T vertex_height(const T* rec, int j, T x, T y, const SelParams<T>& P, T& m) {
+    typedef FM<T> F;
+    const T dx = rec[R_P0X + 3 * j] - x, dy = rec[R_P0Y + 3 * j] - y;
+    const T q2 = F::fma_(dx, dx, dy * dy);
+    m = P.R2 - q2;
+    switch (K) {
+    case K_CYL: return rec[R_P0Z + 3 * j];
+    case K_BALL: return rec[R_P0Z + 3 * j] + F::sqrt_(F::max_(m, (T)0));
+    case K_BULL: {
+        const T q = F::sqrt_(q2);
+        const T w = F::max_(q - P.r1, (T)0);
+        // flat part: z = p.z; torus: p.z - r2 + sqrt(r2^2 - w^2)  (base is p.z - r2)
+        return rec[R_P0Z + 3 * j] + F::sqrt_(F::max_(P.r22 - w * w, (T)0));
+    }
+    default: return rec[R_P0Z + 3 * j] - F::sqrt_(q2) * P.cot;  // cone
+    }
+}
+
+// Lower bound of a lane's final height from one triangle's vertices: a vertex
+// clearly under the cutter is a contact the reference always reaches or
+// exceeds (for Cyl/Ball/Bull the facet contact it may take instead is the
+// maximum over the triangle).
+template <class T, int K>
+OCL_HD T vertex_floor(const T* rec, T x, T y, const SelParams<T>& P, bool on) {
+    typedef FM<T> F;
+    T zc = -F::inf();
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        T m;
+        const T z = vertex_height<T, K>(rec, j, x, y, P, m);
+        zc = F::max_(zc, (on & (m > P.S2V)) ? z - P.E : -F::inf());
+    }
+    return zc;
 }
 
 // --------------------------------------------------------- lane state
@@ -426,22 +515,8 @@ OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>&
     bool vamb = false;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        const T dx = rec[R_P0X + 3 * j] - x, dy = rec[R_P0Y + 3 * j] - y;
-        const T q2 = F::fma_(dx, dx, dy * dy);
-        const T m = P.R2 - q2;
-        T z;
-        switch (K) {
-        case K_CYL: z = rec[R_P0Z + 3 * j]; break;
-        case K_BALL: z = rec[R_P0Z + 3 * j] + F::sqrt_(F::max_(m, (T)0)); break;
-        case K_BULL: {
-            const T q = F::sqrt_(q2);
-            const T w = F::max_(q - P.r1, (T)0);
-            // flat part: z = p.z; torus: p.z - r2 + sqrt(r2^2 - w^2)  (base is p.z - r2)
-            z = rec[R_P0Z + 3 * j] + F::sqrt_(F::max_(P.r22 - w * w, (T)0));
-            break;
-        }
-        default: z = rec[R_P0Z + 3 * j] - F::sqrt_(q2) * P.cot; break;  // cone
-        }
+        T m;
+        const T z = vertex_height<T, K>(rec, j, x, y, P, m);
         const bool val = ve & (m >= (T)0);
         const bool amb = ve & (m <= P.S2V) & (m >= -P.DM);
         S.zt[0] = F::max_(S.zt[0], val ? z + P.E : NEG);

@@ -33,6 +33,7 @@ struct MeshView {
     const double* tri;      // kRec doubles per triangle, rank order
     const uint32_t* order;  // rank -> input index
     uint32_t ntri;
+    uint32_t xcd_group;     // consecutive logical blocks per XCD (see xcd_block)
     BoxTree tree;
 };
 

@@ -55,6 +55,16 @@ struct DevBuf {
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Selection records of one precision: per-triangle records, per-box cull
+// caps and the mesh's largest contact error bound (read back once), cached
+// for the last cutter.
+struct RecSet {
+    DevBuf rec, cap, emax;
+    bool ok = false;
+    CutterDev key{};
+    double emax_h = 0.0;
+};
+
 }  // namespace ocl_amd
 
 struct ocl_mesh {
@@ -72,12 +82,12 @@ struct ocl_mesh {
     // per-call workspace
     ocl_amd::DevBuf keys, perm, hist, cursor, cl, out_z, out_cc, out_ty, out_tri, out_ft, counters, fastrec;
     ocl_amd::DevBuf esc_list, esc_list2, esc_flag, blocksum, qcount, cnt2, cnt3;
-    ocl_amd::DevBuf origins;     // double2 box centre per level-0 box
+    ocl_amd::DevBuf origins;     // frame (BoxOrigin) per level-0 box
+    ocl_amd::DevBuf zfloor;      // per CL point: floor of the final z handed to the escalation passes
     double zmid = 0, zhalf = 0, xyb = 0;
     // cached selection records (rebuilt when the cutter or precision changes)
-    ocl_amd::DevBuf rec32, rec64;
-    bool rec32_ok = false, rec64_ok = false;
-    ocl_amd::CutterDev rec32_cd{}, rec64_cd{};
+    ocl_amd::RecSet rs32, rs64;
+    uint32_t xcd_group = 1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_s0 = nullptr, ev_s1 = nullptr;
 
     ocl_amd::MeshView view() const {
@@ -85,6 +95,7 @@ struct ocl_mesh {
         v.tri = tri.as<const double>();
         v.order = order.as<const uint32_t>();
         v.ntri = (uint32_t)ntri;
+        v.xcd_group = xcd_group;
         v.tree.box = boxes.as<const float4>();
         v.tree.lvl = levels.as<const uint32_t>();
         v.tree.top = top;
@@ -160,57 +171,45 @@ __global__ void k_box_origin(const float4* __restrict__ box, uint32_t nbox, doub
 // counting sort by key.  Order inside a key is arbitrary; per-point results
 // do not depend on it.
 struct BinFrame {
-    double x0, y0, sx, sy;  // key = morton(clamp((x - x0) * sx), clamp((y - y0) * sy))
+    double x0, y0, sx, sy;  // cell = (clamp((x - x0) * sx), clamp((y - y0) * sy)) on a 256 x 256 grid
 };
 
-__device__ inline uint32_t spread8(uint32_t v) {
-    v &= 0xff;
-    v = (v | (v << 4)) & 0x0f0f;
-    v = (v | (v << 2)) & 0x3333;
-    v = (v | (v << 1)) & 0x5555;
-    return v;
+// Position of cell (x, y) along the Hilbert curve through the 256 x 256
+// grid (the classic iterative xy -> d conversion: per level, take the
+// quadrant and rotate/reflect the remaining bits into its frame).  Successive
+// keys are always edge-adjacent cells, so 64 consecutive CL points of the
+// sorted order form one compact patch: no wavefront straddles a jump across
+// the domain, as a Morton order's quadrant boundaries would make it.
+__device__ inline uint32_t hilbert256(uint32_t x, uint32_t y) {
+    uint32_t d = 0;
+    for (uint32_t s = 128; s > 0; s >>= 1) {
+        const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+        d += s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = 255u - x;
+                y = 255u - y;
+            }
+            const uint32_t t = x;
+            x = y;
+            y = t;
+        }
+    }
+    return d;
 }
 
-__global__ void k_cl_keys(const double* __restrict__ cl, uint32_t n, BinFrame f, uint32_t* __restrict__ keys,
-                          uint32_t* __restrict__ hist) {
+__global__ void k_cl_keys(const double* __restrict__ cl, uint32_t n, BinFrame f, uint32_t* __restrict__ keys) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double fx = (cl[3 * (size_t)i] - f.x0) * f.sx, fy = (cl[3 * (size_t)i + 1] - f.y0) * f.sy;
+    // NaN coordinates land in cell 0 (fmax/fmin drop the NaN)
     const uint32_t ix = (uint32_t)fmin(fmax(fx, 0.0), 255.0), iy = (uint32_t)fmin(fmax(fy, 0.0), 255.0);
-    const uint32_t k = spread8(ix) | (spread8(iy) << 1);
-    keys[i] = k;
-    atomicAdd(&hist[k], 1u);
+    keys[i] = hilbert256(ix, iy);
 }
 
-// exclusive scan of 65536 counters by one 1024-thread block
-__global__ void __launch_bounds__(1024) k_scan64k(uint32_t* __restrict__ hist) {
-    __shared__ uint32_t part[1024];
-    const int t = threadIdx.x;
-    uint32_t local[64];
-    uint32_t s = 0;
-    for (int k = 0; k < 64; ++k) {
-        local[k] = s;
-        s += hist[t * 64 + k];
-    }
-    part[t] = s;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        uint32_t v = (t >= o) ? part[t - o] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    const uint32_t base = part[t] - s;
-    for (int k = 0; k < 64; ++k) hist[t * 64 + k] = base + local[k];
-}
+hipError_t cl_sort_by_key(void* temp, size_t* temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
+                          uint32_t* perm, uint32_t n, hipStream_t st);
 
-__global__ void k_cl_scatter(const uint32_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ cursor,
-                             uint32_t* __restrict__ perm) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t slot = atomicAdd(&cursor[keys[i]], 1u);
-    perm[slot] = i;
-}
 
 // ------------------------------------------------------------ drop outputs
 struct DropOut {
@@ -222,7 +221,7 @@ struct DropOut {
 };
 
 struct Counters {
-    unsigned long long pairs, escalated, brent, nesc;
+    unsigned long long pairs, escalated, brent, nesc, evals;  // evals: wavefront runs of the contact code
 };
 
 __device__ inline TriX load_tri(const double* __restrict__ base, uint32_t k) {
@@ -274,7 +273,7 @@ __global__ void __launch_bounds__(256) k_drop_exact(MeshView m, CutterDev cd, co
                                                     const uint32_t* __restrict__ count, uint32_t n, DropOut o,
                                                     Counters* cnt) {
     const uint32_t n_eff = count ? *count : n;
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gid = xcd_block(m.xcd_group) * blockDim.x + threadIdx.x;
     const uint32_t wave_first = gid & ~63u;
     if (wave_first >= n_eff) return;
     const bool act = gid < n_eff;
@@ -323,6 +322,26 @@ __global__ void k_sel_prep(const double* __restrict__ tri, uint32_t ntri, const 
     build_record<T>(t, cd, o.ox, o.oy, o.oz, er, rec + (size_t)kFastRec * i);
 }
 
+// Per level-0 box: the largest cull cap of its records (box frame), and the
+// largest contact error bound of the mesh (atomic max of non-negative float
+// bits, which order like unsigned integers).
+template <class T>
+__global__ void k_box_cap(const T* __restrict__ rec, uint32_t ntri, uint32_t nbox, T* __restrict__ cap,
+                          unsigned int* __restrict__ emax_bits) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    T c = -FM<T>::inf();
+    float em = 0.0f;
+    const uint32_t e = min(ntri, (b + 1) * kLeaf);
+    for (uint32_t i = b * kLeaf; i < e; ++i) {
+        const T* r = rec + (size_t)kFastRec * i;
+        c = FM<T>::max_(c, r[R_ZCAP]);
+        em = fmaxf(em, __double2float_ru((double)r[R_EMAX]));
+    }
+    cap[b] = c;
+    atomicMax(emax_bits, __float_as_uint(em));
+}
+
 // Box origins: xy centre of the level-0 box, mid z of its triangles.
 __global__ void k_box_frames(const double* __restrict__ tri, uint32_t ntri, const float4* __restrict__ box,
                              uint32_t nbox, BoxOrigin* __restrict__ org) {
@@ -342,7 +361,8 @@ __global__ void k_box_frames(const double* __restrict__ tri, uint32_t ntri, cons
     o.ox = 0.5 * ((double)q.x + (double)q.z);
     o.oy = 0.5 * ((double)q.y + (double)q.w);
     o.oz = 0.5 * (z0 + z1);
-    o.pad = 0.0;
+    o.hx = __double2float_ru(0.5 * ((double)q.z - (double)q.x));
+    o.hy = __double2float_ru(0.5 * ((double)q.w - (double)q.y));
     org[b] = o;
 }
 
@@ -356,14 +376,18 @@ struct WaveState {
 // FP32-only mode: the selection result is written as is.
 template <class T, int K>
 __global__ void __launch_bounds__(256) k_drop_select(MeshView m, const T* __restrict__ rec,
-                                                     const BoxOrigin* __restrict__ org, SelParams<T> P, CutterDev cd,
+                                                     const BoxOrigin* __restrict__ org,
+                                                     const T* __restrict__ boxcap, SelParams<T> P, CutterDev cd,
                                                      const double* __restrict__ cl, const uint32_t* __restrict__ list,
                                                      const uint32_t* __restrict__ count, uint32_t n, DropOut o,
                                                      uint32_t* __restrict__ next, uint32_t* __restrict__ next_n,
-                                                     int refine, Counters* cnt) {
+                                                     double* __restrict__ zfloor, int refine, Counters* cnt) {
     typedef FM<T> F;
+    // per-wave LDS copy of the records of the level-0 box being evaluated
+    __shared__ __align__(16) T srec[4][kLeaf * kFastRec];
+    T* my = srec[threadIdx.x >> 6];
     const uint32_t n_eff = count ? *count : n;
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gid = xcd_block(m.xcd_group) * blockDim.x + threadIdx.x;
     const uint32_t wave_first = gid & ~63u;
     if (wave_first >= n_eff) return;
     const bool act = gid < n_eff;
@@ -378,14 +402,63 @@ __global__ void __launch_bounds__(256) k_drop_select(MeshView m, const T* __rest
     const float qy1 = uniform_f(__double2float_ru(wave_max(act ? y : -INFINITY) + R));
     Sel<T> S;
     WaveState<T> W;
+    uint32_t evals = 0;
     double ozw = 0.0;
     bool framed = false;
     sel_init(S, (T)0);
     W.zb = (T)0;
     for (int i = 0; i < 4; ++i) W.zt[i] = -F::inf();
     W.za = -F::inf();
-    walk_boxes(m.tree, qx0, qy0, qx1, qy1, [&](uint32_t b) {
-        const BoxOrigin bo = org[b];
+    // XY hull of the wave's points
+    const double hull_x0 = wave_min(act ? x : INFINITY), hull_x1 = wave_max(act ? x : -INFINITY);
+    const double hull_y0 = wave_min(act ? y : INFINITY), hull_y1 = wave_max(act ? y : -INFINITY);
+    // Floor of every lane's final height from the vertices of the triangles
+    // right under the wavefront's points (boxes overlapping their XY hull).
+    T zc = -F::inf();
+    if (K != K_CONE && P.cull) {
+        const float tx0 = uniform_f(__double2float_rd(hull_x0));
+        const float ty0 = uniform_f(__double2float_rd(hull_y0));
+        const float tx1 = uniform_f(__double2float_ru(hull_x1));
+        const float ty1 = uniform_f(__double2float_ru(hull_y1));
+        walk_boxes(m.tree, tx0, ty0, tx1, ty1, [&](uint32_t b) {
+            const BoxOrigin bo = org[b];
+            if (!framed) {
+                framed = true;
+                ozw = bo.oz;
+                W.zb = (T)(z0 - ozw);
+            }
+            const T dz = (T)(bo.oz - ozw);
+            const T xl = (T)(x - bo.ox), yl = (T)(y - bo.oy);
+            const uint32_t e = min(m.ntri, (b + 1) * kLeaf);
+            for (uint32_t k = b * kLeaf; k < e; ++k)
+                zc = F::max_(zc, vertex_floor<T, K>(rec + (size_t)kFastRec * uniform_u(k), xl, yl, P, act) + dz);
+        });
+    }
+    // lane j holds the frame and cull cap of box base+j of the level-0 block
+    // being walked, loaded together when the walk enters the block
+    const uint32_t nbox0 = (m.ntri + kLeaf - 1) / kLeaf;
+    double pox = 0.0, poy = 0.0, poz = 0.0;
+    float phx = 0.0f, phy = 0.0f;
+    T pcap = (T)0;
+    auto block = [&](uint32_t base) {
+        const uint32_t bj = base + lane_id();
+        if (lane_id() < (uint32_t)kFan && bj < nbox0) {
+            const BoxOrigin o = org[bj];
+            pox = o.ox;
+            poy = o.oy;
+            poz = o.oz;
+            phx = o.hx;
+            phy = o.hy;
+            pcap = boxcap[bj];
+        }
+    };
+    walk_boxes(m.tree, qx0, qy0, qx1, qy1, block, [&](uint32_t b, uint32_t jb) {
+        BoxOrigin bo;
+        bo.ox = readlane_d(pox, jb);
+        bo.oy = readlane_d(poy, jb);
+        bo.oz = readlane_d(poz, jb);
+        bo.hx = readlane_f(phx, jb);
+        bo.hy = readlane_f(phy, jb);
         if (!framed) {
             framed = true;
             ozw = bo.oz;
@@ -393,23 +466,77 @@ __global__ void __launch_bounds__(256) k_drop_select(MeshView m, const T* __rest
         }
         const T dz = (T)(bo.oz - ozw);
         const T xl = (T)(x - bo.ox), yl = (T)(y - bo.oy);
+        if (K != K_CONE && P.cull) {
+            // no lane can gain anything from this box
+            const T cap = (sizeof(T) == 4) ? (T)readlane_f((float)pcap, jb) : (T)readlane_d((double)pcap, jb);
+            const T d2 = box_d2<T>(xl, yl, -(T)bo.hx, (T)bo.hx, -(T)bo.hy, (T)bo.hy);
+            const bool rel = act & (cap + dz - prof_lower<T, K>(d2, P) >= F::max_(W.zb, zc) - P.CUT);
+            if (!__any(rel)) return;
+        }
         const T lx0 = (T)((double)qx0 - bo.ox), lx1 = (T)((double)qx1 - bo.ox);
         const T ly0 = (T)((double)qy0 - bo.oy), ly1 = (T)((double)qy1 - bo.oy);
+        // stage the box's records in LDS with coalesced 16-byte loads: all
+        // of them are in flight together instead of one dependent load chain
+        // per triangle
+        const uint32_t t0 = b * kLeaf;
+        const uint32_t nt = min(m.ntri - t0, (uint32_t)kLeaf);
+        {
+            constexpr uint32_t kVec = kLeaf * kFastRec * sizeof(T) / 16;
+            constexpr uint32_t kIt = (kVec + 63) / 64;
+            const uint32_t nv = nt * (uint32_t)(kFastRec * sizeof(T) / 16);
+            const float4* src = reinterpret_cast<const float4*>(rec + (size_t)kFastRec * t0);
+            float4* dst = reinterpret_cast<float4*>(my);
+            // clamped (always in-bounds) loads first, then the stores, so the
+            // loads of one box are all in flight together
+#pragma unroll
+            for (uint32_t it = 0; it < kIt; ++it) {
+                const uint32_t i = lane_id() + 64 * it;
+                const float4 v = src[min(i, nv - 1)];
+                if (i < nv) dst[i] = v;
+            }
+            wave_lds_sync();
+        }
         // enter the box frame
         const uint32_t bt_in = S.bt;
         S.zb = W.zb - dz;
         for (int i = 0; i < 4; ++i) S.zt[i] = -F::inf();
         S.za = -F::inf();
-        const uint32_t e = min(m.ntri, (b + 1) * kLeaf);
-        for (uint32_t k = b * kLeaf; k < e; ++k) {
-            const uint32_t ku = uniform_u(k);
-            const T* r = rec + (size_t)kFastRec * ku;
-            // one vector of the triangle box, tested without short-circuits
+        // wave-level pre-test, one lane per triangle of the box: its XY box
+        // must meet the wave's query box, and (profile cull) its cap at the
+        // distance to the hull of the wave's points must reach the lowest
+        // threshold of any lane
+        uint32_t tmask;
+        {
+            const uint32_t jl = lane_id();
+            bool keep = false;
+            T thr_min = F::inf();
+            if (K != K_CONE && P.cull) thr_min = wave_min_t<T>(act ? F::max_(S.zb, zc - dz) : F::inf()) - P.CUT;
+            if (jl < nt) {
+                const T* r = my + kFastRec * jl;
+                const T bx0 = r[R_BX0], bx1 = r[R_BX1], by0 = r[R_BY0], by1 = r[R_BY1];
+                keep = !((bx1 < lx0) | (bx0 > lx1) | (by1 < ly0) | (by0 > ly1));
+                if (K != K_CONE && P.cull) {
+                    const T hx0 = (T)(hull_x0 - bo.ox), hx1 = (T)(hull_x1 - bo.ox);
+                    const T hy0 = (T)(hull_y0 - bo.oy), hy1 = (T)(hull_y1 - bo.oy);
+                    const T ddx = F::max_(F::max_(bx0 - hx1, hx0 - bx1), (T)0);
+                    const T ddy = F::max_(F::max_(by0 - hy1, hy0 - by1), (T)0);
+                    keep = keep & (r[R_ZCAP] - prof_lower<T, K>(F::fma_(ddx, ddx, ddy * ddy), P) >= thr_min);
+                }
+            }
+            tmask = (uint32_t)__ballot(keep);
+        }
+        for (; tmask; tmask &= tmask - 1) {
+            const uint32_t j = (uint32_t)__builtin_ctz(tmask);
+            const uint32_t ku = uniform_u(t0 + j);
+            const T* r = my + kFastRec * j;
             const T bx0 = r[R_BX0], bx1 = r[R_BX1], by0 = r[R_BY0], by1 = r[R_BY1];
-            const bool out = (bx1 < lx0) | (bx0 > lx1) | (by1 < ly0) | (by0 > ly1);
-            if (out) continue;
-            const bool on = act & !((bx1 < xl - P.R) | (bx0 > xl + P.R) | (by1 < yl - P.R) | (by0 > yl + P.R));
+            bool on = act & !((bx1 < xl - P.R) | (bx0 > xl + P.R) | (by1 < yl - P.R) | (by0 > yl + P.R));
+            if (K != K_CONE && P.cull) {
+                const T d2 = box_d2<T>(xl, yl, bx0, bx1, by0, by1);
+                on = on & (r[R_ZCAP] - prof_lower<T, K>(d2, P) >= F::max_(S.zb, zc - dz) - P.CUT);
+            }
             if (!__any(on)) continue;
+            ++evals;
             sel_triangle<T, K>(r, ku, xl, yl, P, S, on);
         }
         // leave the box frame
@@ -424,6 +551,7 @@ __global__ void __launch_bounds__(256) k_drop_select(MeshView m, const T* __rest
     D.za = W.za;
     const T z0w = (T)(z0 - ozw);
     bool up = refine && framed && sel_escalate(D, z0w, P);
+    bool rfail = false;
     DropLane L = fresh_lane(x, y, z0);
     unsigned long long brent = 0;
     if (act && !up && D.bt != 0xffffffffu) {
@@ -438,6 +566,7 @@ __global__ void __launch_bounds__(256) k_drop_select(MeshView m, const T* __rest
             if (X.type == CC_NONE || feat_slot(X.feat) != feat_slot(D.bf) || !(X.z > z0) ||
                 fabs(X.z - zsel) > 4.0 * ((double)D.eb + (double)P.E)) {
                 up = true;
+                rfail = true;
             } else {
                 L.z = X.z;
                 L.cc = X.cc;
@@ -456,6 +585,13 @@ __global__ void __launch_bounds__(256) k_drop_select(MeshView m, const T* __rest
     }
     if (!refine) up = false;
     const bool push = act && up;
+    if (push && zfloor) {
+        // Floor of the final height for the next passes: the vertex floor,
+        // and the winner's value when its validity was unambiguous.
+        T fl = zc;
+        if (K != K_CONE && !D.bamb && !rfail && D.bt != 0xffffffffu) fl = F::max_(fl, D.zb - (T)2 * D.eb - P.E);
+        zfloor[src] = (K == K_CONE || !framed) ? -INFINITY : ozw + (double)fl - 1e-9 * (1.0 + fabs(ozw));
+    }
     const unsigned long long bal = __ballot(push);
     if (bal) {
         uint32_t base = 0;
@@ -468,6 +604,7 @@ __global__ void __launch_bounds__(256) k_drop_select(MeshView m, const T* __rest
         wave_add(&cnt->pairs, act ? S.pairs : 0);
         wave_add(&cnt->escalated, (act && up) ? 1 : 0);
         wave_add(&cnt->brent, brent);
+        wave_add(&cnt->evals, lane_id() == 0 ? evals : 0);
     }
 }
 
@@ -480,23 +617,61 @@ __global__ void __launch_bounds__(256) k_drop_select(MeshView m, const T* __rest
 // batch in parallel and folds it into the point's state in rank order.
 constexpr int kCoopWaves = 4;  // wavefronts per 256-thread block
 
-__device__ inline uint32_t lane_id() { return threadIdx.x & 63u; }
 
 // Gather driver: calls batch(list, cnt) with cnt <= 64 candidate triangle
 // indices in rank order.  Candidates are triangles whose XY box overlaps
 // [x-R, x+R] x [y-R, y+R] (the reference overlaps() test, exact FP64).
-template <class Batch>
-__device__ inline void coop_gather(const MeshView& m, uint32_t* qbuf, double x, double y, double R, Batch&& batch) {
+// BoxCull: optional per-level-0-box cull (cap = box cull cap in the box frame,
+// frames in org); a box is skipped when cap + oz - h(distance) < thr().
+struct BoxCull {
+    const double* cap;
+    const BoxOrigin* org;
+};
+
+template <class Thr, class Keep, class Batch>
+__device__ inline void coop_gather(const MeshView& m, uint32_t* qbuf, double x, double y, const CutterP& c,
+                                   const BoxCull& bc, Thr&& thr, Keep&& keep, Batch&& batch) {
     const uint32_t lane = lane_id();
+    const double R = c.radius;
     const float qx0 = __double2float_rd(x - R), qy0 = __double2float_rd(y - R);
     const float qx1 = __double2float_ru(x + R), qy1 = __double2float_ru(y + R);
     uint32_t qn = 0;
-    walk_boxes(m.tree, qx0, qy0, qx1, qy1, [&](uint32_t b) {
+    const uint32_t nbox0 = (m.ntri + kLeaf - 1) / kLeaf;
+    double pcap = INFINITY, pox = 0.0, poy = 0.0, poz = 0.0;
+    float phx = 0.0f, phy = 0.0f;
+    auto block = [&](uint32_t base) {
+        const uint32_t bj = base + lane;
+        if (bc.cap && lane < (uint32_t)kFan && bj < nbox0) {
+            const BoxOrigin o = bc.org[bj];
+            pcap = bc.cap[bj];
+            pox = o.ox;
+            poy = o.oy;
+            poz = o.oz;
+            phx = o.hx;
+            phy = o.hy;
+        }
+    };
+    walk_boxes(m.tree, qx0, qy0, qx1, qy1, block, [&](uint32_t b, uint32_t jb) {
+        if (bc.cap) {
+            // box-level cull: every contact in the box is below cap + oz - h(d)
+            const double ox = readlane_d(pox, jb), oy = readlane_d(poy, jb), oz = readlane_d(poz, jb);
+            const double hx = readlane_f(phx, jb), hy = readlane_f(phy, jb);
+            const double dx = fmax(fabs(x - ox) - hx, 0.0), dy = fmax(fabs(y - oy) - hy, 0.0);
+            const double d2 = dx * dx + dy * dy;
+            double h = 0.0;
+            if (c.kind == K_BALL) {
+                h = R - sqrt(fmax(R * R - d2, 0.0));
+            } else if (c.kind == K_BULL) {
+                const double w = fmax(sqrt(d2) - c.r1, 0.0);
+                h = c.r2 - sqrt(fmax(c.r2 * c.r2 - w * w, 0.0));
+            }
+            if (readlane_d(pcap, jb) + oz - fmax(h - 1e-12 * R, 0.0) < thr()) return;
+        }
         const uint32_t k = b * kLeaf + lane;
         bool cand = false;
         if (lane < (uint32_t)kLeaf && k < m.ntri) {
             const TriX t = load_tri(m.tri, k);
-            cand = cutter_overlaps(R, x, y, t);
+            cand = cutter_overlaps(R, x, y, t) && keep(k, t);
         }
         const unsigned long long bal = __ballot(cand);
         const uint32_t pos = qn + __popcll(bal & ((1ull << lane) - 1ull));
@@ -516,6 +691,24 @@ __device__ inline void coop_gather(const MeshView& m, uint32_t* qbuf, double x, 
     if (qn > 0) batch(qbuf, qn);
 }
 
+// Upper bound of every contact height of triangle t for a CL point at
+// (x, y): max z minus the cutter profile height at the XY distance to the
+// triangle's box (Cyl/Ball/Bull; the profile grows with the distance).
+__device__ inline double contact_cap(const CutterP& c, const TriX& t, double x, double y) {
+    const double bx0 = fmin(fmin(t.p[0].x, t.p[1].x), t.p[2].x), bx1 = fmax(fmax(t.p[0].x, t.p[1].x), t.p[2].x);
+    const double by0 = fmin(fmin(t.p[0].y, t.p[1].y), t.p[2].y), by1 = fmax(fmax(t.p[0].y, t.p[1].y), t.p[2].y);
+    const double dx = fmax(fmax(bx0 - x, x - bx1), 0.0), dy = fmax(fmax(by0 - y, y - by1), 0.0);
+    const double d2 = dx * dx + dy * dy;
+    double h = 0.0;
+    if (c.kind == K_BALL) {
+        h = c.radius - sqrt(fmax(c.radius * c.radius - d2, 0.0));
+    } else if (c.kind == K_BULL) {
+        const double w = fmax(sqrt(d2) - c.r1, 0.0);
+        h = c.r2 - sqrt(fmax(c.r2 * c.r2 - w * w, 0.0));
+    }
+    return t.maxz - fmax(h - 1e-12 * c.radius, 0.0);
+}
+
 // Exact FP64 tier: reproduces the reference's sequential dropCutter fold.
 // Per triangle the lane computes, from z = -inf, the facet (cone: tip)
 // contact, the cone rim contact and the best vertex/edge contact; the fold
@@ -524,7 +717,10 @@ __device__ inline void coop_gather(const MeshView& m, uint32_t* qbuf, double x, 
 // edges; the cone rim is tried only when the tip did not lift).
 __global__ void __launch_bounds__(256) k_drop_coop_exact(MeshView m, CutterDev cd, const double* __restrict__ cl,
                                                          const uint32_t* __restrict__ list,
-                                                         const uint32_t* __restrict__ count, DropOut o,
+                                                         const uint32_t* __restrict__ count,
+                                                         const double* __restrict__ zfloor,
+                                                         const double* __restrict__ boxcap,
+                                                         const BoxOrigin* __restrict__ org, DropOut o,
                                                          Counters* cnt) {
     __shared__ uint32_t qmem[kCoopWaves][80];
     const uint32_t lane = lane_id();
@@ -533,7 +729,7 @@ __global__ void __launch_bounds__(256) k_drop_coop_exact(MeshView m, CutterDev c
     const uint32_t n_eff = *count;
     const uint32_t nwaves = gridDim.x * kCoopWaves;
     unsigned long long brent = 0;
-    for (uint32_t w = blockIdx.x * kCoopWaves + wv; w < n_eff; w += nwaves) {
+    for (uint32_t w = xcd_block(m.xcd_group) * kCoopWaves + wv; w < n_eff; w += nwaves) {
         const uint32_t src = uniform_u(list[w]);
         const double x = cl[3 * (size_t)src], y = cl[3 * (size_t)src + 1], z0 = cl[3 * (size_t)src + 2];
         const CutterP& c = cd.c;
@@ -542,7 +738,19 @@ __global__ void __launch_bounds__(256) k_drop_coop_exact(MeshView m, CutterDev c
         int win_part = -1;
         V3 win_cc = v3(0, 0, 0);
         int win_type = CC_NONE, win_feat = FT_NONE;
-        coop_gather(m, qbuf, x, y, c.radius, [&](uint32_t* q, uint32_t nb) {
+        // Cyl/Ball/Bull: the reference's result is the maximum over the
+        // triangles' own drops, so a triangle whose contacts all stay below
+        // a known floor of that maximum cannot lift or tie.  Cones are
+        // order dependent and never skip.
+        const bool cull = zfloor && (c.kind == K_CYL || c.kind == K_BALL || c.kind == K_BULL);
+        const double zf = cull ? zfloor[src] : -INFINITY;
+        auto thr = [&]() {
+            const double t = fmax(zf, zr);
+            return t - 1e-9 * (1.0 + fabs(t));
+        };
+        auto keep = [&](uint32_t, const TriX& t) { return !cull || contact_cap(c, t, x, y) >= thr(); };
+        const BoxCull bcl{cull ? boxcap : nullptr, org};
+        coop_gather(m, qbuf, x, y, c, bcl, thr, keep, [&](uint32_t* q, uint32_t nb) {
             const bool has = lane < nb;
             const uint32_t k = has ? q[lane] : 0u;
             DropLane A = fresh_lane(x, y, -INFINITY), B = fresh_lane(x, y, -INFINITY), V = fresh_lane(x, y, -INFINITY);
@@ -613,7 +821,9 @@ __global__ void __launch_bounds__(256) k_drop_coop_select(MeshView m, const doub
                                                           const BoxOrigin* __restrict__ org, SelParams<double> P,
                                                           CutterDev cd, const double* __restrict__ cl,
                                                           const uint32_t* __restrict__ list,
-                                                          const uint32_t* __restrict__ count, DropOut o,
+                                                          const uint32_t* __restrict__ count,
+                                                          double* __restrict__ zfloor,
+                                                          const double* __restrict__ boxcap, DropOut o,
                                                           uint32_t* __restrict__ next, uint32_t* __restrict__ next_n,
                                                           Counters* cnt) {
     __shared__ uint32_t qmem[kCoopWaves][80];
@@ -623,7 +833,7 @@ __global__ void __launch_bounds__(256) k_drop_coop_select(MeshView m, const doub
     const uint32_t n_eff = *count;
     const uint32_t nwaves = gridDim.x * kCoopWaves;
     unsigned long long brent = 0, nesc = 0;
-    for (uint32_t w = blockIdx.x * kCoopWaves + wv; w < n_eff; w += nwaves) {
+    for (uint32_t w = xcd_block(m.xcd_group) * kCoopWaves + wv; w < n_eff; w += nwaves) {
         const uint32_t src = uniform_u(list[w]);
         const double x = cl[3 * (size_t)src], y = cl[3 * (size_t)src + 1], z0 = cl[3 * (size_t)src + 2];
         // wave state in the global z frame
@@ -631,7 +841,16 @@ __global__ void __launch_bounds__(256) k_drop_coop_select(MeshView m, const doub
         uint32_t bt = 0xffffffffu;
         int bf = FT_NONE;
         bool bamb = false;
-        coop_gather(m, qbuf, x, y, cd.c.radius, [&](uint32_t* q, uint32_t nb) {
+        // floor of the final height handed down by the first pass: triangles
+        // whose cull cap stays below it cannot lift, tie or be ambiguous
+        const double zf = zfloor[src];
+        auto thr = [&]() { return fmax(zf, zb) - (double)P.CUT; };
+        auto keep = [&](uint32_t k, const TriX& t) {
+            const double cap = rec[(size_t)kFastRec * k + R_ZCAP] - t.maxz;  // slack above max z
+            return contact_cap(cd.c, t, x, y) + cap + org[k / kLeaf].oz >= thr();
+        };
+        const BoxCull bcl{boxcap, org};
+        coop_gather(m, qbuf, x, y, cd.c, bcl, thr, keep, [&](uint32_t* q, uint32_t nb) {
             const bool has = lane < nb;
             const uint32_t k = has ? q[lane] : 0u;
             Sel<double> S;
@@ -667,6 +886,7 @@ __global__ void __launch_bounds__(256) k_drop_coop_select(MeshView m, const doub
         for (int i = 0; i < 4; ++i) D.zt[i] = zt[i];
         D.za = za;
         bool up = sel_escalate(D, z0, P);
+        bool rfail = false;
         DropLane L = fresh_lane(x, y, z0);
         if (!up && bt != 0xffffffffu) {
             // exact refine of the selected feature (lane 0's result is used)
@@ -677,13 +897,20 @@ __global__ void __launch_bounds__(256) k_drop_coop_select(MeshView m, const doub
             exact_feature(cd.c, X, t, bf, it);
             brent += (lane == 0) ? it : 0;
             if (X.type == CC_NONE || feat_slot(X.feat) != feat_slot(bf) || !(X.z > z0) ||
-                fabs(X.z - zb) > 4.0 * (eb + (double)P.E))
+                fabs(X.z - zb) > 4.0 * (eb + (double)P.E)) {
                 up = true;
-            else
+                rfail = true;
+            } else {
                 L = X;
+            }
         }
         if (up) {
-            if (lane == 0) next[atomicAdd(next_n, 1u)] = src;
+            if (lane == 0) {
+                // an unambiguously valid winner is a floor for the exact pass
+                const double fl = (bamb || rfail || bt == 0xffffffffu) ? zf : fmax(zf, zb - 2.0 * eb - (double)P.E);
+                zfloor[src] = fl;
+                next[atomicAdd(next_n, 1u)] = src;
+            }
             nesc += (lane == 0);
         } else if (lane == 0) {
             write_drop(o, m.order, src, L);
@@ -799,31 +1026,46 @@ static SelParams<T> make_params(const CutterDev& cd, const SelErr& er) {
 }
 
 template <class T>
-static int ensure_records(ocl_mesh* m, const CutterDev& cd, DevBuf& buf, bool& ok, CutterDev& key, const SelErr& er,
-                          hipStream_t st) {
-    if (ok && std::memcmp(&key, &cd, sizeof(cd)) == 0) return OCL_OK;
-    int rc = buf.ensure(sizeof(T) * kFastRec * m->ntri);
-    if (rc) return rc;
-    const uint32_t n = (uint32_t)m->ntri;
+static int ensure_records(ocl_mesh* m, const CutterDev& cd, RecSet& rs, const SelErr& er, hipStream_t st) {
+    if (rs.ok && std::memcmp(&rs.key, &cd, sizeof(cd)) == 0) return OCL_OK;
+    rs.ok = false;
+    const uint32_t n = (uint32_t)m->ntri, nbox = m->cnt[0];
+    int rc;
+    if ((rc = rs.rec.ensure(sizeof(T) * kFastRec * m->ntri)) || (rc = rs.cap.ensure(sizeof(T) * nbox + 16)) ||
+        (rc = rs.emax.ensure(16)))
+        return rc;
     k_sel_prep<T><<<(n + 255) / 256, 256, 0, st>>>(m->tri.as<double>(), n, m->origins.as<BoxOrigin>(), cd, er,
-                                                   buf.as<T>());
+                                                   rs.rec.as<T>());
+    HIP_TRY(hipMemsetAsync(rs.emax.p, 0, 4, st));
+    k_box_cap<T><<<(nbox + 255) / 256, 256, 0, st>>>(rs.rec.as<T>(), n, nbox, rs.cap.as<T>(),
+                                                     rs.emax.as<unsigned int>());
     HIP_TRY(hipGetLastError());
-    key = cd;
-    ok = true;
+    float em = 0.0f;
+    HIP_TRY(hipMemcpyAsync(&em, rs.emax.p, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    rs.emax_h = em;
+    rs.key = cd;
+    rs.ok = true;
     return OCL_OK;
 }
 
 template <class T>
-static int launch_select(ocl_mesh* m, const MeshView& mv, const T* rec, const SelParams<T>& P, const CutterDev& cd,
-                         const double* d_cl, const uint32_t* list, uint32_t n, const DropOut& o, uint32_t* next,
-                         uint32_t* next_n, int refine, Counters* cnt, hipStream_t st) {
-    const uint32_t nb = (n + 255) / 256;
+static void set_cut(SelParams<T>& P, const RecSet& rs) {
+    P.CUT = (T)(2.0 * rs.emax_h * (1.0 + 1e-6) + (double)P.HPAD);
+}
+
+template <class T>
+static int launch_select(ocl_mesh* m, const MeshView& mv, const T* rec, const T* cap, const SelParams<T>& P,
+                         const CutterDev& cd, const double* d_cl, const uint32_t* list, uint32_t n, const DropOut& o,
+                         uint32_t* next, uint32_t* next_n, double* zfloor, int refine, Counters* cnt,
+                         hipStream_t st) {
+    const uint32_t nb = xcd_grid((n + 255) / 256, mv.xcd_group);
     const BoxOrigin* org = m->origins.as<BoxOrigin>();
     switch (cd.c.kind) {
-    case K_CYL: k_drop_select<T, K_CYL><<<nb, 256, 0, st>>>(mv, rec, org, P, cd, d_cl, list, nullptr, n, o, next, next_n, refine, cnt); break;
-    case K_BALL: k_drop_select<T, K_BALL><<<nb, 256, 0, st>>>(mv, rec, org, P, cd, d_cl, list, nullptr, n, o, next, next_n, refine, cnt); break;
-    case K_BULL: k_drop_select<T, K_BULL><<<nb, 256, 0, st>>>(mv, rec, org, P, cd, d_cl, list, nullptr, n, o, next, next_n, refine, cnt); break;
-    case K_CONE: k_drop_select<T, K_CONE><<<nb, 256, 0, st>>>(mv, rec, org, P, cd, d_cl, list, nullptr, n, o, next, next_n, refine, cnt); break;
+    case K_CYL: k_drop_select<T, K_CYL><<<nb, 256, 0, st>>>(mv, rec, org, cap, P, cd, d_cl, list, nullptr, n, o, next, next_n, zfloor, refine, cnt); break;
+    case K_BALL: k_drop_select<T, K_BALL><<<nb, 256, 0, st>>>(mv, rec, org, cap, P, cd, d_cl, list, nullptr, n, o, next, next_n, zfloor, refine, cnt); break;
+    case K_BULL: k_drop_select<T, K_BULL><<<nb, 256, 0, st>>>(mv, rec, org, cap, P, cd, d_cl, list, nullptr, n, o, next, next_n, zfloor, refine, cnt); break;
+    case K_CONE: k_drop_select<T, K_CONE><<<nb, 256, 0, st>>>(mv, rec, org, cap, P, cd, d_cl, list, nullptr, n, o, next, next_n, zfloor, refine, cnt); break;
     default: return fail(OCL_E_UNSUPPORTED, "selection pass has no composite specialisation");
     }
     HIP_TRY(hipGetLastError());
@@ -841,36 +1083,42 @@ static int launch_drop_fast(ocl_mesh* m, const MeshView& mv, const CutterDev& cd
                             int precision, const DropOut& o, Counters* cnt, hipStream_t st, bool timed) {
     int rc;
     const uint32_t* perm = m->perm.as<uint32_t>();
-    if ((rc = m->esc_list.ensure(4ull * n + 64)) || (rc = m->esc_list2.ensure(4ull * n + 64))) return rc;
+    if ((rc = m->esc_list.ensure(4ull * n + 64)) || (rc = m->esc_list2.ensure(4ull * n + 64)) ||
+        (rc = m->zfloor.ensure(8ull * n + 64)))
+        return rc;
+    double* zfloor = m->zfloor.as<double>();
     uint32_t* q1 = m->esc_list.as<uint32_t>();
     uint32_t* q2 = m->esc_list2.as<uint32_t>();
     uint32_t* qcount = m->qcount.as<uint32_t>();
     HIP_TRY(hipMemsetAsync(qcount, 0, 16, st));
     const double se = env_scale("OCL_SEL_E_SCALE", 1.0);
     const SelErr er64 = make_err<double>(cd, se);
-    const SelParams<double> P64 = make_params<double>(cd, er64);
-    if ((rc = ensure_records<double>(m, cd, m->rec64, m->rec64_ok, m->rec64_cd, er64, st))) return rc;
+    SelParams<double> P64 = make_params<double>(cd, er64);
+    if ((rc = ensure_records<double>(m, cd, m->rs64, er64, st))) return rc;
+    set_cut(P64, m->rs64);
     const BoxOrigin* org = m->origins.as<BoxOrigin>();
     uint32_t* exact_list = q1;
     uint32_t* exact_n = qcount;
     if (precision == OCL_PREC_FP32 || precision == OCL_PREC_MIXED) {
         const SelErr er32 = make_err<float>(cd, se);
-        const SelParams<float> P32 = make_params<float>(cd, er32);
-        if ((rc = ensure_records<float>(m, cd, m->rec32, m->rec32_ok, m->rec32_cd, er32, st))) return rc;
+        SelParams<float> P32 = make_params<float>(cd, er32);
+        if ((rc = ensure_records<float>(m, cd, m->rs32, er32, st))) return rc;
+        set_cut(P32, m->rs32);
         const int refine = precision == OCL_PREC_MIXED;
         if (timed) HIP_TRY(hipEventRecord(m->ev_s0, st));
-        if ((rc = launch_select<float>(m, mv, m->rec32.as<float>(), P32, cd, d_cl, perm, n, o, q1, qcount, refine, cnt, st)))
+        if ((rc = launch_select<float>(m, mv, m->rs32.rec.as<float>(), m->rs32.cap.as<float>(), P32, cd, d_cl, perm, n,
+                                       o, q1, qcount, zfloor, refine, cnt, st)))
             return rc;
         if (timed) HIP_TRY(hipEventRecord(m->ev_s1, st));
         if (!refine) return OCL_OK;
         if (cd.c.kind != K_CONE) {
             // FP64 selection of the undecided points, one wavefront per point
             Counters* c2 = m->cnt2.as<Counters>();
-            const double* r64 = m->rec64.as<double>();
+            const double* r64 = m->rs64.rec.as<double>();
             switch (cd.c.kind) {
-            case K_CYL: k_drop_coop_select<K_CYL><<<kCoopBlocks, 256, 0, st>>>(mv, r64, org, P64, cd, d_cl, q1, qcount, o, q2, qcount + 1, c2); break;
-            case K_BALL: k_drop_coop_select<K_BALL><<<kCoopBlocks, 256, 0, st>>>(mv, r64, org, P64, cd, d_cl, q1, qcount, o, q2, qcount + 1, c2); break;
-            default: k_drop_coop_select<K_BULL><<<kCoopBlocks, 256, 0, st>>>(mv, r64, org, P64, cd, d_cl, q1, qcount, o, q2, qcount + 1, c2); break;
+            case K_CYL: k_drop_coop_select<K_CYL><<<kCoopBlocks, 256, 0, st>>>(mv, r64, org, P64, cd, d_cl, q1, qcount, zfloor, m->rs64.cap.as<double>(), o, q2, qcount + 1, c2); break;
+            case K_BALL: k_drop_coop_select<K_BALL><<<kCoopBlocks, 256, 0, st>>>(mv, r64, org, P64, cd, d_cl, q1, qcount, zfloor, m->rs64.cap.as<double>(), o, q2, qcount + 1, c2); break;
+            default: k_drop_coop_select<K_BULL><<<kCoopBlocks, 256, 0, st>>>(mv, r64, org, P64, cd, d_cl, q1, qcount, zfloor, m->rs64.cap.as<double>(), o, q2, qcount + 1, c2); break;
             }
             HIP_TRY(hipGetLastError());
             exact_list = q2;
@@ -878,11 +1126,13 @@ static int launch_drop_fast(ocl_mesh* m, const MeshView& mv, const CutterDev& cd
         }
     } else {
         if (timed) HIP_TRY(hipEventRecord(m->ev_s0, st));
-        if ((rc = launch_select<double>(m, mv, m->rec64.as<double>(), P64, cd, d_cl, perm, n, o, q1, qcount, 1, cnt, st)))
+        if ((rc = launch_select<double>(m, mv, m->rs64.rec.as<double>(), m->rs64.cap.as<double>(), P64, cd, d_cl, perm,
+                                        n, o, q1, qcount, zfloor, 1, cnt, st)))
             return rc;
         if (timed) HIP_TRY(hipEventRecord(m->ev_s1, st));
     }
-    k_drop_coop_exact<<<kCoopBlocks, 256, 0, st>>>(mv, cd, d_cl, exact_list, exact_n, o, m->cnt3.as<Counters>());
+    k_drop_coop_exact<<<kCoopBlocks, 256, 0, st>>>(mv, cd, d_cl, exact_list, exact_n, zfloor,
+                                                   m->rs64.cap.as<double>(), org, o, m->cnt3.as<Counters>());
     HIP_TRY(hipGetLastError());
     return OCL_OK;
 }
@@ -976,6 +1226,10 @@ int ocl_hip_mesh_create(const double* tri_xyz, size_t ntri, unsigned bucket, ocl
     ocl_mesh* m = new (std::nothrow) ocl_mesh();
     if (!m) return fail(OCL_E_NOMEM, "out of host memory");
     HIP_TRY(hipGetDevice(&m->device));
+    if (const char* g = std::getenv("OCL_XCD_GROUP")) {
+        const int v = std::atoi(g);
+        if (v >= 1 && v <= 256 && (v & (v - 1)) == 0) m->xcd_group = (uint32_t)v;
+    }
     m->ntri = ntri;
     m->bucket = bucket < 1 ? 1 : bucket;
     // host: bbox per triangle and the reference's KD leaf order
@@ -1094,8 +1348,7 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
     HIP_TRY(hipMemsetAsync(m->cnt2.p, 0, sizeof(Counters), st));
     HIP_TRY(hipMemsetAsync(m->cnt3.p, 0, sizeof(Counters), st));
     // bin CL points
-    if ((rc = m->keys.ensure(4ull * n)) || (rc = m->perm.ensure(4ull * n)) || (rc = m->hist.ensure(4ull * 65536)))
-        return rc;
+    if ((rc = m->keys.ensure(8ull * n)) || (rc = m->perm.ensure(4ull * n))) return rc;
     const double R = cd.c.radius;
     BinFrame f;
     f.x0 = m->bbox[0] - R;
@@ -1103,12 +1356,14 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
     const double ex = std::max(m->bbox[1] + R - f.x0, 1e-9), ey = std::max(m->bbox[3] + R - f.y0, 1e-9);
     f.sx = 256.0 / ex;
     f.sy = 256.0 / ey;
-    uint32_t* hist = m->hist.as<uint32_t>();
-    HIP_TRY(hipMemsetAsync(hist, 0, 4 * 65536, st));
     const uint32_t nb = (n32 + 255) / 256;
-    k_cl_keys<<<nb, 256, 0, st>>>(d_cl, n32, f, m->keys.as<uint32_t>(), hist);
-    k_scan64k<<<1, 1024, 0, st>>>(hist);
-    k_cl_scatter<<<nb, 256, 0, st>>>(m->keys.as<uint32_t>(), n32, hist, m->perm.as<uint32_t>());
+    uint32_t* keys = m->keys.as<uint32_t>();
+    k_cl_keys<<<nb, 256, 0, st>>>(d_cl, n32, f, keys);
+    size_t tmp_bytes = 0;
+    HIP_TRY(cl_sort_by_key(nullptr, &tmp_bytes, keys, keys + n32, m->perm.as<uint32_t>(), n32, st));
+    if ((rc = m->hist.ensure(tmp_bytes + 16))) return rc;
+    tmp_bytes = m->hist.bytes;
+    HIP_TRY(cl_sort_by_key(m->hist.p, &tmp_bytes, keys, keys + n32, m->perm.as<uint32_t>(), n32, st));
     if (stats) {
         if (!m->ev0) HIP_TRY(hipEventCreate(&m->ev0));
         if (!m->ev1) HIP_TRY(hipEventCreate(&m->ev1));
@@ -1120,7 +1375,7 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
     bool fast = false;
     if (precision == OCL_PREC_EXACT64 || m->ntri == 0 || cd.c.kind == K_COMPOSITE) {
         if (m->ntri > 0)
-            k_drop_exact<<<nb, 256, 0, st>>>(mv, cd, d_cl, m->perm.as<uint32_t>(), nullptr, n32, o, cnt);
+            k_drop_exact<<<xcd_grid(nb, mv.xcd_group), 256, 0, st>>>(mv, cd, d_cl, m->perm.as<uint32_t>(), nullptr, n32, o, cnt);
         else
             k_drop_none<<<nb, 256, 0, st>>>(d_cl, n32, o);
     } else {
@@ -1146,6 +1401,7 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
         stats->escalated = qc[0];
         stats->escalated_exact = (precision == OCL_PREC_MIXED && cd.c.kind != K_CONE) ? qc[1] : qc[0];
         stats->brent_iters = hc.brent + h2.brent + h3.brent;
+        stats->wave_evals = hc.evals;
         stats->kernel_ms = ms;
         stats->select_ms = ms;
         if (fast) {

@@ -43,7 +43,8 @@ SelParams<T> params(const CutterDev& cd, const SelErr& er) {
 template <class T, int K>
 void run_points(const std::vector<TriX>& tx, const std::vector<BoxOrigin>& org, const std::vector<double>& bbx,
                 const CutterDev& cd, const SelParams<T>& P, const double* cl, size_t n, const std::vector<T>& rec,
-                const uint32_t* order, double* z_out, int* type_out, int* esc_out, int* feat_out, int* tri_out) {
+                const std::vector<T>& boxcap, const uint32_t* order, double* z_out, int* type_out, int* esc_out,
+                int* feat_out, int* tri_out) {
     typedef FM<T> F;
     const double R = cd.c.radius;
     const size_t ntri = tx.size();
@@ -57,6 +58,24 @@ void run_points(const std::vector<TriX>& tx, const std::vector<BoxOrigin>& org, 
         bool framed = false;
         double ozw = 0;
         const size_t nbox = (ntri + kLeafH - 1) / kLeafH;
+        // floor of the final height from the vertices of the boxes under the point
+        T zc = -F::inf();
+        if (P.cull)
+            for (size_t b = 0; b < nbox; ++b) {
+                const double* bb = &bbx[4 * b];
+                if (bb[1] < x || bb[0] > x || bb[3] < y || bb[2] > y) continue;
+                const BoxOrigin& bo = org[b];
+                if (!framed) {
+                    framed = true;
+                    ozw = bo.oz;
+                    wzb = (T)(z0 - ozw);
+                }
+                const T dz = (T)(bo.oz - ozw);
+                const T xl = (T)(x - bo.ox), yl = (T)(y - bo.oy);
+                const size_t e = std::min(ntri, (b + 1) * kLeafH);
+                for (size_t k = b * kLeafH; k < e; ++k)
+                    zc = F::max_(zc, vertex_floor<T, K>(&rec[(size_t)kFastRec * k], xl, yl, P, true) + dz);
+            }
         for (size_t b = 0; b < nbox; ++b) {
             const double* bb = &bbx[4 * b];
             if (bb[1] < x - R || bb[0] > x + R || bb[3] < y - R || bb[2] > y + R) continue;
@@ -68,6 +87,10 @@ void run_points(const std::vector<TriX>& tx, const std::vector<BoxOrigin>& org, 
             }
             const T dz = (T)(bo.oz - ozw);
             const T xl = (T)(x - bo.ox), yl = (T)(y - bo.oy);
+            if (P.cull) {
+                const T d2 = box_d2<T>(xl, yl, -(T)bo.hx, (T)bo.hx, -(T)bo.hy, (T)bo.hy);
+                if (boxcap[b] + dz - prof_lower<T, K>(d2, P) < F::max_(wzb, zc) - P.CUT) continue;
+            }
             const uint32_t bt_in = S.bt;
             S.zb = wzb - dz;
             for (int q = 0; q < 4; ++q) S.zt[q] = -F::inf();
@@ -75,8 +98,13 @@ void run_points(const std::vector<TriX>& tx, const std::vector<BoxOrigin>& org, 
             const size_t e = std::min(ntri, (b + 1) * kLeafH);
             for (size_t k = b * kLeafH; k < e; ++k) {
                 const T* r = &rec[(size_t)kFastRec * k];
-                if (!(r[R_BX1] < xl - P.R || r[R_BX0] > xl + P.R || r[R_BY1] < yl - P.R || r[R_BY0] > yl + P.R))
-                    sel_triangle<T, K>(r, (uint32_t)k, xl, yl, P, S, true);
+                if (r[R_BX1] < xl - P.R || r[R_BX0] > xl + P.R || r[R_BY1] < yl - P.R || r[R_BY0] > yl + P.R)
+                    continue;
+                if (P.cull) {
+                    const T d2 = box_d2<T>(xl, yl, r[R_BX0], r[R_BX1], r[R_BY0], r[R_BY1]);
+                    if (r[R_ZCAP] - prof_lower<T, K>(d2, P) < F::max_(S.zb, zc - dz) - P.CUT) continue;
+                }
+                sel_triangle<T, K>(r, (uint32_t)k, xl, yl, P, S, true);
             }
             if (S.bt != bt_in) wzb = S.zb + dz;
             for (int q = 0; q < 4; ++q) wzt[q] = F::max_(wzt[q], S.zt[q] + dz);
@@ -175,7 +203,8 @@ extern "C" int ocl_selcheck_drop(const double* tris, size_t ntri, unsigned bucke
         org[b].ox = 0.5 * ((double)fx0 + (double)fx1);
         org[b].oy = 0.5 * ((double)fy0 + (double)fy1);
         org[b].oz = 0.5 * (z0 + z1);
-        org[b].pad = 0;
+        org[b].hx = std::nextafterf((float)(0.5 * ((double)fx1 - (double)fx0)), INFINITY);
+        org[b].hy = std::nextafterf((float)(0.5 * ((double)fy1 - (double)fy0)), INFINITY);
     }
     auto go = [&](auto tag) {
         typedef decltype(tag) T;
@@ -183,17 +212,23 @@ extern "C" int ocl_selcheck_drop(const double* tris, size_t ntri, unsigned bucke
         er.u = FM<T>::eps() * escale;
         er.R = cd.c.radius;
         er.zmag = 4.0 * cd.c.radius + 10.0;
-        const SelParams<T> P = params<T>(cd, er);
+        SelParams<T> P = params<T>(cd, er);
         std::vector<T> rec((size_t)kFastRec * ntri);
+        std::vector<T> boxcap(nbox, -FM<T>::inf());
+        double emax = 0;
         for (size_t k = 0; k < ntri; ++k) {
             const BoxOrigin& o = org[k / kLeafH];
-            build_record<T>(tx[k], cd, o.ox, o.oy, o.oz, er, &rec[(size_t)kFastRec * k]);
+            T* r = &rec[(size_t)kFastRec * k];
+            build_record<T>(tx[k], cd, o.ox, o.oy, o.oz, er, r);
+            boxcap[k / kLeafH] = std::max(boxcap[k / kLeafH], r[R_ZCAP]);
+            emax = std::max(emax, (double)r[R_EMAX]);
         }
+        P.CUT = (T)(2.0 * emax + (double)P.HPAD);
         switch (cd.c.kind) {
-        case K_CYL: run_points<T, K_CYL>(tx, org, bbx, cd, P, cl, n, rec, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
-        case K_BALL: run_points<T, K_BALL>(tx, org, bbx, cd, P, cl, n, rec, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
-        case K_BULL: run_points<T, K_BULL>(tx, org, bbx, cd, P, cl, n, rec, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
-        default: run_points<T, K_CONE>(tx, org, bbx, cd, P, cl, n, rec, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
+        case K_CYL: run_points<T, K_CYL>(tx, org, bbx, cd, P, cl, n, rec, boxcap, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
+        case K_BALL: run_points<T, K_BALL>(tx, org, bbx, cd, P, cl, n, rec, boxcap, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
+        case K_BULL: run_points<T, K_BULL>(tx, org, bbx, cd, P, cl, n, rec, boxcap, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
+        default: run_points<T, K_CONE>(tx, org, bbx, cd, P, cl, n, rec, boxcap, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
         }
     };
     if (use_float) go(0.0f);
