@@ -470,8 +470,11 @@ OCL_HD int feat_slot(int f) {
 // reference's overlaps() test).  Branch-free: every update is a select, so a
 // wavefront runs the triangle as one straight-line block of VALU code.
 // x, y: lane position relative to the record's box origin.
+// hi (optional): the largest upper value (z + error bound, or z + slack of
+// an ambiguous contact) of the triangle's live contacts, -inf if none.
 template <class T, int K>
-OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>& P, Sel<T>& S, bool on) {
+OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>& P, Sel<T>& S, bool on,
+                         T* hi = nullptr) {
     typedef FM<T> F;
     const T NEG = -F::inf();
     const int flags = (int)rec[R_FLAGS];
@@ -489,6 +492,7 @@ OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>&
     const bool famb = live & (fabs(mf) <= ebf);
     S.zt[2] = F::max_(S.zt[2], fval ? zf + ezf : NEG);
     S.za = F::max_(S.za, famb ? zf + ezf : NEG);
+    T th = (fval | famb) ? zf + ezf : NEG;
     const bool flift = fval & (zf > S.zb);
     bool rlift = false, ramb = false;
     T zr = NEG;
@@ -501,6 +505,7 @@ OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>&
         ramb = live & (fabs(mr) <= ebf);
         S.zt[3] = F::max_(S.zt[3], rval ? zr + ezf : NEG);
         S.za = F::max_(S.za, ramb ? zr + ezf : NEG);
+        th = F::max_(th, (rval | ramb) ? zr + ezf : NEG);
         rlift = !flift & rval & (zr > S.zb);
         // the cone's vertices/edges may rise above its facet contacts, so
         // whether the facet lifted (and skipped them) must be decided exactly
@@ -521,6 +526,7 @@ OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>&
         const bool amb = ve & (m <= P.S2V) & (m >= -P.DM);
         S.zt[0] = F::max_(S.zt[0], val ? z + P.E : NEG);
         S.za = F::max_(S.za, amb ? z + P.EAV : NEG);
+        th = F::max_(th, F::max_(val ? z + P.E : NEG, amb ? z + P.EAV : NEG));
         const bool take = val & (z > zv);
         zv = take ? z : zv;
         fv = take ? (FT_VTX0 + j) : fv;
@@ -596,6 +602,7 @@ OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>&
                          ((fabs(mt) <= dt) | ((m <= s2e) & (mt >= -dt - P.SQDM * invl)));
         S.zt[1] = F::max_(S.zt[1], val ? z + eze : NEG);
         S.za = F::max_(S.za, amb ? z + eae : NEG);
+        th = F::max_(th, F::max_(val ? z + eze : NEG, amb ? z + eae : NEG));
         const bool take = val & (z > ze);
         ze = take ? z : ze;
         ee = take ? eze : ee;
@@ -620,6 +627,7 @@ OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>&
     S.bt = any ? k : S.bt;
     S.bf = any ? fn : S.bf;
     S.bamb = any ? an : S.bamb;
+    if (hi) *hi = th;
 }
 
 // Escalation decision at the end of the walk (wave frame).  z0: initial CL

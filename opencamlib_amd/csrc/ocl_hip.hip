@@ -84,6 +84,8 @@ struct ocl_mesh {
     ocl_amd::DevBuf esc_list, esc_list2, esc_flag, blocksum, qcount, cnt2, cnt3;
     ocl_amd::DevBuf origins;     // frame (BoxOrigin) per level-0 box
     ocl_amd::DevBuf zfloor;      // per CL point: floor of the final z handed to the escalation passes
+    ocl_amd::DevBuf ntset, ntcnt, zband, selrec;  // per CL point: selection records and near-tie sets
+    int exact_slot = 0;          // queue-count slot read by the exact pass in the last call
     double zmid = 0, zhalf = 0, xyb = 0;
     // cached selection records (rebuilt when the cutter or precision changes)
     ocl_amd::RecSet rs32, rs64;
@@ -366,6 +368,23 @@ __global__ void k_box_frames(const double* __restrict__ tri, uint32_t ntri, cons
     org[b] = o;
 }
 
+constexpr int kNearTie = 8;  // near-tie set capacity per lane
+
+// Per CL point (escalated ones): triangles (rank indices) whose contacts
+// reach the band below the selection's best, their count (0xff = none
+// usable) and the bottom of the band.
+struct NearTieSets {
+    uint32_t* set;   // kNearTie per point
+    uint8_t* cnt;    // status: near-tie set size, kSelDecided or kSelNone
+    double* zband;
+    uint32_t* bt;    // selection winner (rank index, feature), its height and error bound
+    uint8_t* bf;
+    double* zsel;
+    float* esel;
+};
+constexpr uint8_t kSelDecided = 0xfe;  // winner decided, refine it
+constexpr uint8_t kSelNone = 0xff;     // undecided, no usable near-tie set
+
 template <class T>
 struct WaveState {
     T zb, zt[4], za;
@@ -374,18 +393,26 @@ struct WaveState {
 // Selection over a list of points.  Writes final results for lanes that
 // need no escalation and flags the others (esc[pos] = 1).  refine = 0 is the
 // FP32-only mode: the selection result is written as is.
+// 4 wavefronts per SIMD for the FP32 pass, 3 for the FP64 one (register budget)
 template <class T, int K>
-__global__ void __launch_bounds__(256) k_drop_select(MeshView m, const T* __restrict__ rec,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3))) k_drop_select(MeshView m, const T* __restrict__ rec,
                                                      const BoxOrigin* __restrict__ org,
                                                      const T* __restrict__ boxcap, SelParams<T> P, CutterDev cd,
                                                      const double* __restrict__ cl, const uint32_t* __restrict__ list,
                                                      const uint32_t* __restrict__ count, uint32_t n, DropOut o,
-                                                     uint32_t* __restrict__ next, uint32_t* __restrict__ next_n,
-                                                     double* __restrict__ zfloor, int refine, Counters* cnt) {
+                                                     double* __restrict__ zfloor, NearTieSets nts, int refine,
+                                                     Counters* cnt) {
     typedef FM<T> F;
     // per-wave LDS copy of the records of the level-0 box being evaluated
     __shared__ __align__(16) T srec[4][kLeaf * kFastRec];
     T* my = srec[threadIdx.x >> 6];
+    // per-lane near-tie set: rank indices (and upper values, wave frame) of
+    // the triangles whose contacts reach the band below the lane's known height
+    __shared__ uint32_t nt_k[4][kNearTie][64];
+    __shared__ float nt_h[4][kNearTie][64];
+    const uint32_t wv = threadIdx.x >> 6, ln = lane_id();
+    uint32_t ntn = 0;
+    bool ovf = false;
     const uint32_t n_eff = count ? *count : n;
     const uint32_t gid = xcd_block(m.xcd_group) * blockDim.x + threadIdx.x;
     const uint32_t wave_first = gid & ~63u;
@@ -537,7 +564,39 @@ __global__ void __launch_bounds__(256) k_drop_select(MeshView m, const T* __rest
             }
             if (!__any(on)) continue;
             ++evals;
-            sel_triangle<T, K>(r, ku, xl, yl, P, S, on);
+            T th;
+            sel_triangle<T, K>(r, ku, xl, yl, P, S, on, &th);
+            if (K != K_CONE && refine) {
+                // near-tie set: triangles reaching the band below the best
+                // height the lane is known to reach (running best or floor)
+                const bool keep = on & (th >= F::max_(S.zb, zc - dz) - P.CUT);
+                if (__any(keep) && keep) {
+                    if (ntn == (uint32_t)kNearTie) {
+                        // full: drop the entries that fell below the band since
+                        const float lim = (float)(F::max_(S.zb + dz, zc) - P.CUT);
+                        uint32_t w = 0;
+                        for (uint32_t i = 0; i < (uint32_t)kNearTie; ++i) {
+                            const float h = nt_h[wv][i][ln];
+                            const uint32_t kk = nt_k[wv][i][ln];
+                            if (h >= lim) {
+                                nt_h[wv][w][ln] = h;
+                                nt_k[wv][w][ln] = kk;
+                                ++w;
+                            }
+                        }
+                        ntn = w;
+                    }
+                    if (ntn < (uint32_t)kNearTie) {
+                        // upper value in the wave frame, rounded up
+                        const float hw = (float)(th + dz);
+                        nt_h[wv][ntn][ln] = hw + 1e-6f * (1.0f + fabsf(hw));
+                        nt_k[wv][ntn][ln] = ku;
+                        ++ntn;
+                    } else {
+                        ovf = true;
+                    }
+                }
+            }
         }
         // leave the box frame
         if (S.bt != bt_in) W.zb = S.zb + dz;
@@ -550,31 +609,34 @@ __global__ void __launch_bounds__(256) k_drop_select(MeshView m, const T* __rest
     for (int i = 0; i < 4; ++i) D.zt[i] = W.zt[i];
     D.za = W.za;
     const T z0w = (T)(z0 - ozw);
-    bool up = refine && framed && sel_escalate(D, z0w, P);
-    bool rfail = false;
-    DropLane L = fresh_lane(x, y, z0);
-    unsigned long long brent = 0;
-    if (act && !up && D.bt != 0xffffffffu) {
-        if (refine) {
-            const TriX t = load_tri(m.tri, D.bt);
-            DropLane X = fresh_lane(x, y, -INFINITY);
-            X.cur_tri = D.bt;
-            int it = 0;
-            exact_feature(cd.c, X, t, D.bf, it);
-            brent += it;
-            const double zsel = ozw + (double)D.zb;
-            if (X.type == CC_NONE || feat_slot(X.feat) != feat_slot(D.bf) || !(X.z > z0) ||
-                fabs(X.z - zsel) > 4.0 * ((double)D.eb + (double)P.E)) {
-                up = true;
-                rfail = true;
-            } else {
-                L.z = X.z;
-                L.cc = X.cc;
-                L.type = X.type;
-                L.tri = D.bt;
-                L.feat = X.feat;
+    const bool up = refine && framed && sel_escalate(D, z0w, P);
+    if (refine) {
+        if (act) {
+            // selection record for the resolve pass: the winner to refine,
+            // or the near-tie set of an undecided point
+            const bool usable = K != K_CONE && !ovf && framed && ntn > 0;
+            const uint8_t stt = !up ? (uint8_t)kSelDecided : usable ? (uint8_t)ntn : (uint8_t)kSelNone;
+            nts.cnt[src] = stt;
+            nts.bt[src] = D.bt;
+            nts.bf[src] = (uint8_t)D.bf;
+            nts.zsel[src] = framed ? ozw + (double)D.zb : z0;
+            nts.esel[src] = (float)D.eb;
+            if (up && usable) {
+                for (uint32_t i = 0; i < ntn; ++i) nts.set[(size_t)src * kNearTie + i] = nt_k[wv][i][ln];
+                const double zs0 = ozw + (double)(D.zb - P.CUT);
+                nts.zband[src] = fmax(z0, zs0 - 1e-9 * (1.0 + fabs(zs0)));
             }
-        } else {
+            // Floor of the final height for the later passes: the vertex
+            // floor, and the winner's value when its validity was unambiguous
+            // (a decided point only reaches them when its refine fails).
+            T fl = zc;
+            if (K != K_CONE && up && !D.bamb && D.bt != 0xffffffffu) fl = F::max_(fl, D.zb - (T)2 * D.eb - P.E);
+            zfloor[src] = (K == K_CONE || !framed) ? -INFINITY : ozw + (double)fl - 1e-9 * (1.0 + fabs(ozw));
+        }
+    } else if (act) {
+        // FP32-only mode: the selection result as is (no CC point)
+        DropLane L = fresh_lane(x, y, z0);
+        if (D.bt != 0xffffffffu) {
             L.z = ozw + (double)D.zb;
             L.cc = v3(NAN, NAN, NAN);
             const int sl = feat_slot(D.bf);
@@ -582,24 +644,9 @@ __global__ void __launch_bounds__(256) k_drop_select(MeshView m, const T* __rest
             L.tri = D.bt;
             L.feat = D.bf;
         }
+        write_drop(o, m.order, src, L);
     }
-    if (!refine) up = false;
-    const bool push = act && up;
-    if (push && zfloor) {
-        // Floor of the final height for the next passes: the vertex floor,
-        // and the winner's value when its validity was unambiguous.
-        T fl = zc;
-        if (K != K_CONE && !D.bamb && !rfail && D.bt != 0xffffffffu) fl = F::max_(fl, D.zb - (T)2 * D.eb - P.E);
-        zfloor[src] = (K == K_CONE || !framed) ? -INFINITY : ozw + (double)fl - 1e-9 * (1.0 + fabs(ozw));
-    }
-    const unsigned long long bal = __ballot(push);
-    if (bal) {
-        uint32_t base = 0;
-        if ((threadIdx.x & 63) == (uint32_t)(__ffsll((long long)bal) - 1)) base = atomicAdd(next_n, (uint32_t)__popcll(bal));
-        base = __shfl(base, __ffsll((long long)bal) - 1, 64);
-        if (push) next[base + __popcll(bal & ((1ull << (threadIdx.x & 63)) - 1ull))] = src;
-    }
-    if (act && !up) write_drop(o, m.order, src, L);
+    const unsigned long long brent = 0;
     if (cnt) {
         wave_add(&cnt->pairs, act ? S.pairs : 0);
         wave_add(&cnt->escalated, (act && up) ? 1 : 0);
@@ -608,6 +655,78 @@ __global__ void __launch_bounds__(256) k_drop_select(MeshView m, const T* __rest
     }
 }
 
+
+// Resolve pass, one lane per CL point (input order), after a selection pass:
+//  - decided points: the winner's feature is re-evaluated by the exact FP64
+//    code and kept if it is consistent with the selection (same contact
+//    class, lifted, height within the selection's error bound);
+//  - undecided points with a near-tie set: the reference's fold
+//    (MillingCutter::dropCutter over triangles in rank order) restricted to
+//    the set and started at the bottom of its band.  Every other triangle's
+//    contacts stay below that start, and for Cyl/Ball/Bull the reference's
+//    result is the maximum of the triangles' own drops (ties to the lowest
+//    rank), so when a set triangle lifts above the start the fold's result
+//    is the reference's.
+// Everything else goes on to the cooperative passes.
+__global__ void __launch_bounds__(256) k_drop_resolve(MeshView m, CutterDev cd, const double* __restrict__ cl,
+                                                      const uint32_t* __restrict__ perm, uint32_t n, NearTieSets nts,
+                                                      double etie, DropOut o, uint32_t* __restrict__ next,
+                                                      uint32_t* __restrict__ next_n, Counters* cnt) {
+    // points in the Hilbert order of the selection pass: neighbouring lanes
+    // refine the same few triangles
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((g & ~63u) >= n) return;
+    const bool act = g < n;
+    const uint32_t i = act ? perm[g] : 0u;
+    const uint8_t stt = act ? nts.cnt[i] : kSelNone;
+    bool done = false;
+    unsigned long long brent = 0;
+    if (act && stt != kSelNone) {
+        const double x = cl[3 * (size_t)i], y = cl[3 * (size_t)i + 1], z0 = cl[3 * (size_t)i + 2];
+        if (stt == kSelDecided) {
+            const uint32_t bt = nts.bt[i];
+            if (bt == 0xffffffffu) {
+                write_drop(o, m.order, i, fresh_lane(x, y, z0));
+                done = true;
+            } else {
+                const int bf = nts.bf[i];
+                const TriX t = load_tri(m.tri, bt);
+                DropLane X = fresh_lane(x, y, -INFINITY);
+                X.cur_tri = bt;
+                int it = 0;
+                exact_feature(cd.c, X, t, bf, it);
+                brent += it;
+                if (X.type != CC_NONE && feat_slot(X.feat) == feat_slot(bf) && X.z > z0 &&
+                    fabs(X.z - nts.zsel[i]) <= 4.0 * ((double)nts.esel[i] + etie)) {
+                    X.tri = bt;
+                    write_drop(o, m.order, i, X);
+                    done = true;
+                }
+            }
+        } else {
+            DropLane E = fresh_lane(x, y, nts.zband[i]);
+            for (uint32_t j = 0; j < stt; ++j) {
+                const uint32_t k = nts.set[(size_t)i * kNearTie + j];
+                const TriX t = load_tri(m.tri, k);
+                E.cur_tri = k;
+                if (cutter_overlaps(cd.c.radius, x, y, t)) brent += drop_triangle_exact(cd, E, t);
+            }
+            if (E.type != CC_NONE) {
+                write_drop(o, m.order, i, E);
+                done = true;
+            }
+        }
+    }
+    const bool push = act && !done;
+    const unsigned long long bal = __ballot(push);
+    if (bal) {
+        uint32_t base = 0;
+        if ((threadIdx.x & 63) == (uint32_t)(__ffsll((long long)bal) - 1)) base = atomicAdd(next_n, (uint32_t)__popcll(bal));
+        base = __shfl(base, __ffsll((long long)bal) - 1, 64);
+        if (push) next[base + __popcll(bal & ((1ull << (threadIdx.x & 63)) - 1ull))] = i;
+    }
+    if (cnt) wave_add(&cnt->brent, brent);
+}
 
 // ------------------------------------------------- cooperative tiers
 // One wavefront per point (points of an escalation queue are scattered, so
@@ -1057,15 +1176,14 @@ static void set_cut(SelParams<T>& P, const RecSet& rs) {
 template <class T>
 static int launch_select(ocl_mesh* m, const MeshView& mv, const T* rec, const T* cap, const SelParams<T>& P,
                          const CutterDev& cd, const double* d_cl, const uint32_t* list, uint32_t n, const DropOut& o,
-                         uint32_t* next, uint32_t* next_n, double* zfloor, int refine, Counters* cnt,
-                         hipStream_t st) {
+                         double* zfloor, const NearTieSets& nts, int refine, Counters* cnt, hipStream_t st) {
     const uint32_t nb = xcd_grid((n + 255) / 256, mv.xcd_group);
     const BoxOrigin* org = m->origins.as<BoxOrigin>();
     switch (cd.c.kind) {
-    case K_CYL: k_drop_select<T, K_CYL><<<nb, 256, 0, st>>>(mv, rec, org, cap, P, cd, d_cl, list, nullptr, n, o, next, next_n, zfloor, refine, cnt); break;
-    case K_BALL: k_drop_select<T, K_BALL><<<nb, 256, 0, st>>>(mv, rec, org, cap, P, cd, d_cl, list, nullptr, n, o, next, next_n, zfloor, refine, cnt); break;
-    case K_BULL: k_drop_select<T, K_BULL><<<nb, 256, 0, st>>>(mv, rec, org, cap, P, cd, d_cl, list, nullptr, n, o, next, next_n, zfloor, refine, cnt); break;
-    case K_CONE: k_drop_select<T, K_CONE><<<nb, 256, 0, st>>>(mv, rec, org, cap, P, cd, d_cl, list, nullptr, n, o, next, next_n, zfloor, refine, cnt); break;
+    case K_CYL: k_drop_select<T, K_CYL><<<nb, 256, 0, st>>>(mv, rec, org, cap, P, cd, d_cl, list, nullptr, n, o, zfloor, nts, refine, cnt); break;
+    case K_BALL: k_drop_select<T, K_BALL><<<nb, 256, 0, st>>>(mv, rec, org, cap, P, cd, d_cl, list, nullptr, n, o, zfloor, nts, refine, cnt); break;
+    case K_BULL: k_drop_select<T, K_BULL><<<nb, 256, 0, st>>>(mv, rec, org, cap, P, cd, d_cl, list, nullptr, n, o, zfloor, nts, refine, cnt); break;
+    case K_CONE: k_drop_select<T, K_CONE><<<nb, 256, 0, st>>>(mv, rec, org, cap, P, cd, d_cl, list, nullptr, n, o, zfloor, nts, refine, cnt); break;
     default: return fail(OCL_E_UNSUPPORTED, "selection pass has no composite specialisation");
     }
     HIP_TRY(hipGetLastError());
@@ -1079,14 +1197,32 @@ static double env_scale(const char* name, double def) {
     return v ? std::atof(v) : def;
 }
 
+// Precision pipeline over the Hilbert-sorted points:
+//   mixed: FP32 selection -> near-tie pass (Cyl/Ball/Bull) -> FP64 selection
+//          of what is still undecided (one wavefront per point) -> exact pass;
+//   fp64:  FP64 selection -> near-tie pass -> exact pass;
+//   fp32:  FP32 selection only.
+// Cones skip the near-tie and FP64 passes (their result depends on the
+// evaluation order).  Undecided points carry a floor of their final height
+// (zfloor) that lets the later passes skip triangles that cannot matter.
 static int launch_drop_fast(ocl_mesh* m, const MeshView& mv, const CutterDev& cd, const double* d_cl, uint32_t n,
                             int precision, const DropOut& o, Counters* cnt, hipStream_t st, bool timed) {
     int rc;
     const uint32_t* perm = m->perm.as<uint32_t>();
     if ((rc = m->esc_list.ensure(4ull * n + 64)) || (rc = m->esc_list2.ensure(4ull * n + 64)) ||
-        (rc = m->zfloor.ensure(8ull * n + 64)))
+        (rc = m->zfloor.ensure(8ull * n + 64)) || (rc = m->ntset.ensure(4ull * kNearTie * n + 64)) ||
+        (rc = m->ntcnt.ensure(2ull * n + 64)) || (rc = m->zband.ensure(8ull * n + 64)) ||
+        (rc = m->selrec.ensure(20ull * n + 64)))
         return rc;
     double* zfloor = m->zfloor.as<double>();
+    NearTieSets nts;
+    nts.set = m->ntset.as<uint32_t>();
+    nts.cnt = m->ntcnt.as<uint8_t>();
+    nts.bf = nts.cnt + n;
+    nts.zband = m->zband.as<double>();
+    nts.zsel = m->selrec.as<double>();
+    nts.bt = reinterpret_cast<uint32_t*>(nts.zsel + n);
+    nts.esel = reinterpret_cast<float*>(nts.bt + n);
     uint32_t* q1 = m->esc_list.as<uint32_t>();
     uint32_t* q2 = m->esc_list2.as<uint32_t>();
     uint32_t* qcount = m->qcount.as<uint32_t>();
@@ -1097,43 +1233,51 @@ static int launch_drop_fast(ocl_mesh* m, const MeshView& mv, const CutterDev& cd
     if ((rc = ensure_records<double>(m, cd, m->rs64, er64, st))) return rc;
     set_cut(P64, m->rs64);
     const BoxOrigin* org = m->origins.as<BoxOrigin>();
-    uint32_t* exact_list = q1;
-    uint32_t* exact_n = qcount;
-    if (precision == OCL_PREC_FP32 || precision == OCL_PREC_MIXED) {
+    const double* cap64 = m->rs64.cap.as<double>();
+    const bool mixed = precision == OCL_PREC_MIXED;
+    double etie = (double)P64.E;
+    if (precision == OCL_PREC_FP32 || mixed) {
         const SelErr er32 = make_err<float>(cd, se);
         SelParams<float> P32 = make_params<float>(cd, er32);
         if ((rc = ensure_records<float>(m, cd, m->rs32, er32, st))) return rc;
         set_cut(P32, m->rs32);
-        const int refine = precision == OCL_PREC_MIXED;
+        etie = (double)P32.E;
         if (timed) HIP_TRY(hipEventRecord(m->ev_s0, st));
         if ((rc = launch_select<float>(m, mv, m->rs32.rec.as<float>(), m->rs32.cap.as<float>(), P32, cd, d_cl, perm, n,
-                                       o, q1, qcount, zfloor, refine, cnt, st)))
+                                       o, zfloor, nts, mixed ? 1 : 0, cnt, st)))
             return rc;
         if (timed) HIP_TRY(hipEventRecord(m->ev_s1, st));
-        if (!refine) return OCL_OK;
-        if (cd.c.kind != K_CONE) {
-            // FP64 selection of the undecided points, one wavefront per point
-            Counters* c2 = m->cnt2.as<Counters>();
-            const double* r64 = m->rs64.rec.as<double>();
-            switch (cd.c.kind) {
-            case K_CYL: k_drop_coop_select<K_CYL><<<kCoopBlocks, 256, 0, st>>>(mv, r64, org, P64, cd, d_cl, q1, qcount, zfloor, m->rs64.cap.as<double>(), o, q2, qcount + 1, c2); break;
-            case K_BALL: k_drop_coop_select<K_BALL><<<kCoopBlocks, 256, 0, st>>>(mv, r64, org, P64, cd, d_cl, q1, qcount, zfloor, m->rs64.cap.as<double>(), o, q2, qcount + 1, c2); break;
-            default: k_drop_coop_select<K_BULL><<<kCoopBlocks, 256, 0, st>>>(mv, r64, org, P64, cd, d_cl, q1, qcount, zfloor, m->rs64.cap.as<double>(), o, q2, qcount + 1, c2); break;
-            }
-            HIP_TRY(hipGetLastError());
-            exact_list = q2;
-            exact_n = qcount + 1;
-        }
+        if (!mixed) return OCL_OK;
     } else {
         if (timed) HIP_TRY(hipEventRecord(m->ev_s0, st));
-        if ((rc = launch_select<double>(m, mv, m->rs64.rec.as<double>(), m->rs64.cap.as<double>(), P64, cd, d_cl, perm,
-                                        n, o, q1, qcount, zfloor, 1, cnt, st)))
+        if ((rc = launch_select<double>(m, mv, m->rs64.rec.as<double>(), cap64, P64, cd, d_cl, perm, n, o, zfloor, nts,
+                                        1, cnt, st)))
             return rc;
         if (timed) HIP_TRY(hipEventRecord(m->ev_s1, st));
     }
-    k_drop_coop_exact<<<kCoopBlocks, 256, 0, st>>>(mv, cd, d_cl, exact_list, exact_n, zfloor,
-                                                   m->rs64.cap.as<double>(), org, o, m->cnt3.as<Counters>());
+    // resolve pass over all points -> q1 (count slot 0); mixed Cyl/Ball/Bull:
+    // FP64 selection of what remains -> q2 (slot 1); exact pass on the last
+    k_drop_resolve<<<(n + 255) / 256, 256, 0, st>>>(mv, cd, d_cl, perm, n, nts, etie, o, q1, qcount,
+                                                     m->cnt2.as<Counters>());
     HIP_TRY(hipGetLastError());
+    const uint32_t* exact_list = q1;
+    const uint32_t* exact_n = qcount;
+    if (mixed && cd.c.kind != K_CONE && env_scale("OCL_COOP_SELECT", 0.0) != 0.0) {
+        Counters* c2 = m->cnt2.as<Counters>();
+        const double* r64 = m->rs64.rec.as<double>();
+        switch (cd.c.kind) {
+        case K_CYL: k_drop_coop_select<K_CYL><<<kCoopBlocks, 256, 0, st>>>(mv, r64, org, P64, cd, d_cl, q1, qcount, zfloor, cap64, o, q2, qcount + 1, c2); break;
+        case K_BALL: k_drop_coop_select<K_BALL><<<kCoopBlocks, 256, 0, st>>>(mv, r64, org, P64, cd, d_cl, q1, qcount, zfloor, cap64, o, q2, qcount + 1, c2); break;
+        default: k_drop_coop_select<K_BULL><<<kCoopBlocks, 256, 0, st>>>(mv, r64, org, P64, cd, d_cl, q1, qcount, zfloor, cap64, o, q2, qcount + 1, c2); break;
+        }
+        HIP_TRY(hipGetLastError());
+        exact_list = q2;
+        exact_n = qcount + 1;
+    }
+    k_drop_coop_exact<<<kCoopBlocks, 256, 0, st>>>(mv, cd, d_cl, exact_list, exact_n, zfloor, cap64, org, o,
+                                                   m->cnt3.as<Counters>());
+    HIP_TRY(hipGetLastError());
+    m->exact_slot = (int)(exact_n - qcount);
     return OCL_OK;
 }
 
@@ -1393,13 +1537,12 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
         HIP_TRY(hipMemcpy(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(&h2, m->cnt2.p, sizeof(h2), hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(&h3, m->cnt3.p, sizeof(h3), hipMemcpyDeviceToHost));
-        uint32_t qc[2] = {0, 0};
-        if (precision != OCL_PREC_EXACT64 && cd.c.kind != K_COMPOSITE && m->ntri > 0)
-            HIP_TRY(hipMemcpy(qc, m->qcount.p, 8, hipMemcpyDeviceToHost));
+        uint32_t qc[4] = {0, 0, 0, 0};
+        if (fast && precision != OCL_PREC_FP32) HIP_TRY(hipMemcpy(qc, m->qcount.p, 16, hipMemcpyDeviceToHost));
         stats->points = n;
         stats->pairs = hc.pairs;
-        stats->escalated = qc[0];
-        stats->escalated_exact = (precision == OCL_PREC_MIXED && cd.c.kind != K_CONE) ? qc[1] : qc[0];
+        stats->escalated = hc.escalated;
+        stats->escalated_exact = fast ? qc[m->exact_slot] : 0;
         stats->brent_iters = hc.brent + h2.brent + h3.brent;
         stats->wave_evals = hc.evals;
         stats->kernel_ms = ms;
