@@ -46,6 +46,9 @@ extern "C" {
 #define OCL_PREC_FP64 1    /* FP64 selection kernel + FP64 exact refine */
 #define OCL_PREC_FP32 2    /* FP32 selection kernel only (no refine) */
 #define OCL_PREC_MIXED 3   /* FP32 selection + FP64 refine + FP64 escalation */
+/* push only, OR-ed into precision: evaluate only (fiber, triangle) pairs whose
+ * search boxes overlap, instead of the reference's KD-search superset */
+#define OCL_PUSH_EXACT_FILTER 0x100
 
 /* One simple cutter: CylCutter(d, L), BallCutter(d, L), BullCutter(d, r, L),
  * ConeCutter(d, angle, L) (cylcutter.cpp:39, ballcutter.cpp:38,
@@ -103,8 +106,12 @@ size_t ocl_hip_mesh_size(const ocl_mesh* mesh);
  * (Y fibers, setXZDimensions); kdtree.hpp:117-268.  It is the order in which
  * KDTree::search returns overlapping triangles (kdtree.hpp:273-304), used for
  * getTrianglesUnderCutter / getOverlapTriangles (batchdropcutter_py.hpp:46-55,
- * batchpushcutter_py.hpp:59-84). */
-int ocl_hip_kd_order(const double* tri_xyz, size_t ntri, int dimset, unsigned bucket, uint32_t* order_out);
+ * batchpushcutter_py.hpp:59-84).  accept_out (optional, 4 doubles per rank):
+ * the search reaches rank k for a query box with, per search axis
+ * a0 = first axis, a1 = second, max_a0 >= A0, min_a0 <= B0, max_a1 >= A1,
+ * min_a1 <= B1, stored (A0, B0, A1, B1). */
+int ocl_hip_kd_order(const double* tri_xyz, size_t ntri, int dimset, unsigned bucket, uint32_t* order_out,
+                     double* accept_out);
 
 /* Batch drop-cutter: replaces BatchDropCutter::run / dropCutter5
  * (batchdropcutter.cpp:187-231).  cl_xyz: n x (x, y, z0).  Outputs (any may
@@ -120,6 +127,29 @@ int ocl_hip_drop_device(ocl_mesh* mesh, const ocl_cutter_desc* cutter, const dou
                         size_t n, int precision, double* d_z_out, double* d_cc_xyz_out,
                         uint8_t* d_cc_type_out, uint32_t* d_tri_out, uint8_t* d_feat_out,
                         ocl_stats* stats, void* stream);
+
+/* Batch push-cutter: replaces BatchPushCutter::run / pushCutter3
+ * (batchpushcutter.cpp:136-188) for one fiber direction (dir 0 = X fibers,
+ * setXDirection; 1 = Y fibers, setYDirection; batchpushcutter.cpp:55-66).
+ * fib: nf x (p1.xyz, p2.xyz), horizontal, along the chosen axis.  Computes
+ * every fiber's interval list (the union of its per-triangle intervals,
+ * Fiber::addInterval, fiber.cpp:58-93) on the device and returns
+ * offsets_out[0..nf] (fiber i owns rows offsets_out[i] .. offsets_out[i+1]-1)
+ * and the row total; ocl_hip_push_fetch then copies the rows out.  The
+ * search box is across +- R, z in [f.z, f.z + L]; the computation is FP64 in
+ * every precision mode.
+ * Pairs: every triangle the reference's KD search returns for the fiber
+ * (kdtree.hpp:273-304, a superset of the box overlaps, all of them pushed as
+ * in batchpushcutter.cpp:170-181); with OCL_PUSH_EXACT_FILTER only the
+ * triangles whose boxes overlap the search box. */
+int ocl_hip_push_batch(ocl_mesh* mesh, const ocl_cutter_desc* cutter, int dir, const double* fib_p1p2, size_t nf,
+                       int precision, uint64_t* offsets_out, uint64_t* total_out, ocl_stats* stats, void* stream);
+
+/* Rows of the last ocl_hip_push_batch on this mesh, in fiber order, each
+ * fiber's intervals sorted by lower end: lohi = (lower, upper) fiber
+ * parameters t of p1 + t (p2 - p1); cc = lower CC xyz, lower CCType,
+ * upper CC xyz, upper CCType (8 doubles per row). */
+int ocl_hip_push_fetch(ocl_mesh* mesh, double* lohi, double* cc, void* stream);
 
 /* Thread-local message of the last failure. */
 const char* ocl_hip_last_error(void);

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libocl_hip.so")
 
 CUTTER_TYPES = {"cyl": 0, "ball": 1, "bull": 2, "cone": 3, "composite": 4}
 PREC = {"exact64": 0, "fp64": 1, "fp32": 2, "mixed": 3}
+PUSH_EXACT_FILTER = 0x100
 
 # CCType names (src/geo/ccpoint.hpp:33-39)
 CCTYPE_NAMES = ["NONE", "VERTEX", "VERTEX_CYL", "EDGE", "EDGE_HORIZ", "EDGE_SHAFT", "EDGE_HORIZ_CYL",
@@ -67,7 +68,13 @@ def lib():
         L.ocl_hip_drop_device.argtypes = [vp, C.POINTER(CutterDesc), vp, sz, C.c_int, vp, vp, vp, vp, vp,
                                           C.POINTER(Stats), vp]
         L.ocl_hip_drop_device.restype = C.c_int
-        L.ocl_hip_kd_order.argtypes = [dp, sz, C.c_int, C.c_uint, C.POINTER(C.c_uint32)]
+        u64p = C.POINTER(C.c_uint64)
+        L.ocl_hip_push_batch.argtypes = [vp, C.POINTER(CutterDesc), C.c_int, dp, sz, C.c_int, u64p, u64p,
+                                         C.POINTER(Stats), vp]
+        L.ocl_hip_push_batch.restype = C.c_int
+        L.ocl_hip_push_fetch.argtypes = [vp, dp, dp, vp]
+        L.ocl_hip_push_fetch.restype = C.c_int
+        L.ocl_hip_kd_order.argtypes = [dp, sz, C.c_int, C.c_uint, C.POINTER(C.c_uint32), dp]
         L.ocl_hip_kd_order.restype = C.c_int
         L.ocl_hip_last_error.argtypes = []
         L.ocl_hip_last_error.restype = C.c_char_p
@@ -120,14 +127,16 @@ def _ptr(a, t):
 DIMSETS = {"xy": 0, "yz": 1, "xz": 2}
 
 
-def kd_order(tris, dimset="xy", bucket: int = 1) -> np.ndarray:
+def kd_order(tris, dimset="xy", bucket: int = 1, with_accept: bool = False):
     """order[rank] = triangle index in the reference KD-tree's leaf order
-    (host computation, no device needed)."""
+    (host computation, no device needed).  with_accept: also the (ntri, 4)
+    acceptance bounds per rank (A0, B0, A1, B1) of the KD search."""
     t = np.ascontiguousarray(np.asarray(tris, dtype=np.float64).reshape(-1, 9))
     out = np.empty(t.shape[0], dtype=np.uint32)
+    acc = np.empty((t.shape[0], 4)) if with_accept else None
     check(lib().ocl_hip_kd_order(_ptr(t, C.c_double), t.shape[0], DIMSETS[dimset], bucket,
-                                 _ptr(out, C.c_uint32)))
-    return out
+                                 _ptr(out, C.c_uint32), _ptr(acc, C.c_double) if with_accept else None))
+    return (out, acc) if with_accept else out
 
 
 class DeviceMesh:
@@ -167,6 +176,30 @@ class DeviceMesh:
         tri64 = tri.astype(np.int64)
         tri64[tri == 0xFFFFFFFF] = -1
         return dict(z=z, cc=cc, type=ty, tri=tri64, feat=ft, stats=st.as_dict())
+
+    def push(self, spec: dict, fibers, direction, precision="exact64", want_stats=True, exact_filter=False):
+        """Batch push-cutter of host fibers (nf, 6) along X (direction 0 / "x")
+        or Y (1 / "y").  Returns offsets (nf+1), lohi (rows, 2), cc (rows, 8)
+        and stats; fiber i owns rows offsets[i]:offsets[i+1], sorted by lower.
+        exact_filter: push only triangles whose boxes overlap the search box
+        (the reference pushes its whole KD-search superset)."""
+        d = {"x": 0, "y": 1}.get(direction, direction)
+        fib = np.ascontiguousarray(np.asarray(fibers, dtype=np.float64).reshape(-1, 6))
+        nf = fib.shape[0]
+        offs = np.zeros(nf + 1, dtype=np.uint64)
+        total = C.c_uint64(0)
+        st = Stats()
+        desc = cutter_desc(spec)
+        prec = PREC[precision] | (PUSH_EXACT_FILTER if exact_filter else 0)
+        check(lib().ocl_hip_push_batch(self.h, C.byref(desc), int(d), _ptr(fib, C.c_double), nf, prec,
+                                       _ptr(offs, C.c_uint64), C.byref(total),
+                                       C.byref(st) if want_stats else None, None))
+        n = int(total.value)
+        lohi = np.empty((n, 2))
+        cc = np.empty((n, 8))
+        if n:
+            check(lib().ocl_hip_push_fetch(self.h, _ptr(lohi, C.c_double), _ptr(cc, C.c_double), None))
+        return dict(offsets=offs.astype(np.int64), lohi=lohi, cc=cc, stats=st.as_dict())
 
     def drop_device(self, spec: dict, d_cl: int, n: int, precision, d_z: int, d_cc=0, d_type=0, d_tri=0,
                     d_feat=0, stream=None, stats=False):

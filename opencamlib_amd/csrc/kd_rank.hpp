@@ -25,17 +25,35 @@ namespace ocl_amd {
 // dims: 4 bbox indices into [minx maxx miny maxy minz maxz]
 //   XY (drop) = {0,1,2,3}, YZ (X-fibers) = {2,3,4,5}, XZ (Y-fibers) = {0,1,4,5}
 // bb: 6 doubles per triangle.  Returns order[k] = triangle at rank k.
-inline std::vector<uint32_t> kd_rank_order(const double* bb, size_t n, const int dims[4],
-                                           unsigned bucket) {
+//
+// acc (optional): 4 doubles per rank describing when KDTree::search_node
+// (kdtree.hpp:273-304) reaches that triangle's leaf.  The search always
+// enters the lo child of a min-dimension node (bbox index 0, 2, 4) and the hi
+// child of a max-dimension node (1, 3, 5); it enters the hi child of a
+// min-dimension cut c only if query[d+1] >= c, and the lo child of a
+// max-dimension cut c only if query[d-1] <= c.  Per search axis (dims[0]/2,
+// dims[2]/2) the leaf thus needs query_max >= A and query_min <= B, stored
+// as (A0, B0, A1, B1).
+inline std::vector<uint32_t> kd_rank_order(const double* bb, size_t n, const int dims[4], unsigned bucket,
+                                           std::vector<double>* acc = nullptr) {
     std::vector<uint32_t> items(n), tmp(n);
     for (size_t i = 0; i < n; ++i) items[i] = (uint32_t)i;
+    if (acc) acc->assign(4 * n, 0.0);
     if (n == 0) return items;
     if (bucket < 1) bucket = 1;
-    struct Range { size_t b, e; };
+    struct Range { size_t b, e; double A[3], B[3]; };
     // explicit stack; hi ranges are pushed last so they are split first, which
     // keeps the final array in hi-first depth-first leaf order
     std::vector<Range> stack;
-    stack.push_back({0, n});
+    Range root;
+    root.b = 0;
+    root.e = n;
+    for (int a = 0; a < 3; ++a) {
+        root.A[a] = -INFINITY;
+        root.B[a] = INFINITY;
+    }
+    stack.push_back(root);
+    const int ax0 = dims[0] / 2, ax1 = dims[2] / 2;
     while (!stack.empty()) {
         Range r = stack.back();
         stack.pop_back();
@@ -63,7 +81,17 @@ inline std::vector<uint32_t> kd_rank_order(const double* bb, size_t n, const int
         const int d = dims[pick];
         const double spread = hi[d] - lo[d];
         const double cut = lo[d] + spread / 2;
-        if (cnt <= bucket || std::fabs(spread) < 1e-7) continue;  // leaf: keep order
+        if (cnt <= bucket || std::fabs(spread) < 1e-7) {  // leaf: keep order
+            if (acc)
+                for (size_t i = r.b; i < r.e; ++i) {
+                    double* q = acc->data() + 4 * i;
+                    q[0] = r.A[ax0];
+                    q[1] = r.B[ax0];
+                    q[2] = r.A[ax1];
+                    q[3] = r.B[ax1];
+                }
+            continue;
+        }
         size_t nh = 0;
         for (size_t i = r.b; i < r.e; ++i)
             if (bb[6 * items[i] + d] > cut) tmp[r.b + nh++] = items[i];
@@ -71,8 +99,13 @@ inline std::vector<uint32_t> kd_rank_order(const double* bb, size_t n, const int
         for (size_t i = r.b; i < r.e; ++i)
             if (!(bb[6 * items[i] + d] > cut)) tmp[w++] = items[i];
         for (size_t i = r.b; i < r.e; ++i) items[i] = tmp[i];
-        if (r.b + nh < r.e) stack.push_back({r.b + nh, r.e});  // lo, split later
-        if (nh > 0) stack.push_back({r.b, r.b + nh});          // hi, split next
+        Range rlo = r, rhi = r;
+        rlo.b = r.b + nh;
+        rhi.e = r.b + nh;
+        if (d % 2 == 0) rhi.A[d / 2] = std::fmax(rhi.A[d / 2], cut);  // min-dimension cut
+        else rlo.B[d / 2] = std::fmin(rlo.B[d / 2], cut);             // max-dimension cut
+        if (rlo.b < rlo.e) stack.push_back(rlo);  // lo, split later
+        if (nh > 0) stack.push_back(rhi);         // hi, split next
     }
     return items;
 }

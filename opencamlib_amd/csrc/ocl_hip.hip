@@ -67,8 +67,26 @@ struct RecSet {
 
 }  // namespace ocl_amd
 
+// Box hierarchy over a rank-ordered copy of the triangles (the push trees:
+// YZ order with (y, z) boxes for X fibers, XZ order with (x, z) boxes for Y fibers).
+struct ocl_tree {
+    ocl_amd::DevBuf tri, order, boxes, levels;
+    ocl_amd::DevBuf acc;      // per rank: the KD search's leaf acceptance bounds (kd_rank_order)
+    ocl_amd::DevBuf kdboxes;  // acceptance boxes, same level layout as boxes
+    uint32_t cnt[ocl_amd::kMaxLevels] = {};
+    size_t box_off[ocl_amd::kMaxLevels] = {};
+    int top = 0;
+    bool built = false;
+};
+
 struct ocl_mesh {
     int device = 0;
+    std::vector<double> host_tri;  // input triangles (for the push trees, built on first use)
+    ocl_tree push_tree[2];
+    ocl_amd::DevBuf push_fib, push_perm, push_e, push_n, push_ovf, push_offs, push_cnt, push_lohi, push_cc;
+    uint32_t push_cap = 16;
+    uint32_t push_nf = 0;          // fibers of the last push call (for ocl_hip_push_fetch)
+    uint64_t push_total = 0;
     size_t ntri = 0;
     double bbox[6] = {0, 0, 0, 0, 0, 0};
     unsigned bucket = 1;
@@ -142,6 +160,45 @@ __global__ void k_box_leaf(const double* __restrict__ rec, uint32_t ntri, float4
         }
     }
     out[b] = make_float4(__double2float_rd(x0), __double2float_rd(y0), __double2float_ru(x1), __double2float_ru(y1));
+}
+
+// Level-0 boxes over (coordinate cu, z) for the push trees: float4(u0, z0, u1, z1).
+__global__ void k_box_leaf_uz(const double* __restrict__ rec, uint32_t ntri, float4* __restrict__ out, uint32_t nbox,
+                              int cu) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    double u0 = INFINITY, z0 = INFINITY, u1 = -INFINITY, z1 = -INFINITY;
+    const uint32_t e = min(ntri, (b + 1) * kLeaf);
+    for (uint32_t i = b * kLeaf; i < e; ++i) {
+        const double* r = rec + (size_t)kRec * i;
+        for (int k = 0; k < 3; ++k) {
+            u0 = fmin(u0, r[3 * k + cu]);
+            u1 = fmax(u1, r[3 * k + cu]);
+            z0 = fmin(z0, r[3 * k + 2]);
+            z1 = fmax(z1, r[3 * k + 2]);
+        }
+    }
+    out[b] = make_float4(__double2float_rd(u0), __double2float_rd(z0), __double2float_ru(u1), __double2float_ru(z1));
+}
+
+// Level-0 "acceptance boxes" of the push trees: the KD search reaches a
+// triangle when its query satisfies max >= A and min <= B per axis
+// (kd_rank_order), i.e. when the query interval meets [A, B]; per box the
+// union [min A, max B] (rounded outward) makes the same test a box overlap.
+__global__ void k_kdbox_leaf(const double* __restrict__ acc, uint32_t ntri, float4* __restrict__ out,
+                             uint32_t nbox) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    double a0 = INFINITY, a1 = INFINITY, b0 = -INFINITY, b1 = -INFINITY;
+    const uint32_t e = min(ntri, (b + 1) * kLeaf);
+    for (uint32_t i = b * kLeaf; i < e; ++i) {
+        const double* q = acc + 4 * (size_t)i;
+        a0 = fmin(a0, q[0]);
+        b0 = fmax(b0, q[1]);
+        a1 = fmin(a1, q[2]);
+        b1 = fmax(b1, q[3]);
+    }
+    out[b] = make_float4(__double2float_rd(a0), __double2float_rd(a1), __double2float_ru(b0), __double2float_ru(b1));
 }
 
 __global__ void k_box_up(const float4* __restrict__ in, uint32_t nin, float4* __restrict__ out, uint32_t nout) {
@@ -1127,6 +1184,8 @@ __global__ void k_drop_none(const double* __restrict__ cl, uint32_t n, DropOut o
 
 }  // namespace ocl_amd
 
+#include "push_kernels.hpp"
+
 using namespace ocl_amd;
 
 // ---------------------------------------------------------- fast launcher
@@ -1350,13 +1409,16 @@ extern "C" {
 
 const char* ocl_hip_last_error(void) { return g_err.c_str(); }
 
-int ocl_hip_kd_order(const double* tri_xyz, size_t ntri, int dimset, unsigned bucket, uint32_t* order_out) {
+int ocl_hip_kd_order(const double* tri_xyz, size_t ntri, int dimset, unsigned bucket, uint32_t* order_out,
+                     double* accept_out) {
     if (ntri > 0 && (!tri_xyz || !order_out)) return fail(OCL_E_ARG, "NULL pointer");
     if (dimset < 0 || dimset > 2) return fail(OCL_E_ARG, "dimset must be 0 (XY), 1 (YZ) or 2 (XZ)");
     const int* dims = dimset == 0 ? kDimsXY : dimset == 1 ? kDimsYZ : kDimsXZ;
     std::vector<double> bb = tri_bboxes(tri_xyz, ntri);
-    std::vector<uint32_t> order = kd_rank_order(bb.data(), ntri, dims, bucket);
+    std::vector<double> acc;
+    std::vector<uint32_t> order = kd_rank_order(bb.data(), ntri, dims, bucket, accept_out ? &acc : nullptr);
     std::copy(order.begin(), order.end(), order_out);
+    if (accept_out) std::copy(acc.begin(), acc.end(), accept_out);
     return OCL_OK;
 }
 
@@ -1370,6 +1432,7 @@ int ocl_hip_mesh_create(const double* tri_xyz, size_t ntri, unsigned bucket, ocl
     ocl_mesh* m = new (std::nothrow) ocl_mesh();
     if (!m) return fail(OCL_E_NOMEM, "out of host memory");
     HIP_TRY(hipGetDevice(&m->device));
+    m->host_tri.assign(tri_xyz, tri_xyz + 9 * ntri);
     if (const char* g = std::getenv("OCL_XCD_GROUP")) {
         const int v = std::atoi(g);
         if (v >= 1 && v <= 256 && (v & (v - 1)) == 0) m->xcd_group = (uint32_t)v;
@@ -1552,6 +1615,208 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
             stats->select_ms = ms;
         }
     }
+    return OCL_OK;
+}
+
+}  // extern "C"
+
+// Builds a rank-ordered triangle copy with its box hierarchy over
+// (coordinate cu, z), for the push trees (dims = the reference KD-tree's
+// search dimensions).
+static int build_tree(ocl_mesh* m, ocl_tree& T, const int dims[4], int cu) {
+    const size_t ntri = m->ntri;
+    std::vector<double> bb = tri_bboxes(m->host_tri.data(), ntri);
+    std::vector<double> acc;
+    std::vector<uint32_t> order = kd_rank_order(bb.data(), ntri, dims, m->bucket, &acc);
+    std::vector<double> sorted(9 * ntri);
+    for (size_t r = 0; r < ntri; ++r)
+        std::memcpy(&sorted[9 * r], m->host_tri.data() + 9 * (size_t)order[r], 9 * sizeof(double));
+    int rc;
+    DevBuf staging;
+    if ((rc = T.tri.ensure(sizeof(double) * kRec * ntri)) || (rc = T.order.ensure(sizeof(uint32_t) * ntri)) ||
+        (rc = staging.ensure(sizeof(double) * 9 * ntri)) || (rc = T.acc.ensure(sizeof(double) * 4 * ntri)))
+        return rc;
+    HIP_TRY(hipMemcpy(T.acc.p, acc.data(), sizeof(double) * 4 * ntri, hipMemcpyHostToDevice));
+    uint32_t c = (uint32_t)((ntri + kLeaf - 1) / kLeaf);
+    size_t total = 0;
+    int lvl = 0;
+    for (;;) {
+        T.cnt[lvl] = c;
+        T.box_off[lvl] = total;
+        total += c;
+        if (c <= (uint32_t)kFan || lvl + 1 == kMaxLevels) break;
+        c = (c + kFan - 1) / kFan;
+        ++lvl;
+    }
+    T.top = lvl;
+    if ((rc = T.boxes.ensure(sizeof(float4) * total)) || (rc = T.levels.ensure(sizeof(uint32_t) * 2 * kMaxLevels)))
+        return rc;
+    uint32_t lv[2 * kMaxLevels] = {};
+    for (int l = 0; l <= T.top; ++l) {
+        lv[2 * l] = T.cnt[l];
+        lv[2 * l + 1] = (uint32_t)T.box_off[l];
+    }
+    HIP_TRY(hipMemcpy(T.levels.p, lv, sizeof(lv), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(staging.p, sorted.data(), sizeof(double) * 9 * ntri, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(T.order.p, order.data(), sizeof(uint32_t) * ntri, hipMemcpyHostToDevice));
+    const uint32_t n32 = (uint32_t)ntri;
+    k_tri_prep<<<(n32 + 255) / 256, 256>>>(staging.as<double>(), T.tri.as<double>(), n32);
+    float4* boxes = T.boxes.as<float4>();
+    k_box_leaf_uz<<<(T.cnt[0] + 255) / 256, 256>>>(T.tri.as<double>(), n32, boxes, T.cnt[0], cu);
+    for (int l = 1; l <= T.top; ++l)
+        k_box_up<<<(T.cnt[l] + 255) / 256, 256>>>(boxes + T.box_off[l - 1], T.cnt[l - 1], boxes + T.box_off[l], T.cnt[l]);
+    if ((rc = T.kdboxes.ensure(sizeof(float4) * total))) return rc;
+    float4* kb = T.kdboxes.as<float4>();
+    k_kdbox_leaf<<<(T.cnt[0] + 255) / 256, 256>>>(T.acc.as<double>(), n32, kb, T.cnt[0]);
+    for (int l = 1; l <= T.top; ++l)
+        k_box_up<<<(T.cnt[l] + 255) / 256, 256>>>(kb + T.box_off[l - 1], T.cnt[l - 1], kb + T.box_off[l], T.cnt[l]);
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipGetLastError());
+    T.built = true;
+    return OCL_OK;
+}
+
+static MeshView tree_view(const ocl_tree& T, size_t ntri) {
+    MeshView v;
+    v.tri = T.tri.as<const double>();
+    v.order = T.order.as<const uint32_t>();
+    v.ntri = (uint32_t)ntri;
+    v.xcd_group = 1;
+    v.tree.box = T.boxes.as<const float4>();
+    v.tree.lvl = T.levels.as<const uint32_t>();
+    v.tree.top = T.top;
+    return v;
+}
+
+extern "C" {
+
+int ocl_hip_push_batch(ocl_mesh* m, const ocl_cutter_desc* desc, int dir, const double* fib, size_t nf,
+                       int precision, uint64_t* offsets_out, uint64_t* total_out, ocl_stats* stats, void* stream_) {
+    if (!m) return fail(OCL_E_ARG, "mesh is NULL");
+    if (dir != 0 && dir != 1) return fail(OCL_E_ARG, "dir must be 0 (X fibers) or 1 (Y fibers)");
+    if (nf > 0 && (!fib || !offsets_out)) return fail(OCL_E_ARG, "fiber/offset pointers are NULL");
+    if (nf >= 0x7fffffffu) return fail(OCL_E_ARG, "too many fibers");
+    const int superset = (precision & OCL_PUSH_EXACT_FILTER) ? 0 : 1;
+    precision &= ~OCL_PUSH_EXACT_FILTER;
+    if (precision < OCL_PREC_EXACT64 || precision > OCL_PREC_MIXED) return fail(OCL_E_ARG, "unknown precision mode");
+    CutterDev cd;
+    int rc = cutter_from_desc(desc, cd);
+    if (rc) return rc;
+    // Fiber::calcDir asserts a horizontal fiber; the facet push tests
+    // p1.y == p2.y (X fiber) or p1.x == p2.x (Y fiber)
+    for (size_t i = 0; i < nf; ++i) {
+        const double* q = fib + 6 * i;
+        if (q[2] != q[5]) return fail(OCL_E_ARG, "fiber " + std::to_string(i) + " is not horizontal");
+        if (dir == 0 ? (q[1] != q[4]) : (q[0] != q[3]))
+            return fail(OCL_E_ARG, "fiber " + std::to_string(i) + " does not run along the requested axis");
+    }
+    hipStream_t st = (hipStream_t)stream_;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (total_out) *total_out = 0;
+    m->push_nf = (uint32_t)nf;
+    m->push_total = 0;
+    if (nf == 0) {
+        if (offsets_out) offsets_out[0] = 0;
+        return OCL_OK;
+    }
+    if (m->ntri == 0) {
+        for (size_t i = 0; i <= nf; ++i) offsets_out[i] = 0;
+        return OCL_OK;
+    }
+    ocl_tree& T = m->push_tree[dir];
+    if (!T.built) {
+        static const int kYZ[4] = {2, 3, 4, 5}, kXZ[4] = {0, 1, 4, 5};
+        if ((rc = build_tree(m, T, dir == 0 ? kYZ : kXZ, dir == 0 ? 1 : 0))) return rc;
+    }
+    // wavefronts of neighbouring fibers: sort by (z, across)
+    std::vector<uint32_t> perm(nf);
+    for (size_t i = 0; i < nf; ++i) perm[i] = (uint32_t)i;
+    const int ca = dir == 0 ? 1 : 0;
+    std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) {
+        const double za = fib[6 * (size_t)a + 2], zb = fib[6 * (size_t)b + 2];
+        if (za != zb) return za < zb;
+        return fib[6 * (size_t)a + ca] < fib[6 * (size_t)b + ca];
+    });
+    if ((rc = m->push_fib.ensure(48ull * nf)) || (rc = m->push_perm.ensure(4ull * nf)) ||
+        (rc = m->push_n.ensure(4ull * nf + 64)) || (rc = m->push_ovf.ensure(64)) ||
+        (rc = m->push_cnt.ensure(sizeof(PushCounters))))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(m->push_fib.p, fib, 48ull * nf, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(m->push_perm.p, perm.data(), 4ull * nf, hipMemcpyHostToDevice, st));
+    MeshView pm = tree_view(T, m->ntri);
+    if (superset) pm.tree.box = T.kdboxes.as<const float4>();  // walk the KD acceptance boxes
+    PushCounters hc{};
+    float ms = 0;
+    for (;;) {
+        if ((rc = m->push_e.ensure(sizeof(PEntry) * (size_t)m->push_cap * nf))) return rc;
+        PushLists PL{m->push_e.as<PEntry>(), m->push_n.as<uint32_t>(), m->push_cap, m->push_ovf.as<uint32_t>()};
+        HIP_TRY(hipMemsetAsync(m->push_ovf.p, 0, 4, st));
+        HIP_TRY(hipMemsetAsync(m->push_cnt.p, 0, sizeof(PushCounters), st));
+        if (!m->ev0) HIP_TRY(hipEventCreate(&m->ev0));
+        if (!m->ev1) HIP_TRY(hipEventCreate(&m->ev1));
+        HIP_TRY(hipEventRecord(m->ev0, st));
+        const uint32_t nb = (uint32_t)std::min<size_t>((nf + kPushWaves - 1) / kPushWaves, 2048);
+        const double* fd = m->push_fib.as<double>();
+        const uint32_t* pp = m->push_perm.as<uint32_t>();
+        PushCounters* pc = m->push_cnt.as<PushCounters>();
+        const double* ac = T.acc.as<double>();
+        switch (cd.c.kind) {
+        case K_CYL: k_push_fibers<K_CYL><<<nb, 256, 0, st>>>(pm, ac, superset, cd, fd, pp, (uint32_t)nf, dir, PL, pc); break;
+        case K_BALL: k_push_fibers<K_BALL><<<nb, 256, 0, st>>>(pm, ac, superset, cd, fd, pp, (uint32_t)nf, dir, PL, pc); break;
+        case K_BULL: k_push_fibers<K_BULL><<<nb, 256, 0, st>>>(pm, ac, superset, cd, fd, pp, (uint32_t)nf, dir, PL, pc); break;
+        case K_CONE: k_push_fibers<K_CONE><<<nb, 256, 0, st>>>(pm, ac, superset, cd, fd, pp, (uint32_t)nf, dir, PL, pc); break;
+        default: k_push_fibers<K_COMPOSITE><<<nb, 256, 0, st>>>(pm, ac, superset, cd, fd, pp, (uint32_t)nf, dir, PL, pc); break;
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(m->ev1, st));
+        uint32_t ovf = 0;
+        HIP_TRY(hipMemcpyAsync(&ovf, m->push_ovf.p, 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        HIP_TRY(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+        if (ovf == 0) break;
+        if (m->push_cap >= 4096) return fail(OCL_E_UNSUPPORTED, "push: more than 4096 intervals on one fiber");
+        m->push_cap *= 4;
+    }
+    HIP_TRY(hipMemcpy(&hc, m->push_cnt.p, sizeof(hc), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> cnt(nf);
+    HIP_TRY(hipMemcpy(cnt.data(), m->push_n.p, 4ull * nf, hipMemcpyDeviceToHost));
+    uint64_t run = 0;
+    for (size_t i = 0; i < nf; ++i) {
+        offsets_out[i] = run;
+        run += cnt[i];
+    }
+    offsets_out[nf] = run;
+    m->push_total = run;
+    if (total_out) *total_out = run;
+    if ((rc = m->push_offs.ensure(8ull * (nf + 1)))) return rc;
+    HIP_TRY(hipMemcpy(m->push_offs.p, offsets_out, 8ull * (nf + 1), hipMemcpyHostToDevice));
+    if (stats) {
+        stats->points = nf;
+        stats->pairs = hc.pairs;
+        stats->intervals = run;
+        stats->brent_iters = hc.brent;
+        stats->kernel_ms = ms;
+        stats->select_ms = ms;
+    }
+    return OCL_OK;
+}
+
+int ocl_hip_push_fetch(ocl_mesh* m, double* lohi, double* cc, void* stream_) {
+    if (!m) return fail(OCL_E_ARG, "mesh is NULL");
+    const uint64_t total = m->push_total;
+    if (total == 0) return OCL_OK;
+    if (!lohi || !cc) return fail(OCL_E_ARG, "output pointers are NULL");
+    hipStream_t st = (hipStream_t)stream_;
+    int rc;
+    if ((rc = m->push_lohi.ensure(16ull * total)) || (rc = m->push_cc.ensure(64ull * total))) return rc;
+    PushLists PL{m->push_e.as<PEntry>(), m->push_n.as<uint32_t>(), m->push_cap, m->push_ovf.as<uint32_t>()};
+    const uint32_t nf = m->push_nf;
+    k_push_gather<<<(nf + 255) / 256, 256, 0, st>>>(PL, m->push_offs.as<uint64_t>(), nf, m->push_lohi.as<double>(),
+                                                    m->push_cc.as<double>());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(lohi, m->push_lohi.p, 16ull * total, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(cc, m->push_cc.p, 64ull * total, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
     return OCL_OK;
 }
 

@@ -1,0 +1,222 @@
+// Batch push-cutter kernels (BatchPushCutter::pushCutter3,
+// src/algo/batchpushcutter.cpp:136-188).
+//
+// The mesh copy used here is ordered by the reference's YZ (X fibers) or XZ
+// (Y fibers) KD-tree rank, with box hierarchies over (across, z) and over the
+// KD search's acceptance bounds.  Each fiber's interval list
+// (Fiber::addInterval, fiber.cpp:58-93) is kept sorted and disjoint in
+// global memory.
+#pragma once
+
+#include "mesh.hpp"
+#include "push_exact.hpp"
+#include "walk.hpp"
+
+namespace ocl_amd {
+
+// One union interval: lower/upper fiber parameter with CC points and types.
+struct PEntry {
+    double lo, hi;
+    double lcc[3], ucc[3];
+    int lty, uty;
+};
+
+struct PushLists {
+    PEntry* e;          // cap entries per fiber (input fiber index)
+    uint32_t* n;        // entries per fiber
+    uint32_t cap;
+    uint32_t* overflow; // fibers whose list overflowed
+};
+
+OCL_HD PEntry pentry_of(const PIval& i) {
+    PEntry p;
+    p.lo = i.lo;
+    p.hi = i.hi;
+    p.lcc[0] = i.lcc.x; p.lcc[1] = i.lcc.y; p.lcc[2] = i.lcc.z;
+    p.ucc[0] = i.ucc.x; p.ucc[1] = i.ucc.y; p.ucc[2] = i.ucc.z;
+    p.lty = i.lty;
+    p.uty = i.uty;
+    return p;
+}
+
+// Fiber::addInterval on a list sorted by lower end with pairwise disjoint
+// entries.  Entries touching or overlapping the new interval merge into one
+// (strict Interval::outside, interval.cpp:96-103); the merged lower end
+// (upper end) keeps the CC of the existing extreme unless the new interval is
+// strictly lower (higher), as the reference's sum interval keeps the first
+// extreme it meets, with the existing intervals listed before the new one.
+OCL_HD bool plist_insert(PEntry* E, uint32_t& n, uint32_t cap, const PEntry& iv) {
+    uint32_t j = 0;
+    while (j < n && E[j].hi < iv.lo) ++j;
+    if (j == n || E[j].lo > iv.hi) {
+        if (n == cap) return false;
+        for (uint32_t k = n; k > j; --k) E[k] = E[k - 1];
+        E[j] = iv;
+        ++n;
+        return true;
+    }
+    // existing entries overlapping iv: E[j..k); being disjoint, they have
+    // distinct extremes, and the new interval only wins a strictly lower
+    // lower end or a strictly higher upper end
+    PEntry m = E[j];
+    uint32_t k = j + 1;
+    while (k < n && E[k].lo <= iv.hi) {
+        m.hi = E[k].hi;
+        m.ucc[0] = E[k].ucc[0]; m.ucc[1] = E[k].ucc[1]; m.ucc[2] = E[k].ucc[2];
+        m.uty = E[k].uty;
+        ++k;
+    }
+    if (iv.lo < m.lo) {
+        m.lo = iv.lo;
+        m.lcc[0] = iv.lcc[0]; m.lcc[1] = iv.lcc[1]; m.lcc[2] = iv.lcc[2];
+        m.lty = iv.lty;
+    }
+    if (iv.hi > m.hi) {
+        m.hi = iv.hi;
+        m.ucc[0] = iv.ucc[0]; m.ucc[1] = iv.ucc[1]; m.ucc[2] = iv.ucc[2];
+        m.uty = iv.uty;
+    }
+    E[j] = m;
+    const uint32_t gone = k - (j + 1);
+    if (gone) {
+        for (uint32_t q = j + 1; q + gone < n; ++q) E[q] = E[q + gone];
+        n -= gone;
+    }
+    return true;
+}
+
+struct PushCounters {
+    unsigned long long pairs, intervals, brent;
+};
+
+// One wavefront per fiber (persistent grid): the wave walks the tree with
+// the fiber's search box, gathers the candidate triangles in rank order in
+// batches of 64 (one per lane), computes the 64 pair intervals in parallel
+// (FP64) and lane 0 merges them into the fiber's list in rank order.
+// dir 0: X fibers (across = y), 1: Y fibers (across = x); K = cutter kind.
+//
+// Candidates (superset = 1, the reference's behaviour): every triangle the
+// reference's KD search returns for the fiber's search box, reproduced
+// exactly from the per-rank leaf acceptance bounds (acc, see
+// kd_rank_order); the reference pushes all of them, without an overlap test
+// (batchpushcutter.cpp:170-181).  The tree's boxes are then the KD
+// acceptance boxes, so the walk's overlap test visits every box holding such
+// a triangle.  superset = 0: only triangles whose boxes overlap the search box.
+constexpr int kPushWaves = 4;  // wavefronts per 256-thread block
+
+template <int K>
+__global__ void __launch_bounds__(256) k_push_fibers(MeshView pm, const double* __restrict__ acc, int superset,
+                                                     CutterDev cd, const double* __restrict__ fib,
+                                                     const uint32_t* __restrict__ perm, uint32_t nf, int dir,
+                                                     PushLists PL, PushCounters* cnt) {
+    __shared__ uint32_t qmem[kPushWaves][80];
+    __shared__ PEntry emem[kPushWaves][64];
+    __shared__ uint32_t fmem[kPushWaves][64];
+    const uint32_t lane = lane_id();
+    const uint32_t wv = threadIdx.x >> 6;
+    uint32_t* qbuf = qmem[wv];
+    PEntry* ebuf = emem[wv];
+    uint32_t* fbuf = fmem[wv];
+    const double R = cd.c.radius, L = cd.c.length;
+    unsigned long long pairs = 0, brent = 0, nint = 0;
+    for (uint32_t w = blockIdx.x * kPushWaves + wv; w < nf; w += gridDim.x * kPushWaves) {
+        const uint32_t src = uniform_u(perm[w]);
+        const FibX f = make_fiber(fib + 6 * (size_t)src);
+        const double across = (dir == 0) ? f.p1.y : f.p1.x;
+        // search box (batchpushcutter.cpp:170-181), in (across, z)
+        const double a0 = across - R, a1 = across + R, v0 = f.p1.z, v1 = f.p1.z + L;
+        const float qa0 = __double2float_rd(a0), qa1 = __double2float_ru(a1);
+        const float qv0 = __double2float_rd(v0), qv1 = __double2float_ru(v1);
+        PEntry* E = PL.e + (size_t)src * PL.cap;
+        uint32_t n = 0;  // list size (kept by lane 0, broadcast when needed)
+        bool ovf = false;
+        uint32_t qn = 0;
+        auto batch = [&](uint32_t nb) {
+            const bool has = lane < nb;
+            const uint32_t k = has ? qbuf[lane] : 0u;
+            PIval iv;
+            piv_init(iv);
+            if (has) {
+                const TriX t = load_tri(pm.tri, k);
+                int it = 0;
+                push_pair<K>(cd, f, t, iv, it);
+                brent += it;
+            }
+            const bool keep = has && !piv_empty(iv);
+            if (keep) ebuf[lane] = pentry_of(iv);
+            fbuf[lane] = keep ? 1u : 0u;
+            wave_lds_sync();
+            if (lane == 0) {
+                for (uint32_t i = 0; i < nb; ++i)
+                    if (fbuf[i] && !ovf && !plist_insert(E, n, PL.cap, ebuf[i])) ovf = true;
+            }
+            wave_lds_sync();
+            pairs += has ? 1 : 0;
+        };
+        walk_boxes(pm.tree, qa0, qv0, qa1, qv1, [&](uint32_t b) {
+            const uint32_t k = b * kLeaf + lane;
+            bool cand = false;
+            if (lane < (uint32_t)kLeaf && k < pm.ntri) {
+                if (superset) {
+                    const double* q = acc + 4 * (size_t)k;  // needs max >= A and min <= B per axis
+                    cand = a1 >= q[0] && a0 <= q[1] && v1 >= q[2] && v0 <= q[3];
+                } else {
+                    const TriX t = load_tri(pm.tri, k);
+                    const double ta0 = fmin(fmin(t.p[0].x, t.p[1].x), t.p[2].x);
+                    const double ta1 = fmax(fmax(t.p[0].x, t.p[1].x), t.p[2].x);
+                    const double tb0 = fmin(fmin(t.p[0].y, t.p[1].y), t.p[2].y);
+                    const double tb1 = fmax(fmax(t.p[0].y, t.p[1].y), t.p[2].y);
+                    const double lo_a = (dir == 0) ? tb0 : ta0, hi_a = (dir == 0) ? tb1 : ta1;
+                    const double lo_z = fmin(fmin(t.p[0].z, t.p[1].z), t.p[2].z);
+                    cand = !(hi_a < a0 || lo_a > a1 || t.maxz < v0 || lo_z > v1);
+                }
+            }
+            const unsigned long long bal = __ballot(cand);
+            const uint32_t pos = qn + __popcll(bal & ((1ull << lane) - 1ull));
+            if (cand) qbuf[pos] = k;
+            qn += __popcll(bal);
+            wave_lds_sync();
+            if (qn >= 64) {
+                batch(64u);
+                const uint32_t rest = qn - 64;
+                const uint32_t moved = (lane < rest) ? qbuf[64 + lane] : 0u;
+                wave_lds_sync();
+                if (lane < rest) qbuf[lane] = moved;
+                wave_lds_sync();
+                qn = rest;
+            }
+        });
+        if (qn > 0) batch(qn);
+        if (lane == 0) {
+            PL.n[src] = n;
+            if (ovf) atomicAdd(PL.overflow, 1u);
+            nint += n;
+        }
+    }
+    if (cnt) {
+        wave_add(&cnt->pairs, pairs);
+        wave_add(&cnt->intervals, nint);
+        wave_add(&cnt->brent, brent);
+    }
+}
+
+// Output gather: fiber f's entries to rows offs[f]..offs[f+1]-1 of
+// lohi (2 per row) and cc (lower xyz, type, upper xyz, type).
+__global__ void k_push_gather(PushLists PL, const uint64_t* __restrict__ offs, uint32_t nf, double* __restrict__ lohi,
+                              double* __restrict__ cc) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const PEntry* E = PL.e + (size_t)f * PL.cap;
+    const uint64_t o = offs[f];
+    const uint32_t n = PL.n[f];
+    for (uint32_t j = 0; j < n; ++j) {
+        const PEntry& p = E[j];
+        lohi[2 * (o + j)] = p.lo;
+        lohi[2 * (o + j) + 1] = p.hi;
+        double* c = cc + 8 * (o + j);
+        c[0] = p.lcc[0]; c[1] = p.lcc[1]; c[2] = p.lcc[2]; c[3] = (double)p.lty;
+        c[4] = p.ucc[0]; c[5] = p.ucc[1]; c[6] = p.ucc[2]; c[7] = (double)p.uty;
+    }
+}
+
+}  // namespace ocl_amd

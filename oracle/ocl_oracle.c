@@ -947,6 +947,23 @@ long oc_kd_search_drop(oc_mesh* m, const oc_cutter* c, double x, double y, doubl
     return n <= cap ? n : -n;
 }
 
+/* KD search of one push fiber (batchpushcutter.cpp:160-170): dimset YZ for
+   X fibers (dir 0), XZ for Y fibers (dir 1) */
+long oc_kd_search_push(oc_mesh* m, const oc_cutter* c, int dir, const double* fib6, unsigned bucket, int* out,
+                       long cap) {
+    if (m->n == 0) return 0;
+    KDT* k = mesh_kd(m, dir == 0 ? 1 : 2, bucket);
+    double r = c->radius;
+    double clx = dir == 0 ? 0 : fib6[0], cly = dir == 0 ? fib6[1] : 0, clz = fib6[2];
+    double bb[6] = {clx - r, clx + r, cly - r, cly + r, clz, clz + c->length};
+    IVec v = {0, 0, 0};
+    kd_search_node(k, bb, k->root, &v);
+    long n = v.n;
+    if (n <= cap) memcpy(out, v.v, sizeof(int) * (size_t)n);
+    free(v.v);
+    return n <= cap ? n : -n;
+}
+
 /* ------------------------------------------------------------ batch drop */
 typedef struct { int key; int tri; } KR;
 static int kr_cmp(const void* a, const void* b) {

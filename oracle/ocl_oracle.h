@@ -82,6 +82,11 @@ int oc_kd_rank(const oc_mesh* m, int dimset, unsigned bucket, int* rank_out);
 long oc_kd_search_drop(oc_mesh* m, const oc_cutter* c, double x, double y, double z,
                        unsigned bucket, int* out, long cap);
 
+/* KD search of one push fiber (batchpushcutter.cpp:160-170), dir 0 = X
+   fiber (YZ tree), 1 = Y fiber (XZ tree); same return convention. */
+long oc_kd_search_push(oc_mesh* m, const oc_cutter* c, int dir, const double* fib6, unsigned bucket, int* out,
+                       long cap);
+
 /* Batch drop-cutter (batchdropcutter.cpp:187-231).
    search_mode 0: reference KD search then overlaps/below filter;
    search_mode 1: exhaustive overlap scan sorted by KD rank (same result);

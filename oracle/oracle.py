@@ -66,6 +66,8 @@ def lib():
         L.oc_kd_search_drop.argtypes = [C.c_void_p, cp, C.c_double, C.c_double, C.c_double,
                                         C.c_uint, ip, C.c_long]
         L.oc_kd_search_drop.restype = C.c_long
+        L.oc_kd_search_push.argtypes = [C.c_void_p, cp, C.c_int, dp, C.c_uint, ip, C.c_long]
+        L.oc_kd_search_push.restype = C.c_long
         L.oc_batch_drop.argtypes = [C.c_void_p, cp, dp, C.c_long, C.c_int, C.c_int, C.c_uint,
                                     dp, dp, ip, ip, ip, lp]
         L.oc_batch_drop.restype = C.c_int
@@ -172,6 +174,13 @@ class Mesh:
         cap = max(16, len(self))
         out = np.empty(cap, dtype=np.int32)
         n = lib().oc_kd_search_drop(self.h, C.byref(cutter.c), x, y, z, bucket, _ip(out), cap)
+        return out[:n]
+
+    def kd_search_push(self, cutter: Cutter, fib6, direction, bucket=1):
+        cap = max(16, len(self))
+        out = np.empty(cap, dtype=np.int32)
+        f = np.ascontiguousarray(np.asarray(fib6, dtype=np.float64).reshape(6))
+        n = lib().oc_kd_search_push(self.h, C.byref(cutter.c), direction, _dp(f), bucket, _ip(out), cap)
         return out[:n]
 
     def batch_drop(self, cutter: Cutter, cl_xyz, search_mode=1, nthreads=0, bucket=1):

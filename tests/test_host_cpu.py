@@ -37,13 +37,13 @@ def test_library_exports_every_declared_symbol():
 
 def test_error_paths_without_device_work():
     L = capi.lib()
-    rc = L.ocl_hip_kd_order(None, 3, 0, 1, None)
+    rc = L.ocl_hip_kd_order(None, 3, 0, 1, None, None)
     assert rc == -1 and b"NULL" in L.ocl_hip_last_error()
     t = np.zeros((1, 9))
     out = np.zeros(1, dtype=np.uint32)
     with pytest.raises(capi.OclError, match="dimset"):
         capi.check(L.ocl_hip_kd_order(t.ctypes.data_as(C.POINTER(C.c_double)), 1, 7, 1,
-                                      out.ctypes.data_as(C.POINTER(C.c_uint32))))
+                                      out.ctypes.data_as(C.POINTER(C.c_uint32)), None))
 
 
 @pytest.mark.parametrize("dimset,odim", [("xy", 0), ("yz", 1), ("xz", 2)])
@@ -61,6 +61,27 @@ def test_kd_order_matches_oracle(dimset, odim, mesh):
         inv = np.empty_like(rank)
         inv[rank] = np.arange(len(rank))
         assert np.array_equal(order, inv)
+
+
+@pytest.mark.parametrize("direction,dimset", [(0, "yz"), (1, "xz")])
+def test_kd_acceptance_bounds_reproduce_push_search(direction, dimset):
+    """The per-rank acceptance bounds select exactly the triangles the
+    reference's KD search returns for a push fiber (kdtree.hpp:273-304), in
+    the same order: the device pushes that superset."""
+    tris = W.spheres_scene(3, 2, 50.0, seed=9)
+    order, acc = capi.kd_order(tris, dimset, 1, with_accept=True)
+    spec = {"kind": "cone", "d": 10.0, "angle": math.pi / 4, "L": 10.0}
+    cut = O.cutter_from_spec(spec)
+    m = O.Mesh(tris)
+    R, L = 5.0, cut.length
+    rng = np.random.default_rng(2)
+    for _ in range(40):
+        a, z = rng.uniform(-5, 55), rng.uniform(-2, 25)
+        fib = [-10, a, z, 60, a, z] if direction == 0 else [a, -10, z, a, 60, z]
+        ref = m.kd_search_push(cut, fib, direction)
+        ok = (a + R >= acc[:, 0]) & (a - R <= acc[:, 1]) & (z + L >= acc[:, 2]) & (z <= acc[:, 3])
+        got = order[ok]
+        assert np.array_equal(got.astype(np.int64), ref.astype(np.int64))
 
 
 class SelCheck:
