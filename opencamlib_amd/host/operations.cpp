@@ -1,0 +1,337 @@
+// Operation classes of the drop-in host API (see ocl/operations.hpp).
+#include <algorithm>
+#include <cstring>
+#include <sstream>
+
+#include "ocl/operations.hpp"
+
+namespace ocl {
+
+static void check(int rc) {
+    if (rc != OCL_OK) throw Error(std::string("ocl_hip: ") + ocl_hip_last_error());
+}
+
+// ------------------------------------------------------------ Interval / Fiber
+
+Interval::Interval(double l, double u) : upper(u), lower(l) {
+    if (l > u) throw Error("Interval: lower > upper");
+}
+
+void Interval::updateUpper(double t, const CCPoint& p) {
+    if (upper_cc.type == NONE) {
+        upper = lower = t;
+        upper_cc = lower_cc = p;
+    }
+    if (t > upper) {
+        upper = t;
+        upper_cc = p;
+    }
+}
+
+void Interval::updateLower(double t, const CCPoint& p) {
+    if (lower_cc.type == NONE) {
+        upper = lower = t;
+        upper_cc = lower_cc = p;
+    }
+    if (t < lower) {
+        lower = t;
+        lower_cc = p;
+    }
+}
+
+std::string Interval::str() const {
+    std::ostringstream o;
+    o << "[" << lower << ", " << upper << "]";
+    return o.str();
+}
+
+Fiber::Fiber(const Point& a, const Point& b) : p1(a), p2(b) {
+    if (a.z != b.z) throw Error("Fiber: p1 and p2 must have the same z");
+    dir = p2 - p1;
+    dir.normalize();
+}
+
+bool Fiber::contains(const Interval& i) const {
+    for (const Interval& f : ints)
+        if (i.inside(f)) return true;
+    return false;
+}
+
+bool Fiber::missing(const Interval& i) const {
+    for (const Interval& f : ints)
+        if (!i.outside(f)) return false;
+    return true;
+}
+
+// fiber.cpp:58-93: drop empty intervals and intervals strictly inside one
+// already held, append disjoint ones, otherwise replace every overlapping
+// interval and i by their hull (earlier intervals win equal extremes)
+void Fiber::addInterval(const Interval& i) {
+    if (i.empty()) return;
+    if (ints.empty() || missing(i)) {
+        ints.push_back(i);
+        return;
+    }
+    if (contains(i)) return;
+    std::vector<Interval> keep, merged;
+    for (const Interval& f : ints) (f.outside(i) ? keep : merged).push_back(f);
+    merged.push_back(i);
+    Interval hull;
+    for (const Interval& m : merged) {
+        hull.updateLower(m.lower, m.lower_cc);
+        hull.updateUpper(m.upper, m.upper_cc);
+    }
+    keep.push_back(hull);
+    ints.swap(keep);
+}
+
+double Fiber::tval(const Point& p) const {
+    const Point d = p2 - p1;
+    return (p - p1).dot(d) / d.dot(d);
+}
+
+std::string Fiber::str() const {
+    std::ostringstream o;
+    o << "Fiber " << p1.str() << " - " << p2.str() << " with " << ints.size() << " intervals";
+    return o.str();
+}
+
+// ------------------------------------------------------------ device surface
+
+DeviceSurface::~DeviceSurface() {
+    if (mesh) ocl_hip_mesh_destroy(mesh);
+}
+
+void DeviceSurface::build(const STLSurf& s, unsigned bucket) {
+    if (mesh) {
+        ocl_hip_mesh_destroy(mesh);
+        mesh = nullptr;
+    }
+    tris.assign(s.tris.begin(), s.tris.end());
+    const std::vector<double> v = s.flat();
+    check(ocl_hip_mesh_create(v.data(), tris.size(), bucket, &mesh));
+}
+
+void Operation::setSTL(const STLSurf& s) {
+    surf = &s;
+    dev.build(s, bucketSize);
+}
+
+void Operation::require_ready() const {
+    if (!dev.handle()) throw Error("setSTL() must be called before run()");
+    if (!cutter) throw Error("setCutter() must be called before run()");
+}
+
+static std::vector<CLPoint> drop_points(ocl_mesh* m, const MillingCutter& c, std::vector<CLPoint> pts, int precision,
+                                        ocl_stats* st) {
+    const size_t n = pts.size();
+    std::vector<double> xyz(3 * n), z(n), cc(3 * n);
+    std::vector<uint8_t> ty(n);
+    for (size_t i = 0; i < n; ++i) {
+        xyz[3 * i] = pts[i].x;
+        xyz[3 * i + 1] = pts[i].y;
+        xyz[3 * i + 2] = pts[i].z;
+    }
+    const ocl_cutter_desc d = c.desc();
+    check(ocl_hip_drop_batch(m, &d, xyz.data(), n, precision, z.data(), cc.data(), ty.data(), nullptr, nullptr, st,
+                             nullptr));
+    for (size_t i = 0; i < n; ++i) {
+        pts[i].z = z[i];
+        if (ty[i] != NONE) pts[i].cc = CCPoint(cc[3 * i], cc[3 * i + 1], cc[3 * i + 2], (CCType)ty[i]);
+    }
+    return pts;
+}
+
+// ------------------------------------------------------------ BatchDropCutter
+
+void BatchDropCutter::run() {
+    require_ready();
+    clpoints = drop_points(dev.handle(), *cutter, std::move(clpoints), precision, &stats);
+    nCalls = stats.pairs;
+}
+
+// Triangles returned by the reference's KD search for a query box, in its
+// order: the search reaches rank k iff the box reaches the rank's acceptance
+// bounds on both search axes (include/ocl_hip.h, ocl_hip_kd_order).
+static std::vector<Triangle> kd_query(const std::vector<Triangle>& tris, unsigned bucket, int dimset, double a0,
+                                      double a1, double b0, double b1) {
+    std::vector<double> v;
+    v.reserve(9 * tris.size());
+    for (const Triangle& t : tris)
+        for (const Point& q : t.p) {
+            v.push_back(q.x);
+            v.push_back(q.y);
+            v.push_back(q.z);
+        }
+    std::vector<uint32_t> order(tris.size());
+    std::vector<double> acc(4 * tris.size());
+    check(ocl_hip_kd_order(v.data(), tris.size(), dimset, bucket, order.data(), acc.data()));
+    std::vector<Triangle> out;
+    for (size_t r = 0; r < tris.size(); ++r) {
+        const double* q = &acc[4 * r];
+        if (a1 >= q[0] && a0 <= q[1] && b1 >= q[2] && b0 <= q[3]) out.push_back(tris[order[r]]);
+    }
+    return out;
+}
+
+std::vector<Triangle> BatchDropCutter::getTrianglesUnderCutter(const CLPoint& cl, const MillingCutter& c) {
+    if (!dev.handle()) throw Error("setSTL() must be called first");
+    const double r = c.getRadius();
+    return kd_query(dev.triangles(), bucketSize, 0, cl.x - r, cl.x + r, cl.y - r, cl.y + r);
+}
+
+void PointDropCutter::run(CLPoint& cl) {
+    require_ready();
+    cl = drop_points(dev.handle(), *cutter, {cl}, precision, &stats)[0];
+    nCalls = stats.pairs;
+}
+
+// ------------------------------------------------------------ BatchPushCutter
+
+void BatchPushCutter::run() {
+    require_ready();
+    if (dir < 0) throw Error("BatchPushCutter: call setXDirection() or setYDirection() first");
+    const size_t nf = fibers.size();
+    std::vector<double> fib(6 * nf);
+    for (size_t i = 0; i < nf; ++i) {
+        const Fiber& f = fibers[i];
+        const double v[6] = {f.p1.x, f.p1.y, f.p1.z, f.p2.x, f.p2.y, f.p2.z};
+        std::memcpy(&fib[6 * i], v, sizeof(v));
+    }
+    const ocl_cutter_desc d = cutter->desc();
+    std::vector<uint64_t> offs(nf + 1, 0);
+    uint64_t total = 0;
+    const int prec = precision | (exact_filter ? OCL_PUSH_EXACT_FILTER : 0);
+    check(ocl_hip_push_batch(dev.handle(), &d, dir, fib.data(), nf, prec, offs.data(), &total, &stats, nullptr));
+    nCalls = stats.pairs;
+    std::vector<double> lohi(2 * total), cc(8 * total);
+    if (total) check(ocl_hip_push_fetch(dev.handle(), lohi.data(), cc.data(), nullptr));
+    for (size_t i = 0; i < nf; ++i) {
+        Fiber& f = fibers[i];
+        const bool fresh = f.ints.empty();
+        for (uint64_t k = offs[i]; k < offs[i + 1]; ++k) {
+            Interval iv;
+            iv.lower = lohi[2 * k];
+            iv.upper = lohi[2 * k + 1];
+            const double* c = &cc[8 * k];
+            iv.lower_cc = CCPoint(c[0], c[1], c[2], (CCType)(int)c[3]);
+            iv.upper_cc = CCPoint(c[4], c[5], c[6], (CCType)(int)c[7]);
+            if (fresh)
+                f.ints.push_back(iv);
+            else
+                f.addInterval(iv);
+        }
+    }
+}
+
+std::vector<CLPoint> BatchPushCutter::getCLPoints() const {
+    std::vector<CLPoint> out;
+    for (const Fiber& f : fibers)
+        for (const Interval& i : f.ints) {
+            if (i.empty()) continue;
+            out.emplace_back(f.point(i.lower));
+            out.back().cc = i.lower_cc;
+            out.emplace_back(f.point(i.upper));
+            out.back().cc = i.upper_cc;
+        }
+    return out;
+}
+
+std::vector<Triangle> BatchPushCutter::getOverlapTriangles(const Fiber& f) {
+    if (!dev.handle() || !cutter) throw Error("setSTL() and setCutter() must be called first");
+    if (dir < 0) throw Error("BatchPushCutter: no fiber direction set");
+    const double r = cutter->getRadius(), L = cutter->getLength();
+    // X fibers: the YZ tree at (y, z); Y fibers: the XZ tree at (x, z)
+    const double across = dir == 0 ? f.p1.y : f.p1.x;
+    return kd_query(dev.triangles(), bucketSize, dir == 0 ? 1 : 2, across - r, across + r, f.p1.z, f.p1.z + L);
+}
+
+// ------------------------------------------------------------ paths
+
+void PathDropCutter::run() {
+    require_ready();
+    if (!path) throw Error("PathDropCutter: setPath() first");
+    if (!(sampling > 0)) throw Error("PathDropCutter: sampling must be > 0");
+    std::vector<CLPoint> pts;
+    for (const Span* s : path->span_list) {
+        // pathdropcutter.cpp:70-80: num_steps + 1 samples, z = minimumZ
+        const unsigned steps = (unsigned)(s->length2d() / sampling + 1);
+        for (unsigned i = 0; i <= steps; ++i) {
+            const Point p = s->getPoint((double)i / steps);
+            pts.emplace_back(p.x, p.y, minimumZ);
+        }
+    }
+    clpoints = drop_points(dev.handle(), *cutter, std::move(pts), precision, &stats);
+    nCalls = stats.pairs;
+}
+
+namespace {
+struct Seg {
+    size_t span;
+    double t0, t1;
+    CLPoint c0, c1;
+};
+struct Leaf {
+    size_t span;
+    double t1;
+    CLPoint c1;
+};
+}  // namespace
+
+static bool flat3(const CLPoint& a, const CLPoint& b, const CLPoint& c, double cos_limit) {
+    Point v1 = b - a, v2 = c - b;
+    v1.normalize();
+    v2.normalize();
+    return v1.dot(v2) > cos_limit;
+}
+
+void AdaptivePathDropCutter::run() {
+    require_ready();
+    if (!path) throw Error("AdaptivePathDropCutter: setPath() first");
+    std::vector<const Span*> spans(path->span_list.begin(), path->span_list.end());
+    // span ends, all in one batch (adaptivepathdropcutter.cpp:59-69)
+    std::vector<CLPoint> ends;
+    for (const Span* s : spans) {
+        ends.emplace_back(s->getPoint(0.0));
+        ends.emplace_back(s->getPoint(1.0));
+    }
+    ends = drop_points(dev.handle(), *cutter, std::move(ends), precision, &stats);
+    unsigned long long calls = stats.pairs;
+    std::vector<Seg> level;
+    for (size_t k = 0; k < spans.size(); ++k) level.push_back({k, 0.0, 1.0, ends[2 * k], ends[2 * k + 1]});
+    std::vector<Leaf> leaves;
+    // adaptivepathdropcutter.cpp:72-86, one recursion level per batch
+    while (!level.empty()) {
+        std::vector<CLPoint> mids;
+        mids.reserve(level.size());
+        for (const Seg& g : level) mids.emplace_back(spans[g.span]->getPoint(g.t0 + (g.t1 - g.t0) / 2.0));
+        mids = drop_points(dev.handle(), *cutter, std::move(mids), precision, &stats);
+        calls += stats.pairs;
+        std::vector<Seg> next;
+        for (size_t i = 0; i < level.size(); ++i) {
+            const Seg& g = level[i];
+            const double tm = g.t0 + (g.t1 - g.t0) / 2.0;
+            const double step = (g.c1 - g.c0).xyNorm();
+            if (step > sampling || (!flat3(g.c0, mids[i], g.c1, cosLimit) && step > min_sampling)) {
+                next.push_back({g.span, g.t0, tm, g.c0, mids[i]});
+                next.push_back({g.span, tm, g.t1, mids[i], g.c1});
+            } else {
+                leaves.push_back({g.span, g.t1, g.c1});
+            }
+        }
+        level.swap(next);
+    }
+    // depth-first order = by span, then by segment end
+    std::stable_sort(leaves.begin(), leaves.end(), [](const Leaf& a, const Leaf& b) {
+        return a.span != b.span ? a.span < b.span : a.t1 < b.t1;
+    });
+    clpoints.clear();
+    size_t li = 0;
+    for (size_t k = 0; k < spans.size(); ++k) {
+        clpoints.push_back(ends[2 * k]);
+        for (; li < leaves.size() && leaves[li].span == k; ++li) clpoints.push_back(leaves[li].c1);
+    }
+    nCalls = calls;
+}
+
+}  // namespace ocl
