@@ -11,6 +11,8 @@ import os
 
 import numpy as np
 
+from ._runtime import preload_hip_runtime
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libocl_hip.so")
 
@@ -53,6 +55,7 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"native library missing: {LIB_PATH}; run __graft_entry__.build()")
+        preload_hip_runtime()
         L = C.CDLL(LIB_PATH)
         vp, dp, sz = C.c_void_p, C.POINTER(C.c_double), C.c_size_t
         L.ocl_hip_mesh_create.argtypes = [dp, sz, C.c_uint, C.POINTER(C.c_void_p)]

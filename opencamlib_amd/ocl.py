@@ -13,6 +13,8 @@ import importlib.machinery
 import importlib.util
 import os
 
+from ._runtime import preload_hip_runtime
+
 _LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
 
 
@@ -20,6 +22,7 @@ def _load():
     found = sorted(glob.glob(os.path.join(_LIB, "_ocl*.so")))
     if not found:
         raise ImportError(f"native module _ocl missing in {_LIB}; run __graft_entry__.build()")
+    preload_hip_runtime()
     loader = importlib.machinery.ExtensionFileLoader("_ocl", found[0])
     spec = importlib.util.spec_from_file_location("_ocl", found[0], loader=loader)
     mod = importlib.util.module_from_spec(spec)
