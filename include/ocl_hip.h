@@ -128,6 +128,28 @@ int ocl_hip_drop_device(ocl_mesh* mesh, const ocl_cutter_desc* cutter, const dou
                         uint8_t* d_cc_type_out, uint32_t* d_tri_out, uint8_t* d_feat_out,
                         ocl_stats* stats, void* stream);
 
+/* Backward of the drop-cutter.  No counterpart in the reference: it serves the
+ * differentiable path (the torch.autograd.Function of the Python binding and
+ * slope-driven path refinement).  From the active contact a forward call on
+ * the same mesh and cutter recorded (its cc_xyz, cc_type, tri and feat
+ * outputs, device arrays), gives per point dz/dx, dz/dy, dz/dr and the
+ * derivative of z with respect to the contact triangle's vertices.  Device
+ * pointers, asynchronous on stream.
+ *   d_grad_z   upstream dL/dz per point (NULL: all ones)
+ *   d_dz       optional, n x 3: dz/dx, dz/dy, dz/dr per point (dz/dr is NaN
+ *              for composite cutters, which have no single radius)
+ *   d_grad_tri optional, ntri x 9 in input triangle order: ADDS
+ *              sum_i grad_z[i] dz_i/d(p0, p1, p2 xyz); zero it first
+ *   d_grad_r   optional, one double: ADDS sum_i grad_z[i] dz_i/dr
+ * precision: OCL_PREC_FP32 / OCL_PREC_MIXED compute facet and vertex contacts
+ * in FP32 (FP64 where the profile is ill-conditioned) and edge contacts in
+ * FP64; OCL_PREC_FP64 / OCL_PREC_EXACT64 compute everything in FP64.  The
+ * accumulated sums use FP64 atomics: their last bits vary run to run. */
+int ocl_hip_drop_backward(ocl_mesh* mesh, const ocl_cutter_desc* cutter, const double* d_cl_xyz, size_t n,
+                          const double* d_cc_xyz, const uint8_t* d_cc_type, const uint32_t* d_tri,
+                          const uint8_t* d_feat, const double* d_grad_z, int precision, double* d_dz,
+                          double* d_grad_tri, double* d_grad_r, void* stream);
+
 /* Batch push-cutter: replaces BatchPushCutter::run / pushCutter3
  * (batchpushcutter.cpp:136-188) for one fiber direction (dir 0 = X fibers,
  * setXDirection; 1 = Y fibers, setYDirection; batchpushcutter.cpp:55-66).

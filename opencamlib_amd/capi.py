@@ -77,6 +77,9 @@ def lib():
         L.ocl_hip_push_batch.restype = C.c_int
         L.ocl_hip_push_fetch.argtypes = [vp, dp, dp, vp]
         L.ocl_hip_push_fetch.restype = C.c_int
+        L.ocl_hip_drop_backward.argtypes = [vp, C.POINTER(CutterDesc), vp, sz, vp, vp, vp, vp, vp, C.c_int, vp, vp,
+                                            vp, vp]
+        L.ocl_hip_drop_backward.restype = C.c_int
         L.ocl_hip_kd_order.argtypes = [dp, sz, C.c_int, C.c_uint, C.POINTER(C.c_uint32), dp]
         L.ocl_hip_kd_order.restype = C.c_int
         L.ocl_hip_last_error.argtypes = []
@@ -213,3 +216,14 @@ class DeviceMesh:
                                         d_cc or None, d_type or None, d_tri or None, d_feat or None,
                                         C.byref(st) if stats else None, stream))
         return st.as_dict() if stats else None
+
+    def drop_backward(self, spec: dict, d_cl: int, n: int, d_cc: int, d_type: int, d_tri: int, d_feat: int,
+                      d_grad_z=0, precision="mixed", d_dz=0, d_grad_tri=0, d_grad_r=0, stream=None):
+        """Asynchronous backward on device pointers (see ocl_hip_drop_backward):
+        from a forward call's cc/type/tri/feat arrays, writes dz/dx, dz/dy,
+        dz/dr per point to d_dz (n x 3) and adds the upstream-weighted vertex
+        and radius sums to d_grad_tri (ntri x 9) and d_grad_r."""
+        desc = cutter_desc(spec)
+        check(lib().ocl_hip_drop_backward(self.h, C.byref(desc), d_cl, n, d_cc, d_type, d_tri, d_feat,
+                                          d_grad_z or None, PREC[precision], d_dz or None, d_grad_tri or None,
+                                          d_grad_r or None, stream))

@@ -15,6 +15,7 @@
 #include "cutter.hpp"
 #include "drop_exact.hpp"
 #include "drop_fast.hpp"
+#include "drop_grad.hpp"
 #include "kd_rank.hpp"
 #include "mesh.hpp"
 #include "walk.hpp"
@@ -92,6 +93,7 @@ struct ocl_mesh {
     unsigned bucket = 1;
     ocl_amd::DevBuf tri;     // kRec doubles per triangle, rank order
     ocl_amd::DevBuf order;   // rank -> input index
+    ocl_amd::DevBuf rank_of; // input index -> rank
     ocl_amd::DevBuf boxes;   // all hierarchy levels
     ocl_amd::DevBuf levels;  // (count, offset) per level
     uint32_t cnt[ocl_amd::kMaxLevels] = {};
@@ -1457,7 +1459,10 @@ int ocl_hip_mesh_create(const double* tri_xyz, size_t ntri, unsigned bucket, ocl
     std::vector<double> sorted(9 * ntri);
     for (size_t r = 0; r < ntri; ++r) std::memcpy(&sorted[9 * r], tri_xyz + 9 * (size_t)order[r], 9 * sizeof(double));
     int rc;
-    if ((rc = m->tri.ensure(sizeof(double) * kRec * ntri)) || (rc = m->order.ensure(sizeof(uint32_t) * ntri))) {
+    std::vector<uint32_t> rank_of(ntri);
+    for (size_t r = 0; r < ntri; ++r) rank_of[order[r]] = (uint32_t)r;
+    if ((rc = m->tri.ensure(sizeof(double) * kRec * ntri)) || (rc = m->order.ensure(sizeof(uint32_t) * ntri)) ||
+        (rc = m->rank_of.ensure(sizeof(uint32_t) * ntri))) {
         delete m;
         return rc;
     }
@@ -1496,6 +1501,7 @@ int ocl_hip_mesh_create(const double* tri_xyz, size_t ntri, unsigned bucket, ocl
     }
     hipError_t e = hipMemcpy(staging.p, sorted.data(), sizeof(double) * 9 * ntri, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(m->order.p, order.data(), sizeof(uint32_t) * ntri, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(m->rank_of.p, rank_of.data(), sizeof(uint32_t) * ntri, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         delete m;
         return fail(OCL_E_HIP, std::string("mesh upload: ") + hipGetErrorString(e));
@@ -1849,3 +1855,94 @@ int ocl_hip_drop_batch(ocl_mesh* m, const ocl_cutter_desc* desc, const double* c
 }
 
 }  // extern "C"
+
+// ============================================================= backward
+namespace ocl_amd {
+
+// One lane per CL point: derivatives of its dropped z from the recorded
+// active contact (drop_grad.hpp), the per-vertex sums accumulated with FP64
+// atomics (so their summation order, and the last bits, vary run to run).
+template <class T>
+__global__ void __launch_bounds__(256) k_drop_grad(MeshView m, const uint32_t* __restrict__ rank_of, CutterDev cd,
+                                                   const double* __restrict__ cl, const double* __restrict__ cc,
+                                                   const uint8_t* __restrict__ type, const uint32_t* __restrict__ tri,
+                                                   const uint8_t* __restrict__ feat, const double* __restrict__ gz,
+                                                   uint32_t n, double* __restrict__ dz, double* __restrict__ gtri,
+                                                   double* __restrict__ gr_sum) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc_r = 0.0;
+    if (i < n) {
+        const uint32_t t = tri[i];
+        const int ty = type[i];
+        GradOut o{};
+        if (t != 0xffffffffu && t < m.ntri && ty != CC_NONE) {
+            const double* r = m.tri + (size_t)kRec * rank_of[t];
+            const V3 V[3] = {v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), v3(r[6], r[7], r[8])};
+            const double x = cl[3 * (size_t)i], y = cl[3 * (size_t)i + 1];
+            const V3 P = v3(cc[3 * (size_t)i], cc[3 * (size_t)i + 1], cc[3 * (size_t)i + 2]);
+            const int f = feat[i];
+            if (cd.c.kind == K_COMPOSITE) {
+                const int sn = grad_sub_index(cd, f, sqrt(sqr(P.x - x) + sqr(P.y - y)));
+                // constant indices only: a dynamic one would copy the cutter to scratch
+                if (sn == 0) o = contact_grad<T>(cd.sub[0], V, f, ty, x, y, P);
+                else if (sn == 1) o = contact_grad<T>(cd.sub[1], V, f, ty, x, y, P);
+                else if (sn == 2) o = contact_grad<T>(cd.sub[2], V, f, ty, x, y, P);
+                else o = contact_grad<T>(cd.sub[3], V, f, ty, x, y, P);
+                o.gr = NAN;
+            } else {
+                o = contact_grad<T>(cd.c, V, f, ty, x, y, P);
+            }
+            const double g = gz ? gz[i] : 1.0;
+            if (gtri) {
+#pragma unroll
+                for (int j = 0; j < 9; ++j)
+                    if (o.gv[j] != 0.0) atomicAdd(gtri + 9 * (size_t)t + j, g * o.gv[j]);
+            }
+            acc_r = g * o.gr;
+        }
+        if (dz) {
+            dz[3 * (size_t)i] = o.gx;
+            dz[3 * (size_t)i + 1] = o.gy;
+            dz[3 * (size_t)i + 2] = o.gr;
+        }
+    }
+    if (gr_sum) {
+        for (int off = 32; off > 0; off >>= 1) acc_r += __shfl_xor(acc_r, off, 64);
+        if (lane_id() == 0) atomicAdd(gr_sum, acc_r);
+    }
+}
+
+}  // namespace ocl_amd
+
+extern "C" int ocl_hip_drop_backward(ocl_mesh* m, const ocl_cutter_desc* desc, const double* d_cl, size_t n,
+                                     const double* d_cc, const uint8_t* d_type, const uint32_t* d_tri,
+                                     const uint8_t* d_feat, const double* d_grad_z, int precision, double* d_dz,
+                                     double* d_grad_tri, double* d_grad_r, void* stream_) {
+    using namespace ocl_amd;
+    if (!m) return fail(OCL_E_ARG, "mesh is NULL");
+    if (precision < OCL_PREC_EXACT64 || precision > OCL_PREC_MIXED) return fail(OCL_E_ARG, "unknown precision mode");
+    if (n >= 0x7fffffffu) return fail(OCL_E_ARG, "too many points");
+    if (n == 0) return OCL_OK;
+    if (!d_cl || !d_cc || !d_type || !d_tri || !d_feat) return fail(OCL_E_ARG, "forward-pass arrays are NULL");
+    CutterDev cd;
+    int rc = cutter_from_desc(desc, cd);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream_;
+    const uint32_t n32 = (uint32_t)n;
+    const MeshView mv = m->view();
+    const uint32_t* rk = m->rank_of.as<const uint32_t>();
+    const uint32_t nb = (n32 + 255) / 256;
+    if (m->ntri == 0) {
+        // no contact anywhere: zero derivatives
+        if (d_dz) HIP_TRY(hipMemsetAsync(d_dz, 0, 24 * n, st));
+        return OCL_OK;
+    }
+    if (precision == OCL_PREC_FP32 || precision == OCL_PREC_MIXED)
+        k_drop_grad<float><<<nb, 256, 0, st>>>(mv, rk, cd, d_cl, d_cc, d_type, d_tri, d_feat, d_grad_z, n32, d_dz,
+                                               d_grad_tri, d_grad_r);
+    else
+        k_drop_grad<double><<<nb, 256, 0, st>>>(mv, rk, cd, d_cl, d_cc, d_type, d_tri, d_feat, d_grad_z, n32, d_dz,
+                                                d_grad_tri, d_grad_r);
+    HIP_TRY(hipGetLastError());
+    return OCL_OK;
+}

@@ -1770,3 +1770,166 @@ long oc_batch_push(oc_mesh* m, const oc_cutter* c, int dir, const double* fib, l
     if (ncalls_out) *ncalls_out = calls;
     return ret;
 }
+
+/* ============================================================ gradient */
+/* Analytic derivative of the drop height at one CL point, given the active
+   contact the forward pass recorded (triangle, feature code, CC point).  The
+   reference has no gradient; this is the FP64 CPU check of the device
+   backward kernel, and it is itself checked against central finite
+   differences of oc_drop_triangle / oc_batch_drop (tests/test_drop_grad.py).
+
+   With P(s) the contact on the feature (s: none for a vertex, the segment
+   parameter for an edge, the barycentrics for a facet), the drop height is
+   z = max_s P(s).z - h(q(s)) with q = |P_xy - cl_xy|, h the profile height:
+     - smooth profile at q (ball, bull torus and flat, cone flank, cyl inside
+       the disc): envelope theorem, dz/dtheta = partial of P.z - h(q) at the
+       optimum, so dz/dx = h'(q) (P.x - x)/q, dz/dP.z = 1, dz/dP.xy = -dz/dxy,
+       spread over the vertices by the contact's weights, dz/dr = -dh/dr(q);
+     - contact on the rim q = r (cyl edges and facets, cone base): the
+       constraint q^2 = r^2 is active, with multiplier
+       lambda = (grad_s P.z . grad_s c)/|grad_s c|^2, c = q^2 - r^2, so
+       dz/dx = 2 lambda (P.x - x), dz/dP.xy = -2 lambda (P - cl)_xy,
+       dz/dr = -d h(r)/dr + 2 lambda r;
+     - cone tip on a facet (q = 0): z is the plane height under the CL point.
+   out = dz/dx, dz/dy, dz/dr, dz/dp0.xyz, dz/dp1.xyz, dz/dp2.xyz.
+   Composite cutters: the derivatives of the active sub-cutter; dz/dr is NaN
+   (no single radius).  Returns 1 when there is no contact. */
+static void prof_slope(const oc_cutter* s, double q, double* hp, double* hr) {
+    *hp = 0.0;
+    *hr = 0.0;
+    switch (s->kind) {
+    case OC_BALL: {
+        double w = sqrt(sq(s->radius) - sq(q));
+        *hp = q / w;
+        *hr = 1.0 - s->radius / w;
+        break;
+    }
+    case OC_BULL:
+        if (q > s->radius1) {
+            double w = sqrt(sq(s->radius2) - sq(q - s->radius1));
+            *hp = (q - s->radius1) / w;
+            *hr = -*hp; /* d/d(radius) with the corner radius fixed */
+        }
+        break;
+    case OC_CONE: *hp = 1.0 / tan(s->angle); break;
+    default: break;
+    }
+}
+
+static double rim_height_slope(const oc_cutter* s) {
+    switch (s->kind) {
+    case OC_BALL: return 1.0;
+    case OC_CONE: return 1.0 / tan(s->angle);
+    default: return 0.0;
+    }
+}
+
+int oc_drop_grad(const oc_cutter* c, const double* tri9, int feat, int cctype, double x, double y,
+                 const double* cc3, double* out) {
+    for (int k = 0; k < 12; ++k) out[k] = 0.0;
+    if (cctype == OC_NONE || feat < 0) return 1;
+    const int k = feat & 15;
+    P3 V[3];
+    for (int j = 0; j < 3; ++j) V[j] = p3(tri9[3 * j], tri9[3 * j + 1], tri9[3 * j + 2]);
+    const P3 P = p3(cc3[0], cc3[1], cc3[2]);
+    const double dx = P.x - x, dy = P.y - y;
+    const double q = sqrt(dx * dx + dy * dy);
+    /* composite: facets and edges carry their sub-cutter; a vertex contact
+       uses the profile of the sub-cutter at its radius (compositecutter.cpp:58-61) */
+    const oc_cutter* s = c;
+    if (c->kind == OC_COMPOSITE)
+        s = c->sub[(k >= F_VTX0 && k < F_EDGE0) ? comp_radius_to_index(c, q) : ((feat >> 4) & 3)];
+    double w[3] = {0, 0, 0};
+    double fs[2] = {0, 0}, cs[2] = {0, 0};
+    int m = 0;
+    if (k >= F_EDGE0) {
+        const int a = k - F_EDGE0, b = (a + 1) % 3;
+        const P3 e = psub(V[b], V[a]);
+        const double t = ((P.x - V[a].x) * e.x + (P.y - V[a].y) * e.y) / (e.x * e.x + e.y * e.y);
+        w[a] = 1.0 - t;
+        w[b] = t;
+        m = 1;
+        fs[0] = e.z;
+        cs[0] = 2.0 * (dx * e.x + dy * e.y);
+    } else if (k >= F_VTX0) {
+        w[k - F_VTX0] = 1.0;
+    } else {
+        const P3 e1 = psub(V[1], V[0]), e2 = psub(V[2], V[0]);
+        const double det = e1.x * e2.y - e1.y * e2.x;
+        const double px = P.x - V[0].x, py = P.y - V[0].y;
+        const double u = (px * e2.y - py * e2.x) / det, v = (e1.x * py - e1.y * px) / det;
+        w[0] = 1.0 - u - v;
+        w[1] = u;
+        w[2] = v;
+        m = 2;
+        fs[0] = e1.z;
+        fs[1] = e2.z;
+        cs[0] = 2.0 * (dx * e1.x + dy * e1.y);
+        cs[1] = 2.0 * (dx * e2.x + dy * e2.y);
+    }
+    double gx, gy, gr;
+    double gxy_v[2]; /* dz/dP.xy, spread by w */
+    const int tip = (s->kind == OC_CONE) && m == 2 && cctype == OC_FACET_TIP;
+    /* rim contacts: every cyl edge or facet contact; the cone's base facet
+       contact (its own feature code) and cone edge contacts on the base circle */
+    const int rim = m > 0 && (s->kind == OC_CYL ||
+                              (s->kind == OC_CONE && (m == 2 ? k == F_FACET_RIM : q >= s->radius * (1.0 - 1e-9))));
+    if (tip) {
+        /* plane height under the CL point: slope from the facet normal */
+        const P3 n = pcross(psub(V[0], V[1]), psub(V[0], V[2]));
+        gx = -n.x / n.z;
+        gy = -n.y / n.z;
+        gxy_v[0] = -gx;
+        gxy_v[1] = -gy;
+        gr = 0.0;
+    } else if (rim) {
+        const double den = cs[0] * cs[0] + cs[1] * cs[1];
+        const double lam = den > 0.0 ? (fs[0] * cs[0] + fs[1] * cs[1]) / den : 0.0;
+        gx = 2.0 * lam * dx;
+        gy = 2.0 * lam * dy;
+        gxy_v[0] = -gx;
+        gxy_v[1] = -gy;
+        gr = -rim_height_slope(s) + 2.0 * lam * s->radius;
+    } else {
+        double hp, hr;
+        prof_slope(s, q, &hp, &hr);
+        gx = q > 0.0 ? hp * dx / q : 0.0;
+        gy = q > 0.0 ? hp * dy / q : 0.0;
+        gxy_v[0] = -gx;
+        gxy_v[1] = -gy;
+        gr = -hr;
+    }
+    out[0] = gx;
+    out[1] = gy;
+    out[2] = (c->kind == OC_COMPOSITE) ? NAN : gr;
+    for (int j = 0; j < 3; ++j) {
+        out[3 + 3 * j] = gxy_v[0] * w[j];
+        out[4 + 3 * j] = gxy_v[1] * w[j];
+        out[5 + 3 * j] = w[j];
+    }
+    return 0;
+}
+
+/* oc_drop_grad over a batch (OpenMP): tri = input triangle index (-1: none). */
+void oc_batch_drop_grad(const oc_mesh* m, const oc_cutter* c, const double* cl_xyz, long n, const int* tri,
+                        const int* feat, const int* cctype, const double* cc, int nthreads, double* out12) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static)
+#endif
+    for (long i = 0; i < n; ++i) {
+        double* o = out12 + 12 * i;
+        if (tri[i] < 0) {
+            for (int k = 0; k < 12; ++k) o[k] = 0.0;
+            continue;
+        }
+        const Tri* t = &m->t[tri[i]];
+        double q[9];
+        for (int j = 0; j < 3; ++j) {
+            q[3 * j] = t->p[j].x;
+            q[3 * j + 1] = t->p[j].y;
+            q[3 * j + 2] = t->p[j].z;
+        }
+        oc_drop_grad(c, q, feat[i], cctype[i], cl_xyz[3 * i], cl_xyz[3 * i + 1], cc + 3 * i, o);
+    }
+}

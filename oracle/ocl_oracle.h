@@ -103,6 +103,19 @@ int oc_batch_drop(oc_mesh* m, const oc_cutter* c, const double* cl_xyz, long n,
 int oc_drop_triangle(const oc_cutter* c, const double* tri9, double x, double y,
                      double* z_io, double* cc_out, int* cctype_out, int* feat_out);
 
+/* Analytic gradient of the drop height at a CL point from its active contact
+   (feature code and CC point of the forward pass).  Not a reference
+   function: the FP64 check of the device backward kernel.  out (12) = dz/dx,
+   dz/dy, dz/dr (NaN for composites), dz/d(p0, p1, p2 xyz).  Returns 1 when
+   there is no contact. */
+int oc_drop_grad(const oc_cutter* c, const double* tri9, int feat, int cctype, double x, double y,
+                 const double* cc3, double* out);
+
+/* oc_drop_grad over a batch (OpenMP); tri = input triangle index per point
+   (-1: no contact, zero gradient), out12 = 12 doubles per point. */
+void oc_batch_drop_grad(const oc_mesh* m, const oc_cutter* c, const double* cl_xyz, long n, const int* tri,
+                        const int* feat, const int* cctype, const double* cc, int nthreads, double* out12);
+
 /* Batch push-cutter (batchpushcutter.cpp:136-188).  dir 0 = X fibers,
    1 = Y fibers.  fib = 6 doubles per fiber (p1, p2).
    search_mode 0: reference KD superset (no filter);

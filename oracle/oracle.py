@@ -80,6 +80,10 @@ def lib():
         L.oc_push_triangle.restype = C.c_int
         L.oc_fiber_union.argtypes = [dp, C.c_long, C.c_long, dp]
         L.oc_fiber_union.restype = C.c_long
+        L.oc_drop_grad.argtypes = [cp, dp, C.c_int, C.c_int, C.c_double, C.c_double, dp, dp]
+        L.oc_drop_grad.restype = C.c_int
+        L.oc_batch_drop_grad.argtypes = [C.c_void_p, cp, dp, C.c_long, ip, ip, ip, dp, C.c_int, dp]
+        L.oc_batch_drop_grad.restype = None
         _lib = L
     return _lib
 
@@ -196,6 +200,20 @@ class Mesh:
                             _dp(z), _dp(cc), _ip(ty), _ip(tri), _ip(feat), _lp(calls))
         return dict(z=z, cc=cc, type=ty, tri=tri, feat=feat, calls=int(calls[0]))
 
+    def batch_drop_grad(self, cutter: Cutter, cl_xyz, res, nthreads=0):
+        """Analytic FP64 gradient (n, 12) at every point of a batch_drop result
+        res (tri, feat, type, cc): dz/dx, dz/dy, dz/dr, dz/d(p0, p1, p2)."""
+        cl = np.ascontiguousarray(np.asarray(cl_xyz, dtype=np.float64).reshape(-1, 3))
+        n = cl.shape[0]
+        tri = np.ascontiguousarray(res["tri"], dtype=np.int32)
+        feat = np.ascontiguousarray(res["feat"], dtype=np.int32)
+        ty = np.ascontiguousarray(res["type"], dtype=np.int32)
+        cc = np.ascontiguousarray(res["cc"], dtype=np.float64).reshape(-1, 3)
+        out = np.empty((n, 12))
+        lib().oc_batch_drop_grad(self.h, C.byref(cutter.c), _dp(cl), n, _ip(tri), _ip(feat), _ip(ty),
+                                 _dp(cc), nthreads, _dp(out))
+        return out
+
     def batch_push(self, cutter: Cutter, fibers, direction, search_mode=2, nthreads=0, bucket=1):
         fib = np.ascontiguousarray(np.asarray(fibers, dtype=np.float64).reshape(-1, 6))
         nf = fib.shape[0]
@@ -230,6 +248,15 @@ def push_triangle(cutter: Cutter, fib6, tri9):
     out = np.zeros(10)
     ne = lib().oc_push_triangle(C.byref(cutter.c), _dp(f), _dp(t), _dp(out))
     return bool(ne), out
+
+
+def drop_grad(cutter: Cutter, tri9, feat, cctype, x, y, cc3):
+    """Analytic FP64 gradient of one drop result (see oc_drop_grad)."""
+    t = np.ascontiguousarray(np.asarray(tri9, dtype=np.float64).reshape(9))
+    c = np.ascontiguousarray(np.asarray(cc3, dtype=np.float64).reshape(3))
+    out = np.zeros(12)
+    lib().oc_drop_grad(C.byref(cutter.c), _dp(t), int(feat), int(cctype), float(x), float(y), _dp(c), _dp(out))
+    return out
 
 
 def fiber_union(ints10):
