@@ -150,6 +150,15 @@ int ocl_hip_drop_backward(ocl_mesh* mesh, const ocl_cutter_desc* cutter, const d
                           const uint8_t* d_feat, const double* d_grad_z, int precision, double* d_dz,
                           double* d_grad_tri, double* d_grad_r, void* stream);
 
+/* Forward plus slopes with host buffers (no counterpart in the reference; the
+ * input of slope-driven path refinement, which replaces the flatness test of
+ * AdaptivePathDropCutter::adaptive_sample, adaptivepathdropcutter.cpp:72-94):
+ * z_out (n), dz_dxy_out (n x 2: dz/dx, dz/dy of each point's active contact,
+ * 0 where nothing was touched) and optionally the CC point and type. */
+int ocl_hip_drop_slopes(ocl_mesh* mesh, const ocl_cutter_desc* cutter, const double* cl_xyz, size_t n,
+                        int precision, double* z_out, double* dz_dxy_out, double* cc_xyz_out,
+                        uint8_t* cc_type_out, ocl_stats* stats, void* stream);
+
 /* Batch push-cutter: replaces BatchPushCutter::run / pushCutter3
  * (batchpushcutter.cpp:136-188) for one fiber direction (dir 0 = X fibers,
  * setXDirection; 1 = Y fibers, setYDirection; batchpushcutter.cpp:55-66).

@@ -100,7 +100,7 @@ struct ocl_mesh {
     size_t box_off[ocl_amd::kMaxLevels] = {};
     int top = 0;
     // per-call workspace
-    ocl_amd::DevBuf keys, perm, hist, cursor, cl, out_z, out_cc, out_ty, out_tri, out_ft, counters, fastrec;
+    ocl_amd::DevBuf keys, perm, hist, cursor, cl, out_z, out_cc, out_ty, out_tri, out_ft, out_dz, counters, fastrec;
     ocl_amd::DevBuf esc_list, esc_list2, esc_flag, blocksum, qcount, cnt2, cnt3;
     ocl_amd::DevBuf origins;     // frame (BoxOrigin) per level-0 box
     ocl_amd::DevBuf zfloor;      // per CL point: floor of the final z handed to the escalation passes
@@ -1944,5 +1944,43 @@ extern "C" int ocl_hip_drop_backward(ocl_mesh* m, const ocl_cutter_desc* desc, c
         k_drop_grad<double><<<nb, 256, 0, st>>>(mv, rk, cd, d_cl, d_cc, d_type, d_tri, d_feat, d_grad_z, n32, d_dz,
                                                 d_grad_tri, d_grad_r);
     HIP_TRY(hipGetLastError());
+    return OCL_OK;
+}
+
+extern "C" int ocl_hip_drop_slopes(ocl_mesh* m, const ocl_cutter_desc* desc, const double* cl, size_t n,
+                                   int precision, double* z, double* dz_dxy, double* cc, uint8_t* type,
+                                   ocl_stats* stats, void* stream_) {
+    using namespace ocl_amd;
+    if (!m) return fail(OCL_E_ARG, "mesh is NULL");
+    if (n > 0 && (!cl || !z || !dz_dxy)) return fail(OCL_E_ARG, "cl/z/dz_dxy pointers are NULL");
+    hipStream_t st = (hipStream_t)stream_;
+    int rc;
+    if ((rc = m->cl.ensure(24ull * n + 8)) || (rc = m->out_z.ensure(8ull * n + 8)) ||
+        (rc = m->out_cc.ensure(24ull * n + 8)) || (rc = m->out_ty.ensure(n + 8)) ||
+        (rc = m->out_tri.ensure(4ull * n + 8)) || (rc = m->out_ft.ensure(n + 8)) ||
+        (rc = m->out_dz.ensure(24ull * n + 8)))
+        return rc;
+    if (n == 0) {
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        return OCL_OK;
+    }
+    HIP_TRY(hipMemcpyAsync(m->cl.p, cl, 24 * n, hipMemcpyHostToDevice, st));
+    rc = ocl_hip_drop_device(m, desc, m->cl.as<double>(), n, precision, m->out_z.as<double>(), m->out_cc.as<double>(),
+                             m->out_ty.as<uint8_t>(), m->out_tri.as<uint32_t>(), m->out_ft.as<uint8_t>(), stats, stream_);
+    if (rc) return rc;
+    rc = ocl_hip_drop_backward(m, desc, m->cl.as<double>(), n, m->out_cc.as<double>(), m->out_ty.as<uint8_t>(),
+                               m->out_tri.as<uint32_t>(), m->out_ft.as<uint8_t>(), nullptr, precision,
+                               m->out_dz.as<double>(), nullptr, nullptr, stream_);
+    if (rc) return rc;
+    std::vector<double> d3(3 * n);
+    HIP_TRY(hipMemcpyAsync(z, m->out_z.p, 8 * n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(d3.data(), m->out_dz.p, 24 * n, hipMemcpyDeviceToHost, st));
+    if (cc) HIP_TRY(hipMemcpyAsync(cc, m->out_cc.p, 24 * n, hipMemcpyDeviceToHost, st));
+    if (type) HIP_TRY(hipMemcpyAsync(type, m->out_ty.p, n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (size_t i = 0; i < n; ++i) {
+        dz_dxy[2 * i] = d3[3 * i];
+        dz_dxy[2 * i + 1] = d3[3 * i + 1];
+    }
     return OCL_OK;
 }

@@ -195,14 +195,30 @@ class PathDropCutter : public Operation {
 // is not flat and longer than the minimum step.  The recursion is evaluated
 // level by level, all segments of a level in one device batch; the output is
 // the reference's depth-first order.
+//
+// setChordTolerance(tol > 0) replaces the flatness test by slope-driven
+// refinement: every sample carries the slope of z along the path (from the
+// device backward pass).  An interval is closed when the piecewise cubic
+// Hermite curve through its two ends and its middle (heights and slopes)
+// stays within tol/2 of the chord (the other half is the margin for what
+// three samples cannot see), or at the minimum step; otherwise it is
+// split at the middle.  The first samples are spaced at most `sampling`
+// apart; each level of intervals is one device batch.
 class AdaptivePathDropCutter : public PathDropCutter {
   public:
     void setMinSampling(double s) { min_sampling = s; }
     void setCosLimit(double c) { cosLimit = c; }
+    void setChordTolerance(double tol) { chord_tol = tol; }
+    double getChordTolerance() const { return chord_tol; }
+    // recursion levels of the last run (one device batch each)
+    int getLevels() const { return levels; }
     void run() override;
 
   protected:
-    double min_sampling = 0.01, cosLimit = 0.999;
+    void flatness_run();
+    void slope_run();
+    double min_sampling = 0.01, cosLimit = 0.999, chord_tol = 0.0;
+    int levels = 0;
 };
 
 }  // namespace ocl

@@ -1,5 +1,6 @@
 // Operation classes of the drop-in host API (see ocl/operations.hpp).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <sstream>
 
@@ -288,6 +289,13 @@ static bool flat3(const CLPoint& a, const CLPoint& b, const CLPoint& c, double c
 void AdaptivePathDropCutter::run() {
     require_ready();
     if (!path) throw Error("AdaptivePathDropCutter: setPath() first");
+    if (chord_tol > 0)
+        slope_run();
+    else
+        flatness_run();
+}
+
+void AdaptivePathDropCutter::flatness_run() {
     std::vector<const Span*> spans(path->span_list.begin(), path->span_list.end());
     // span ends, all in one batch (adaptivepathdropcutter.cpp:59-69)
     std::vector<CLPoint> ends;
@@ -301,7 +309,9 @@ void AdaptivePathDropCutter::run() {
     for (size_t k = 0; k < spans.size(); ++k) level.push_back({k, 0.0, 1.0, ends[2 * k], ends[2 * k + 1]});
     std::vector<Leaf> leaves;
     // adaptivepathdropcutter.cpp:72-86, one recursion level per batch
+    levels = 0;
     while (!level.empty()) {
+        ++levels;
         std::vector<CLPoint> mids;
         mids.reserve(level.size());
         for (const Seg& g : level) mids.emplace_back(spans[g.span]->getPoint(g.t0 + (g.t1 - g.t0) / 2.0));
@@ -330,6 +340,146 @@ void AdaptivePathDropCutter::run() {
     for (size_t k = 0; k < spans.size(); ++k) {
         clpoints.push_back(ends[2 * k]);
         for (; li < leaves.size() && leaves[li].span == k; ++li) clpoints.push_back(leaves[li].c1);
+    }
+    nCalls = calls;
+}
+
+// ------------------------------------------------ slope-driven refinement
+
+namespace {
+struct SPt {
+    CLPoint p;
+    double m;  // slope of z along the path direction
+};
+struct SSeg {
+    size_t span;
+    double t0, t1;
+    SPt a, b;
+};
+}  // namespace
+
+// unit XY tangent of a span at parameter t (direction of increasing t)
+static Point span_tangent(const Span* s, double t) {
+    Point d;
+    if (s->type() == LineSpanType) {
+        const Line& l = static_cast<const LineSpan*>(s)->line;
+        d = l.p2 - l.p1;
+    } else {
+        const Arc& a = static_cast<const ArcSpan*>(s)->arc;
+        const Point r = a.getPoint(t) - a.c;
+        d = a.dir ? Point(-r.y, r.x, 0) : Point(r.y, -r.x, 0);
+    }
+    d.z = 0;
+    const double l = d.xyNorm();
+    return l > 0 ? d * (1.0 / l) : Point(0, 0, 0);
+}
+
+// Largest departure from the chord (0, z0) - (D, z1) of the piecewise cubic
+// Hermite curve through (0, z0, m0), (D/2, zm, mm), (D, z1, m1), sampled at
+// 1/16 steps: the interval's three heights and three slopes model z on it.
+static double hermite_chord_error(double D, double z0, double zm, double z1, double m0, double mm, double m1) {
+    const double h = D / 2;
+    double e = std::fabs(zm - (z0 + z1) / 2);
+    for (int half = 0; half < 2; ++half) {
+        const double za = half ? zm : z0, zb = half ? z1 : zm;
+        const double ma = half ? mm : m0, mb = half ? m1 : mm;
+        for (int k = 1; k < 8; ++k) {
+            const double u = k / 8.0;
+            const double h00 = (1 + 2 * u) * (1 - u) * (1 - u), h10 = u * (1 - u) * (1 - u);
+            const double h01 = u * u * (3 - 2 * u), h11 = u * u * (u - 1);
+            const double zh = h00 * za + h10 * h * ma + h01 * zb + h11 * h * mb;
+            const double pos = (half + u) / 2;  // fraction of D
+            e = std::max(e, std::fabs(zh - (z0 + (z1 - z0) * pos)));
+        }
+    }
+    return e;
+}
+
+static std::vector<SPt> drop_with_slopes(ocl_mesh* m, const MillingCutter& c, const std::vector<CLPoint>& pts,
+                                         const std::vector<Point>& dirs, int precision, ocl_stats* st) {
+    const size_t n = pts.size();
+    std::vector<double> xyz(3 * n), z(n), g(2 * n), cc(3 * n);
+    std::vector<uint8_t> ty(n);
+    for (size_t i = 0; i < n; ++i) {
+        xyz[3 * i] = pts[i].x;
+        xyz[3 * i + 1] = pts[i].y;
+        xyz[3 * i + 2] = pts[i].z;
+    }
+    const ocl_cutter_desc d = c.desc();
+    check(ocl_hip_drop_slopes(m, &d, xyz.data(), n, precision, z.data(), g.data(), cc.data(), ty.data(), st,
+                              nullptr));
+    std::vector<SPt> out(n);
+    for (size_t i = 0; i < n; ++i) {
+        out[i].p = pts[i];
+        out[i].p.z = z[i];
+        if (ty[i] != NONE) out[i].p.cc = CCPoint(cc[3 * i], cc[3 * i + 1], cc[3 * i + 2], (CCType)ty[i]);
+        out[i].m = g[2 * i] * dirs[i].x + g[2 * i + 1] * dirs[i].y;
+    }
+    return out;
+}
+
+void AdaptivePathDropCutter::slope_run() {
+    if (!(sampling > 0) || !(min_sampling > 0)) throw Error("AdaptivePathDropCutter: sampling steps must be > 0");
+    std::vector<const Span*> spans(path->span_list.begin(), path->span_list.end());
+    // first samples: at most `sampling` apart
+    std::vector<CLPoint> pts;
+    std::vector<Point> dirs;
+    std::vector<std::pair<size_t, double>> where;
+    for (size_t k = 0; k < spans.size(); ++k) {
+        const unsigned steps = std::max(1u, (unsigned)std::ceil(spans[k]->length2d() / sampling));
+        for (unsigned i = 0; i <= steps; ++i) {
+            const double t = (double)i / steps;
+            pts.emplace_back(spans[k]->getPoint(t));
+            dirs.push_back(span_tangent(spans[k], t));
+            where.emplace_back(k, t);
+        }
+    }
+    std::vector<SPt> first = drop_with_slopes(dev.handle(), *cutter, pts, dirs, precision, &stats);
+    unsigned long long calls = stats.pairs;
+    std::vector<SSeg> level;
+    for (size_t i = 0; i + 1 < first.size(); ++i)
+        if (where[i].first == where[i + 1].first)
+            level.push_back({where[i].first, where[i].second, where[i + 1].second, first[i], first[i + 1]});
+    // each level: drop the middles of all open intervals in one batch; an
+    // interval closes when the Hermite model through its ends and middle
+    // stays within the tolerance of its chord, or at the minimum step;
+    // otherwise its halves (whose ends are now known) form the next level
+    std::vector<SSeg> done;
+    levels = 1;
+    while (!level.empty()) {
+        ++levels;
+        pts.clear();
+        dirs.clear();
+        for (const SSeg& g : level) {
+            const double tm = g.t0 + (g.t1 - g.t0) / 2.0;
+            pts.emplace_back(spans[g.span]->getPoint(tm));
+            dirs.push_back(span_tangent(spans[g.span], tm));
+        }
+        std::vector<SPt> mids = drop_with_slopes(dev.handle(), *cutter, pts, dirs, precision, &stats);
+        calls += stats.pairs;
+        std::vector<SSeg> next;
+        for (size_t i = 0; i < level.size(); ++i) {
+            const SSeg& g = level[i];
+            const double D = spans[g.span]->length2d() * (g.t1 - g.t0);
+            const double e = hermite_chord_error(D, g.a.p.z, mids[i].p.z, g.b.p.z, g.a.m, mids[i].m, g.b.m);
+            // half the tolerance: margin for what three samples cannot see
+            if (e <= 0.5 * chord_tol || D <= min_sampling) {
+                done.push_back(g);
+            } else {
+                const double tm = g.t0 + (g.t1 - g.t0) / 2.0;
+                next.push_back({g.span, g.t0, tm, g.a, mids[i]});
+                next.push_back({g.span, tm, g.t1, mids[i], g.b});
+            }
+        }
+        level.swap(next);
+    }
+    std::sort(done.begin(), done.end(), [](const SSeg& a, const SSeg& b) {
+        return a.span != b.span ? a.span < b.span : a.t0 < b.t0;
+    });
+    clpoints.clear();
+    for (size_t i = 0; i < done.size(); ++i) {
+        if (i == 0 || done[i].span != done[i - 1].span) clpoints.push_back(done[i].a.p);
+        clpoints.push_back(done[i].b.p);
     }
     nCalls = calls;
 }

@@ -240,5 +240,8 @@ PYBIND11_MODULE(_ocl, m) {
         .def("run", &AdaptivePathDropCutter::run, py::call_guard<py::gil_scoped_release>())
         .def("setMinSampling", &AdaptivePathDropCutter::setMinSampling)
         .def("setCosLimit", &AdaptivePathDropCutter::setCosLimit)
+        .def("setChordTolerance", &AdaptivePathDropCutter::setChordTolerance)
+        .def("getChordTolerance", &AdaptivePathDropCutter::getChordTolerance)
+        .def("getLevels", &AdaptivePathDropCutter::getLevels)
         .def("getSampling", &AdaptivePathDropCutter::getSampling);
 }

@@ -371,3 +371,70 @@ def test_adaptive_path_drop_cutter_matches_recursive_oracle(gpu, scene):
     want = np.array(out)
     assert got.shape == want.shape
     assert np.array_equal(got, want)
+
+
+def random_polylines(n, extent, seed, nv=(3, 6)):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        k = rng.integers(nv[0], nv[1] + 1)
+        out.append(rng.uniform(0.05 * extent, 0.95 * extent, size=(k, 2)))
+    return out
+
+
+def refine_and_measure(tris, spec, polylines, tol, sampling, min_sampling, z0=-20.0):
+    """Slope-driven refinement of each polyline, then the true chord error of
+    every output interval from 15 extra drops inside it."""
+    s = surf_from(tris)
+    cut = cutter_from(spec)
+    pts_all, segs = [], []
+    levels = 0
+    for poly in polylines:
+        pth = ocl.Path()
+        for a, b in zip(poly[:-1], poly[1:]):
+            pth.append(ocl.Line(ocl.Point(a[0], a[1], z0), ocl.Point(b[0], b[1], z0)))
+        apdc = ocl.AdaptivePathDropCutter()
+        apdc.setSTL(s)
+        apdc.setCutter(cut)
+        apdc.setPath(pth)
+        apdc.setSampling(sampling)
+        apdc.setMinSampling(min_sampling)
+        apdc.setChordTolerance(tol)
+        apdc.run()
+        levels = max(levels, apdc.getLevels())
+        p = np.array([[q.x, q.y, q.z] for q in apdc.getCLPoints()])
+        pts_all.append(p)
+        for i in range(len(p) - 1):
+            if np.hypot(*(p[i + 1, :2] - p[i, :2])) > 0:
+                segs.append((p[i], p[i + 1]))
+    a = np.array([s_[0] for s_ in segs])
+    b = np.array([s_[1] for s_ in segs])
+    tau = np.arange(1, 16) / 16.0
+    probe = (a[:, None, :2] + (b - a)[:, None, :2] * tau[None, :, None]).reshape(-1, 2)
+    cl = np.concatenate([probe, np.full((len(probe), 1), z0)], axis=1)
+    dev = capi.DeviceMesh(tris)
+    zt = dev.drop(spec, cl, "mixed")["z"].reshape(len(segs), 15)
+    dev.close()
+    chord = a[:, None, 2] + (b - a)[:, None, 2] * tau[None, :]
+    err = np.max(np.abs(zt - chord), axis=1)
+    length = np.hypot(*(b - a)[:, :2].T)
+    return sum(len(p) for p in pts_all), err, length, levels
+
+
+@pytest.mark.gpu
+def test_slope_driven_refinement_meets_chord_tolerance(gpu, scene):
+    """setChordTolerance replaces the flatness test: intervals longer than
+    the minimum step meet the chord tolerance (checked by dense drops), and
+    the refinement uses far fewer points than uniform sampling at the
+    minimum step."""
+    tol, min_step = 0.01, 0.01
+    polys = random_polylines(20, 30.0, seed=7)
+    npts, err, length, levels = refine_and_measure(scene, ALL["ball"], polys, tol, 1.0, min_step)
+    long_ = length > 2 * min_step
+    frac = float(np.mean(err[long_] <= tol))
+    total_len = sum(np.sum(np.hypot(*np.diff(p, axis=0).T)) for p in polys)
+    print(f"[refine] points={npts} intervals={len(err)} levels={levels} max_err={err.max():.4f} "
+          f"p99_err={np.percentile(err, 99):.4f} frac_ok(long)={frac:.4f} "
+          f"uniform_points_at_min_step={int(total_len / min_step)}", flush=True)
+    assert frac >= 0.995 and err[long_].max() <= 1.5 * tol
+    assert npts < total_len / min_step / 5
