@@ -159,6 +159,59 @@ int ocl_hip_drop_slopes(ocl_mesh* mesh, const ocl_cutter_desc* cutter, const dou
                         int precision, double* z_out, double* dz_dxy_out, double* cc_xyz_out,
                         uint8_t* cc_type_out, ocl_stats* stats, void* stream);
 
+/* ---------------------------------------------------------------- batches
+ * Many drop-cutter jobs of uneven mesh size, cutter and CL count in one
+ * submission.  No counterpart in the reference, where each job is its own
+ * BatchDropCutter::run (batchdropcutter.cpp:187-231); every job's results
+ * are those ocl_hip_drop_device gives for it alone.  The batch runs
+ *   - one key pass and one deterministic radix sort over all jobs' points,
+ *   - the selection passes of all jobs in ONE persistent launch whose
+ *     wavefronts pull 64-point chunks from a device work queue (largest jobs
+ *     first), or one launch per job with per_job_launch (the baseline),
+ *   - each job's resolve and exact passes,
+ * with no host synchronisation and no memset, so with use_graph the whole
+ * sequence is captured once in a hipGraph and every ocl_hip_batch_run
+ * replays it (re-reading the jobs' CL arrays).  Composite cutters and
+ * OCL_PREC_EXACT64 take the exact pass.  A mesh may appear in one job per
+ * batch (its workspace holds that job's state). */
+typedef struct {
+    ocl_mesh* mesh;
+    ocl_cutter_desc cutter;
+    const double* d_cl_xyz; /* device, n x 3 */
+    size_t n;
+    double* d_z_out;        /* device outputs as ocl_hip_drop_device; all but z may be NULL */
+    double* d_cc_xyz_out;
+    uint8_t* d_cc_type_out;
+    uint32_t* d_tri_out;
+    uint8_t* d_feat_out;
+} ocl_drop_job;
+
+/* Tunables of a batch (zero = default). */
+typedef struct {
+    int32_t hilbert_bits;   /* CL binning: Hilbert curve over 2^bits x 2^bits cells per job domain (4..10, 8) */
+    int32_t blocks_per_cu;  /* persistent selection grid: 256-thread blocks per CU (4) */
+    int32_t use_graph;      /* capture in a hipGraph at create time and replay it in run */
+    int32_t per_job_launch; /* one selection launch per job instead of the work queue */
+} ocl_tune;
+
+typedef struct ocl_batch ocl_batch;
+
+/* Builds the batch: validates the jobs, builds and caches each mesh's
+ * selection records for its cutter (synchronous), uploads the job tables
+ * and, with use_graph, captures the graph. */
+int ocl_hip_batch_create(const ocl_drop_job* jobs, size_t njobs, int precision, const ocl_tune* tune,
+                         ocl_batch** out);
+/* Runs (or replays) the batch on stream; with stats, waits for it and fills
+ * points, pairs, escalated, wave_evals and kernel_ms (select_ms: the
+ * selection launch, eager runs only). */
+int ocl_hip_batch_run(ocl_batch* batch, ocl_stats* stats, void* stream);
+int ocl_hip_batch_destroy(ocl_batch* batch);
+/* Autotuner: times every combination of hilbert_bits {7, 8, 9} and
+ * blocks_per_cu {2, 4, 8} on the jobs themselves (graph replays, best of
+ * `trials` after a warm-up run) and returns the fastest in *best. */
+int ocl_hip_batch_autotune(const ocl_drop_job* jobs, size_t njobs, int precision, int trials, ocl_tune* best,
+                           double* best_ms);
+
 /* Batch push-cutter: replaces BatchPushCutter::run / pushCutter3
  * (batchpushcutter.cpp:136-188) for one fiber direction (dir 0 = X fibers,
  * setXDirection; 1 = Y fibers, setYDirection; batchpushcutter.cpp:55-66).

@@ -47,6 +47,20 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class DropJob(C.Structure):
+    _fields_ = [("mesh", C.c_void_p), ("cutter", CutterDesc), ("d_cl_xyz", C.c_void_p), ("n", C.c_size_t),
+                ("d_z_out", C.c_void_p), ("d_cc_xyz_out", C.c_void_p), ("d_cc_type_out", C.c_void_p),
+                ("d_tri_out", C.c_void_p), ("d_feat_out", C.c_void_p)]
+
+
+class Tune(C.Structure):
+    _fields_ = [("hilbert_bits", C.c_int32), ("blocks_per_cu", C.c_int32), ("use_graph", C.c_int32),
+                ("per_job_launch", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -80,6 +94,18 @@ def lib():
         L.ocl_hip_drop_backward.argtypes = [vp, C.POINTER(CutterDesc), vp, sz, vp, vp, vp, vp, vp, C.c_int, vp, vp,
                                             vp, vp]
         L.ocl_hip_drop_backward.restype = C.c_int
+        L.ocl_hip_drop_slopes.argtypes = [vp, C.POINTER(CutterDesc), dp, sz, C.c_int, dp, dp, dp,
+                                          C.POINTER(C.c_uint8), C.POINTER(Stats), vp]
+        L.ocl_hip_drop_slopes.restype = C.c_int
+        L.ocl_hip_batch_create.argtypes = [C.POINTER(DropJob), sz, C.c_int, C.POINTER(Tune), C.POINTER(C.c_void_p)]
+        L.ocl_hip_batch_create.restype = C.c_int
+        L.ocl_hip_batch_run.argtypes = [vp, C.POINTER(Stats), vp]
+        L.ocl_hip_batch_run.restype = C.c_int
+        L.ocl_hip_batch_destroy.argtypes = [vp]
+        L.ocl_hip_batch_destroy.restype = C.c_int
+        L.ocl_hip_batch_autotune.argtypes = [C.POINTER(DropJob), sz, C.c_int, C.c_int, C.POINTER(Tune),
+                                             C.POINTER(C.c_double)]
+        L.ocl_hip_batch_autotune.restype = C.c_int
         L.ocl_hip_kd_order.argtypes = [dp, sz, C.c_int, C.c_uint, C.POINTER(C.c_uint32), dp]
         L.ocl_hip_kd_order.restype = C.c_int
         L.ocl_hip_last_error.argtypes = []
@@ -227,3 +253,52 @@ class DeviceMesh:
         check(lib().ocl_hip_drop_backward(self.h, C.byref(desc), d_cl, n, d_cc, d_type, d_tri, d_feat,
                                           d_grad_z or None, PREC[precision], d_dz or None, d_grad_tri or None,
                                           d_grad_r or None, stream))
+
+
+def make_jobs(items):
+    """ctypes array of DropJob from dicts {mesh: DeviceMesh, spec, cl, n, z, cc,
+    type, tri, feat} whose arrays are device pointers (ints)."""
+    arr = (DropJob * len(items))()
+    for i, it in enumerate(items):
+        j = arr[i]
+        j.mesh = it["mesh"].h
+        j.cutter = cutter_desc(it["spec"])
+        j.d_cl_xyz = it["cl"]
+        j.n = it["n"]
+        j.d_z_out = it["z"]
+        j.d_cc_xyz_out = it.get("cc") or None
+        j.d_cc_type_out = it.get("type") or None
+        j.d_tri_out = it.get("tri") or None
+        j.d_feat_out = it.get("feat") or None
+    return arr
+
+
+class Batch:
+    """ocl_batch handle (see ocl_hip_batch_create)."""
+
+    def __init__(self, jobs, precision="mixed", tune=None):
+        t = Tune(**(tune or {}))
+        h = C.c_void_p()
+        check(lib().ocl_hip_batch_create(jobs, len(jobs), PREC[precision], C.byref(t), C.byref(h)))
+        self.h = h
+        self.jobs = jobs
+
+    def run(self, stream=None, want_stats=True):
+        st = Stats()
+        check(lib().ocl_hip_batch_run(self.h, C.byref(st) if want_stats else None, stream))
+        return st.as_dict() if want_stats else None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ocl_hip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def autotune(jobs, precision="mixed", trials=3):
+    best = Tune()
+    ms = C.c_double(0)
+    check(lib().ocl_hip_batch_autotune(jobs, len(jobs), PREC[precision], trials, C.byref(best), C.byref(ms)))
+    return best.as_dict(), ms.value

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -449,31 +450,55 @@ struct WaveState {
     T zb, zt[4], za;
 };
 
-// Selection over a list of points.  Writes final results for lanes that
-// need no escalation and flags the others (esc[pos] = 1).  refine = 0 is the
+// Arguments of one selection pass (one job of a batch).
+template <class T>
+struct SelArgs {
+    MeshView m;
+    const T* rec;
+    const BoxOrigin* org;
+    const T* boxcap;
+    SelParams<T> P;
+    double R;                // cutter radius
+    const double* cl;
+    const uint32_t* list;    // points in selection order
+    const uint32_t* count;   // point count in device memory (or NULL: n)
+    uint32_t n;
+    DropOut o;
+    double* zfloor;
+    NearTieSets nts;
+    int refine;
+    Counters* cnt;
+};
+
+// Selection over a list of points, one wavefront of 64 neighbouring points
+// (gid = position in the list).  Writes final results for lanes that need no
+// escalation and the selection records of the others.  refine = 0 is the
 // FP32-only mode: the selection result is written as is.
-// 4 wavefronts per SIMD for the FP32 pass, 3 for the FP64 one (register budget)
+//   my:   the wavefront's LDS copy of the records of the level-0 box being
+//         evaluated (kLeaf * kFastRec values)
+//   nt_k, nt_h: per-lane near-tie sets: rank indices (and upper values, wave
+//         frame) of the triangles whose contacts reach the band below the
+//         lane's known height
 template <class T, int K>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3))) k_drop_select(MeshView m, const T* __restrict__ rec,
-                                                     const BoxOrigin* __restrict__ org,
-                                                     const T* __restrict__ boxcap, SelParams<T> P, CutterDev cd,
-                                                     const double* __restrict__ cl, const uint32_t* __restrict__ list,
-                                                     const uint32_t* __restrict__ count, uint32_t n, DropOut o,
-                                                     double* __restrict__ zfloor, NearTieSets nts, int refine,
-                                                     Counters* cnt) {
+__device__ __attribute__((always_inline)) inline void select_points(const SelArgs<T>& A, uint32_t gid, T* my,
+                                                                   uint32_t (*nt_k)[64], float (*nt_h)[64]) {
     typedef FM<T> F;
-    // per-wave LDS copy of the records of the level-0 box being evaluated
-    __shared__ __align__(16) T srec[4][kLeaf * kFastRec];
-    T* my = srec[threadIdx.x >> 6];
-    // per-lane near-tie set: rank indices (and upper values, wave frame) of
-    // the triangles whose contacts reach the band below the lane's known height
-    __shared__ uint32_t nt_k[4][kNearTie][64];
-    __shared__ float nt_h[4][kNearTie][64];
-    const uint32_t wv = threadIdx.x >> 6, ln = lane_id();
+    const MeshView& m = A.m;
+    const T* __restrict__ rec = A.rec;
+    const BoxOrigin* __restrict__ org = A.org;
+    const T* __restrict__ boxcap = A.boxcap;
+    const SelParams<T>& P = A.P;
+    const double* __restrict__ cl = A.cl;
+    const uint32_t* __restrict__ list = A.list;
+    const DropOut& o = A.o;
+    double* __restrict__ zfloor = A.zfloor;
+    const NearTieSets& nts = A.nts;
+    const int refine = A.refine;
+    Counters* cnt = A.cnt;
+    const uint32_t ln = lane_id();
     uint32_t ntn = 0;
     bool ovf = false;
-    const uint32_t n_eff = count ? *count : n;
-    const uint32_t gid = xcd_block(m.xcd_group) * blockDim.x + threadIdx.x;
+    const uint32_t n_eff = A.count ? *A.count : A.n;
     const uint32_t wave_first = gid & ~63u;
     if (wave_first >= n_eff) return;
     const bool act = gid < n_eff;
@@ -481,7 +506,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
     const double x = act ? cl[3 * (size_t)src] : 0.0;
     const double y = act ? cl[3 * (size_t)src + 1] : 0.0;
     const double z0 = act ? cl[3 * (size_t)src + 2] : 0.0;
-    const double R = cd.c.radius;
+    const double R = A.R;
     const float qx0 = uniform_f(__double2float_rd(wave_min(act ? x : INFINITY) - R));
     const float qy0 = uniform_f(__double2float_rd(wave_min(act ? y : INFINITY) - R));
     const float qx1 = uniform_f(__double2float_ru(wave_max(act ? x : -INFINITY) + R));
@@ -635,11 +660,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
                         const float lim = (float)(F::max_(S.zb + dz, zc) - P.CUT);
                         uint32_t w = 0;
                         for (uint32_t i = 0; i < (uint32_t)kNearTie; ++i) {
-                            const float h = nt_h[wv][i][ln];
-                            const uint32_t kk = nt_k[wv][i][ln];
+                            const float h = nt_h[i][ln];
+                            const uint32_t kk = nt_k[i][ln];
                             if (h >= lim) {
-                                nt_h[wv][w][ln] = h;
-                                nt_k[wv][w][ln] = kk;
+                                nt_h[w][ln] = h;
+                                nt_k[w][ln] = kk;
                                 ++w;
                             }
                         }
@@ -648,8 +673,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
                     if (ntn < (uint32_t)kNearTie) {
                         // upper value in the wave frame, rounded up
                         const float hw = (float)(th + dz);
-                        nt_h[wv][ntn][ln] = hw + 1e-6f * (1.0f + fabsf(hw));
-                        nt_k[wv][ntn][ln] = ku;
+                        nt_h[ntn][ln] = hw + 1e-6f * (1.0f + fabsf(hw));
+                        nt_k[ntn][ln] = ku;
                         ++ntn;
                     } else {
                         ovf = true;
@@ -681,7 +706,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
             nts.zsel[src] = framed ? ozw + (double)D.zb : z0;
             nts.esel[src] = (float)D.eb;
             if (up && usable) {
-                for (uint32_t i = 0; i < ntn; ++i) nts.set[(size_t)src * kNearTie + i] = nt_k[wv][i][ln];
+                for (uint32_t i = 0; i < ntn; ++i) nts.set[(size_t)src * kNearTie + i] = nt_k[i][ln];
                 const double zs0 = ozw + (double)(D.zb - P.CUT);
                 nts.zband[src] = fmax(z0, zs0 - 1e-9 * (1.0 + fabs(zs0)));
             }
@@ -714,6 +739,56 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
     }
 }
 
+
+
+// One selection pass: a grid over the point list (XCD-aware block mapping).
+// 4 wavefronts per SIMD for the FP32 pass, 3 for the FP64 one (register budget)
+template <class T, int K>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3)))
+k_drop_select(SelArgs<T> A) {
+    __shared__ __align__(16) T srec[4][kLeaf * kFastRec];
+    __shared__ uint32_t nt_k[4][kNearTie][64];
+    __shared__ float nt_h[4][kNearTie][64];
+    const uint32_t wv = threadIdx.x >> 6;
+    const uint32_t gid = xcd_block(A.m.xcd_group) * blockDim.x + threadIdx.x;
+    select_points<T, K>(A, gid, srec[wv], nt_k[wv], nt_h[wv]);
+}
+
+// The selection passes of a batch of jobs in one persistent launch: the
+// wavefronts pull 64-point chunks from a device work queue that runs over
+// every job's chunks (chunk_off: prefix sums of the jobs' chunk counts,
+// largest jobs first so the tail is short) and run the job's pass on them.
+// Cutter kinds are dispatched per chunk; composite jobs are not queued.
+template <class T>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3)))
+k_drop_select_jobs(const SelArgs<T>* __restrict__ jobs, const uint32_t* __restrict__ chunk_off, uint32_t njobs,
+                   uint32_t* __restrict__ queue) {
+    __shared__ __align__(16) T srec[4][kLeaf * kFastRec];
+    __shared__ uint32_t nt_k[4][kNearTie][64];
+    __shared__ float nt_h[4][kNearTie][64];
+    const uint32_t wv = threadIdx.x >> 6;
+    const uint32_t nchunks = chunk_off[njobs];
+    for (;;) {
+        uint32_t c = 0;
+        if (lane_id() == 0) c = atomicAdd(queue, 1u);
+        c = uniform_u(__shfl(c, 0, 64));
+        if (c >= nchunks) break;
+        uint32_t lo = 0, hi = njobs;  // the job: last j with chunk_off[j] <= c
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (chunk_off[mid] <= c) lo = mid;
+            else hi = mid;
+        }
+        const SelArgs<T>& A = jobs[lo];
+        const uint32_t gid = (c - chunk_off[lo]) * 64u + lane_id();
+        switch (A.P.kind) {
+        case K_CYL: select_points<T, K_CYL>(A, gid, srec[wv], nt_k[wv], nt_h[wv]); break;
+        case K_BALL: select_points<T, K_BALL>(A, gid, srec[wv], nt_k[wv], nt_h[wv]); break;
+        case K_BULL: select_points<T, K_BULL>(A, gid, srec[wv], nt_k[wv], nt_h[wv]); break;
+        default: select_points<T, K_CONE>(A, gid, srec[wv], nt_k[wv], nt_h[wv]); break;
+        }
+    }
+}
 
 // Resolve pass, one lane per CL point (input order), after a selection pass:
 //  - decided points: the winner's feature is re-evaluated by the exact FP64
@@ -1235,20 +1310,26 @@ static void set_cut(SelParams<T>& P, const RecSet& rs) {
 }
 
 template <class T>
-static int launch_select(ocl_mesh* m, const MeshView& mv, const T* rec, const T* cap, const SelParams<T>& P,
-                         const CutterDev& cd, const double* d_cl, const uint32_t* list, uint32_t n, const DropOut& o,
-                         double* zfloor, const NearTieSets& nts, int refine, Counters* cnt, hipStream_t st) {
-    const uint32_t nb = xcd_grid((n + 255) / 256, mv.xcd_group);
-    const BoxOrigin* org = m->origins.as<BoxOrigin>();
-    switch (cd.c.kind) {
-    case K_CYL: k_drop_select<T, K_CYL><<<nb, 256, 0, st>>>(mv, rec, org, cap, P, cd, d_cl, list, nullptr, n, o, zfloor, nts, refine, cnt); break;
-    case K_BALL: k_drop_select<T, K_BALL><<<nb, 256, 0, st>>>(mv, rec, org, cap, P, cd, d_cl, list, nullptr, n, o, zfloor, nts, refine, cnt); break;
-    case K_BULL: k_drop_select<T, K_BULL><<<nb, 256, 0, st>>>(mv, rec, org, cap, P, cd, d_cl, list, nullptr, n, o, zfloor, nts, refine, cnt); break;
-    case K_CONE: k_drop_select<T, K_CONE><<<nb, 256, 0, st>>>(mv, rec, org, cap, P, cd, d_cl, list, nullptr, n, o, zfloor, nts, refine, cnt); break;
-    default: return fail(OCL_E_UNSUPPORTED, "selection pass has no composite specialisation");
-    }
-    HIP_TRY(hipGetLastError());
-    return OCL_OK;
+static SelArgs<T> sel_args(ocl_mesh* m, const MeshView& mv, const T* rec, const T* cap, const SelParams<T>& P,
+                           const CutterDev& cd, const double* d_cl, const uint32_t* list, uint32_t n,
+                           const DropOut& o, double* zfloor, const NearTieSets& nts, int refine, Counters* cnt) {
+    SelArgs<T> A;
+    A.m = mv;
+    A.rec = rec;
+    A.org = m->origins.as<BoxOrigin>();
+    A.boxcap = cap;
+    A.P = P;
+    A.R = cd.c.radius;
+    A.cl = d_cl;
+    A.list = list;
+    A.count = nullptr;
+    A.n = n;
+    A.o = o;
+    A.zfloor = zfloor;
+    A.nts = nts;
+    A.refine = refine;
+    A.cnt = cnt;
+    return A;
 }
 
 constexpr uint32_t kCoopBlocks = 2048;  // persistent grid of the cooperative tiers
@@ -1266,80 +1347,146 @@ static double env_scale(const char* name, double def) {
 // Cones skip the near-tie and FP64 passes (their result depends on the
 // evaluation order).  Undecided points carry a floor of their final height
 // (zfloor) that lets the later passes skip triangles that cannot matter.
-static int launch_drop_fast(ocl_mesh* m, const MeshView& mv, const CutterDev& cd, const double* d_cl, uint32_t n,
-                            int precision, const DropOut& o, Counters* cnt, hipStream_t st, bool timed) {
+// One job of the fast pipeline: workspace, cached selection records and the
+// arguments of its passes.
+struct FastPlan {
+    ocl_mesh* m;
+    MeshView mv;
+    CutterDev cd;
+    const double* cl;
+    const uint32_t* perm;  // selection order (Hilbert-sorted point indices)
+    uint32_t n;
+    int precision;
+    DropOut o;
+    Counters* cnt;
+    NearTieSets nts;
+    double* zfloor;
+    uint32_t *q1, *q2, *qcount;
+    SelParams<double> P64;
+    SelParams<float> P32;
+    double etie;
+};
+
+// Workspace and records of a job (ensure_records may synchronise the stream
+// once per mesh and cutter: call it before any graph capture).
+static int fast_prepare(ocl_mesh* m, const MeshView& mv, const CutterDev& cd, const double* d_cl,
+                        const uint32_t* perm, uint32_t n, int precision, const DropOut& o, Counters* cnt,
+                        hipStream_t st, FastPlan& F) {
     int rc;
-    const uint32_t* perm = m->perm.as<uint32_t>();
     if ((rc = m->esc_list.ensure(4ull * n + 64)) || (rc = m->esc_list2.ensure(4ull * n + 64)) ||
         (rc = m->zfloor.ensure(8ull * n + 64)) || (rc = m->ntset.ensure(4ull * kNearTie * n + 64)) ||
         (rc = m->ntcnt.ensure(2ull * n + 64)) || (rc = m->zband.ensure(8ull * n + 64)) ||
         (rc = m->selrec.ensure(20ull * n + 64)))
         return rc;
-    double* zfloor = m->zfloor.as<double>();
-    NearTieSets nts;
-    nts.set = m->ntset.as<uint32_t>();
-    nts.cnt = m->ntcnt.as<uint8_t>();
-    nts.bf = nts.cnt + n;
-    nts.zband = m->zband.as<double>();
-    nts.zsel = m->selrec.as<double>();
-    nts.bt = reinterpret_cast<uint32_t*>(nts.zsel + n);
-    nts.esel = reinterpret_cast<float*>(nts.bt + n);
-    uint32_t* q1 = m->esc_list.as<uint32_t>();
-    uint32_t* q2 = m->esc_list2.as<uint32_t>();
-    uint32_t* qcount = m->qcount.as<uint32_t>();
-    HIP_TRY(hipMemsetAsync(qcount, 0, 16, st));
+    F.m = m;
+    F.mv = mv;
+    F.cd = cd;
+    F.cl = d_cl;
+    F.perm = perm;
+    F.n = n;
+    F.precision = precision;
+    F.o = o;
+    F.cnt = cnt;
+    F.zfloor = m->zfloor.as<double>();
+    F.nts.set = m->ntset.as<uint32_t>();
+    F.nts.cnt = m->ntcnt.as<uint8_t>();
+    F.nts.bf = F.nts.cnt + n;
+    F.nts.zband = m->zband.as<double>();
+    F.nts.zsel = m->selrec.as<double>();
+    F.nts.bt = reinterpret_cast<uint32_t*>(F.nts.zsel + n);
+    F.nts.esel = reinterpret_cast<float*>(F.nts.bt + n);
+    F.q1 = m->esc_list.as<uint32_t>();
+    F.q2 = m->esc_list2.as<uint32_t>();
+    F.qcount = m->qcount.as<uint32_t>();
     const double se = env_scale("OCL_SEL_E_SCALE", 1.0);
     const SelErr er64 = make_err<double>(cd, se);
-    SelParams<double> P64 = make_params<double>(cd, er64);
+    F.P64 = make_params<double>(cd, er64);
     if ((rc = ensure_records<double>(m, cd, m->rs64, er64, st))) return rc;
-    set_cut(P64, m->rs64);
-    const BoxOrigin* org = m->origins.as<BoxOrigin>();
-    const double* cap64 = m->rs64.cap.as<double>();
-    const bool mixed = precision == OCL_PREC_MIXED;
-    double etie = (double)P64.E;
-    if (precision == OCL_PREC_FP32 || mixed) {
+    set_cut(F.P64, m->rs64);
+    F.etie = (double)F.P64.E;
+    if (precision == OCL_PREC_FP32 || precision == OCL_PREC_MIXED) {
         const SelErr er32 = make_err<float>(cd, se);
-        SelParams<float> P32 = make_params<float>(cd, er32);
+        F.P32 = make_params<float>(cd, er32);
         if ((rc = ensure_records<float>(m, cd, m->rs32, er32, st))) return rc;
-        set_cut(P32, m->rs32);
-        etie = (double)P32.E;
-        if (timed) HIP_TRY(hipEventRecord(m->ev_s0, st));
-        if ((rc = launch_select<float>(m, mv, m->rs32.rec.as<float>(), m->rs32.cap.as<float>(), P32, cd, d_cl, perm, n,
-                                       o, zfloor, nts, mixed ? 1 : 0, cnt, st)))
-            return rc;
-        if (timed) HIP_TRY(hipEventRecord(m->ev_s1, st));
-        if (!mixed) return OCL_OK;
-    } else {
-        if (timed) HIP_TRY(hipEventRecord(m->ev_s0, st));
-        if ((rc = launch_select<double>(m, mv, m->rs64.rec.as<double>(), cap64, P64, cd, d_cl, perm, n, o, zfloor, nts,
-                                        1, cnt, st)))
-            return rc;
-        if (timed) HIP_TRY(hipEventRecord(m->ev_s1, st));
+        set_cut(F.P32, m->rs32);
+        F.etie = (double)F.P32.E;
     }
-    // resolve pass over all points -> q1 (count slot 0); mixed Cyl/Ball/Bull:
-    // FP64 selection of what remains -> q2 (slot 1); exact pass on the last
-    k_drop_resolve<<<(n + 255) / 256, 256, 0, st>>>(mv, cd, d_cl, perm, n, nts, etie, o, q1, qcount,
+    return OCL_OK;
+}
+
+static SelArgs<float> fast_args32(const FastPlan& F) {
+    return sel_args<float>(F.m, F.mv, F.m->rs32.rec.as<float>(), F.m->rs32.cap.as<float>(), F.P32, F.cd, F.cl,
+                           F.perm, F.n, F.o, F.zfloor, F.nts, F.precision == OCL_PREC_MIXED ? 1 : 0, F.cnt);
+}
+
+static SelArgs<double> fast_args64(const FastPlan& F) {
+    return sel_args<double>(F.m, F.mv, F.m->rs64.rec.as<double>(), F.m->rs64.cap.as<double>(), F.P64, F.cd, F.cl,
+                            F.perm, F.n, F.o, F.zfloor, F.nts, 1, F.cnt);
+}
+
+// Passes after the selection: resolve over all points -> q1 (count slot 0);
+// mixed Cyl/Ball/Bull with OCL_COOP_SELECT: FP64 selection of what remains
+// -> q2 (slot 1); the exact pass on the last list.  The queue counts must be
+// zero when the selection starts.
+static int fast_post(const FastPlan& F, hipStream_t st) {
+    if (F.precision == OCL_PREC_FP32) return OCL_OK;
+    ocl_mesh* m = F.m;
+    const MeshView& mv = F.mv;
+    const CutterDev& cd = F.cd;
+    const uint32_t n = F.n;
+    const bool mixed = F.precision == OCL_PREC_MIXED;
+    k_drop_resolve<<<(n + 255) / 256, 256, 0, st>>>(mv, cd, F.cl, F.perm, n, F.nts, F.etie, F.o, F.q1, F.qcount,
                                                      m->cnt2.as<Counters>());
     HIP_TRY(hipGetLastError());
-    const uint32_t* exact_list = q1;
-    const uint32_t* exact_n = qcount;
+    const uint32_t* exact_list = F.q1;
+    const uint32_t* exact_n = F.qcount;
+    const BoxOrigin* org = m->origins.as<BoxOrigin>();
+    const double* cap64 = m->rs64.cap.as<double>();
     if (mixed && cd.c.kind != K_CONE && env_scale("OCL_COOP_SELECT", 0.0) != 0.0) {
         Counters* c2 = m->cnt2.as<Counters>();
         const double* r64 = m->rs64.rec.as<double>();
+        const SelParams<double>& P64 = F.P64;
+        double* zfloor = F.zfloor;
+        uint32_t *q1 = F.q1, *q2 = F.q2, *qcount = F.qcount;
+        const double* d_cl = F.cl;
+        const DropOut& o = F.o;
         switch (cd.c.kind) {
         case K_CYL: k_drop_coop_select<K_CYL><<<kCoopBlocks, 256, 0, st>>>(mv, r64, org, P64, cd, d_cl, q1, qcount, zfloor, cap64, o, q2, qcount + 1, c2); break;
         case K_BALL: k_drop_coop_select<K_BALL><<<kCoopBlocks, 256, 0, st>>>(mv, r64, org, P64, cd, d_cl, q1, qcount, zfloor, cap64, o, q2, qcount + 1, c2); break;
         default: k_drop_coop_select<K_BULL><<<kCoopBlocks, 256, 0, st>>>(mv, r64, org, P64, cd, d_cl, q1, qcount, zfloor, cap64, o, q2, qcount + 1, c2); break;
         }
         HIP_TRY(hipGetLastError());
-        exact_list = q2;
-        exact_n = qcount + 1;
+        exact_list = F.q2;
+        exact_n = F.qcount + 1;
     }
-    k_drop_coop_exact<<<kCoopBlocks, 256, 0, st>>>(mv, cd, d_cl, exact_list, exact_n, zfloor, cap64, org, o,
+    k_drop_coop_exact<<<kCoopBlocks, 256, 0, st>>>(mv, cd, F.cl, exact_list, exact_n, F.zfloor, cap64, org, F.o,
                                                    m->cnt3.as<Counters>());
     HIP_TRY(hipGetLastError());
-    m->exact_slot = (int)(exact_n - qcount);
+    m->exact_slot = (int)(exact_n - F.qcount);
     return OCL_OK;
+}
+
+static int launch_drop_fast(ocl_mesh* m, const MeshView& mv, const CutterDev& cd, const double* d_cl, uint32_t n,
+                            int precision, const DropOut& o, Counters* cnt, hipStream_t st, bool timed) {
+    FastPlan F;
+    int rc = fast_prepare(m, mv, cd, d_cl, m->perm.as<uint32_t>(), n, precision, o, cnt, st, F);
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(F.qcount, 0, 16, st));
+    if (timed) HIP_TRY(hipEventRecord(m->ev_s0, st));
+    const uint32_t nb = xcd_grid((n + 255) / 256, mv.xcd_group);
+    const bool f32 = precision == OCL_PREC_FP32 || precision == OCL_PREC_MIXED;
+    const SelArgs<float> A32 = f32 ? fast_args32(F) : SelArgs<float>{};
+    const SelArgs<double> A64 = f32 ? SelArgs<double>{} : fast_args64(F);
+    switch (cd.c.kind) {
+    case K_CYL: if (f32) k_drop_select<float, K_CYL><<<nb, 256, 0, st>>>(A32); else k_drop_select<double, K_CYL><<<nb, 256, 0, st>>>(A64); break;
+    case K_BALL: if (f32) k_drop_select<float, K_BALL><<<nb, 256, 0, st>>>(A32); else k_drop_select<double, K_BALL><<<nb, 256, 0, st>>>(A64); break;
+    case K_BULL: if (f32) k_drop_select<float, K_BULL><<<nb, 256, 0, st>>>(A32); else k_drop_select<double, K_BULL><<<nb, 256, 0, st>>>(A64); break;
+    case K_CONE: if (f32) k_drop_select<float, K_CONE><<<nb, 256, 0, st>>>(A32); else k_drop_select<double, K_CONE><<<nb, 256, 0, st>>>(A64); break;
+    default: return fail(OCL_E_UNSUPPORTED, "selection pass has no composite specialisation");
+    }
+    HIP_TRY(hipGetLastError());
+    if (timed) HIP_TRY(hipEventRecord(m->ev_s1, st));
+    return fast_post(F, st);
 }
 
 static CutterP simple_from(const ocl_cutter_simple& s, bool& ok) {
@@ -1984,3 +2131,388 @@ extern "C" int ocl_hip_drop_slopes(ocl_mesh* m, const ocl_cutter_desc* desc, con
     }
     return OCL_OK;
 }
+
+// ============================================================= batches
+// Many drop-cutter jobs (each a mesh, a cutter and a CL array) in one
+// submission: one key pass and one deterministic sort over every job's
+// points, the selection passes of all fast jobs in one persistent launch fed
+// by a device work queue, then each job's resolve and exact passes.  Nothing
+// in the sequence synchronises with the host or records a memset, so it is
+// captured once in a hipGraph and replayed.
+#include "radix.hpp"
+
+namespace ocl_amd {
+
+// Hilbert index of cell (x, y) on a 2^hb x 2^hb grid
+__device__ inline uint32_t hilbert_key(uint32_t x, uint32_t y, uint32_t hb) {
+    const uint32_t side = 1u << hb;
+    uint32_t d = 0;
+    for (uint32_t s = side >> 1; s > 0; s >>= 1) {
+        const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+        d += s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = side - 1u - x;
+                y = side - 1u - y;
+            }
+            const uint32_t t = x;
+            x = y;
+            y = t;
+        }
+    }
+    return d;
+}
+
+struct KeyJob {
+    const double* cl;
+    uint32_t n, base;
+    BinFrame f;
+};
+
+// keys = (job << 2 hb) | Hilbert cell, vals = index inside the job
+__global__ void k_cl_keys_jobs(const KeyJob* __restrict__ kj, uint32_t njobs, uint32_t total, uint32_t hb,
+                               uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (kj[mid].base <= g) lo = mid;
+        else hi = mid;
+    }
+    const KeyJob& J = kj[lo];
+    const uint32_t i = g - J.base;
+    const double cmax = (double)((1u << hb) - 1u);
+    const double fx = (J.cl[3 * (size_t)i] - J.f.x0) * J.f.sx, fy = (J.cl[3 * (size_t)i + 1] - J.f.y0) * J.f.sy;
+    const uint32_t ix = (uint32_t)fmin(fmax(fx, 0.0), cmax), iy = (uint32_t)fmin(fmax(fy, 0.0), cmax);
+    keys[g] = (lo << (2 * hb)) | hilbert_key(ix, iy, hb);
+    vals[g] = i;
+}
+
+// zeroes the device ranges of a table (one block per range)
+struct ZeroRange {
+    uint32_t* p;
+    uint32_t words;
+};
+__global__ void k_zero_ranges(const ZeroRange* __restrict__ r) {
+    const ZeroRange z = r[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < z.words; i += blockDim.x) z.p[i] = 0u;
+}
+
+struct BatchJob {
+    ocl_mesh* m;
+    CutterDev cd;
+    const double* cl;
+    uint32_t n, base;
+    DropOut o;
+    bool fast;
+    FastPlan F;
+    MeshView mv;
+};
+
+}  // namespace ocl_amd
+
+struct ocl_batch {
+    std::vector<ocl_amd::BatchJob> jobs;
+    int precision = OCL_PREC_MIXED;
+    ocl_tune tune{};
+    uint32_t total = 0, hb = 8, key_bits = 16;
+    ocl_amd::DevBuf keys, vals, scratch, keyjobs, zeros, seljobs, chunk_off, queue;
+    uint32_t nzero = 0, nsel = 0, grid = 0;
+    uint32_t* sorted_vals = nullptr;
+    std::vector<uint32_t> sel_order;  // job indices of the queue, largest first
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, s0 = nullptr, s1 = nullptr;
+    ~ocl_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cap) (void)hipStreamDestroy(cap);
+        for (hipEvent_t e : {e0, e1, s0, s1})
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+namespace ocl_amd {
+
+// Issues the whole batch on st: no host synchronisation, no memset.
+static int batch_record(ocl_batch* b, hipStream_t st, bool timed) {
+    const bool f32 = b->precision == OCL_PREC_FP32 || b->precision == OCL_PREC_MIXED;
+    if (timed) HIP_TRY(hipEventRecord(b->e0, st));
+    k_zero_ranges<<<b->nzero, 256, 0, st>>>(b->zeros.as<ZeroRange>());
+    k_cl_keys_jobs<<<(b->total + 255) / 256, 256, 0, st>>>(b->keyjobs.as<KeyJob>(), (uint32_t)b->jobs.size(),
+                                                           b->total, b->hb, b->keys.as<uint32_t>(),
+                                                           b->vals.as<uint32_t>());
+    uint32_t *ko = nullptr, *vo = nullptr;
+    rs_sort(b->keys.as<uint32_t>(), b->vals.as<uint32_t>(), b->total, b->key_bits, b->scratch.as<uint32_t>(), &ko,
+            &vo, st);
+    if (vo != b->sorted_vals) return fail(OCL_E_HIP, "batch: sort buffer mismatch");
+    HIP_TRY(hipGetLastError());
+    for (const BatchJob& J : b->jobs) {
+        if (J.fast || J.n == 0) continue;
+        const uint32_t nb = (J.n + 255) / 256;
+        if (J.m->ntri > 0)
+            k_drop_exact<<<xcd_grid(nb, J.mv.xcd_group), 256, 0, st>>>(J.mv, J.cd, J.cl, b->sorted_vals + J.base,
+                                                                       nullptr, J.n, J.o, J.m->counters.as<Counters>());
+        else
+            k_drop_none<<<nb, 256, 0, st>>>(J.cl, J.n, J.o);
+    }
+    if (timed) HIP_TRY(hipEventRecord(b->s0, st));
+    if (b->nsel) {
+        if (b->tune.per_job_launch) {
+            for (uint32_t j : b->sel_order) {
+                const BatchJob& J = b->jobs[j];
+                const uint32_t nb = xcd_grid((J.n + 255) / 256, J.mv.xcd_group);
+                if (f32) {
+                    const SelArgs<float> A = fast_args32(J.F);
+                    switch (J.cd.c.kind) {
+                    case K_CYL: k_drop_select<float, K_CYL><<<nb, 256, 0, st>>>(A); break;
+                    case K_BALL: k_drop_select<float, K_BALL><<<nb, 256, 0, st>>>(A); break;
+                    case K_BULL: k_drop_select<float, K_BULL><<<nb, 256, 0, st>>>(A); break;
+                    default: k_drop_select<float, K_CONE><<<nb, 256, 0, st>>>(A); break;
+                    }
+                } else {
+                    const SelArgs<double> A = fast_args64(J.F);
+                    switch (J.cd.c.kind) {
+                    case K_CYL: k_drop_select<double, K_CYL><<<nb, 256, 0, st>>>(A); break;
+                    case K_BALL: k_drop_select<double, K_BALL><<<nb, 256, 0, st>>>(A); break;
+                    case K_BULL: k_drop_select<double, K_BULL><<<nb, 256, 0, st>>>(A); break;
+                    default: k_drop_select<double, K_CONE><<<nb, 256, 0, st>>>(A); break;
+                    }
+                }
+            }
+        } else if (f32) {
+            k_drop_select_jobs<float><<<b->grid, 256, 0, st>>>(b->seljobs.as<SelArgs<float>>(),
+                                                               b->chunk_off.as<uint32_t>(), b->nsel,
+                                                               b->queue.as<uint32_t>());
+        } else {
+            k_drop_select_jobs<double><<<b->grid, 256, 0, st>>>(b->seljobs.as<SelArgs<double>>(),
+                                                                b->chunk_off.as<uint32_t>(), b->nsel,
+                                                                b->queue.as<uint32_t>());
+        }
+        HIP_TRY(hipGetLastError());
+    }
+    if (timed) HIP_TRY(hipEventRecord(b->s1, st));
+    for (const BatchJob& J : b->jobs) {
+        if (!J.fast) continue;
+        int rc = fast_post(J.F, st);
+        if (rc) return rc;
+    }
+    if (timed) HIP_TRY(hipEventRecord(b->e1, st));
+    return OCL_OK;
+}
+
+}  // namespace ocl_amd
+
+extern "C" {
+
+int ocl_hip_batch_destroy(ocl_batch* b) {
+    delete b;
+    return OCL_OK;
+}
+
+int ocl_hip_batch_create(const ocl_drop_job* jobs, size_t njobs, int precision, const ocl_tune* tune,
+                         ocl_batch** out) {
+    using namespace ocl_amd;
+    if (!out) return fail(OCL_E_ARG, "out is NULL");
+    *out = nullptr;
+    if (njobs == 0 || !jobs) return fail(OCL_E_ARG, "a batch needs at least one job");
+    if (njobs > 4096) return fail(OCL_E_ARG, "at most 4096 jobs per batch");
+    if (precision < OCL_PREC_EXACT64 || precision > OCL_PREC_MIXED) return fail(OCL_E_ARG, "unknown precision mode");
+    std::unique_ptr<ocl_batch> b(new (std::nothrow) ocl_batch());
+    if (!b) return fail(OCL_E_NOMEM, "out of host memory");
+    b->precision = precision;
+    if (tune) b->tune = *tune;
+    b->hb = b->tune.hilbert_bits ? (uint32_t)b->tune.hilbert_bits : 8u;
+    if (b->hb < 4 || b->hb > 10) return fail(OCL_E_ARG, "hilbert_bits must be in 4..10");
+    const uint32_t bpc = b->tune.blocks_per_cu > 0 ? (uint32_t)b->tune.blocks_per_cu : 4u;
+    uint32_t jbits = 0;
+    while ((1ull << jbits) < njobs) ++jbits;
+    b->key_bits = 2 * b->hb + jbits;
+    if (b->key_bits > 32) return fail(OCL_E_ARG, "too many jobs for this hilbert_bits");
+    std::vector<KeyJob> kj(njobs);
+    std::vector<ZeroRange> zr;
+    uint64_t total = 0;
+    hipStream_t st = nullptr;
+    for (size_t j = 0; j < njobs; ++j) {
+        const ocl_drop_job& q = jobs[j];
+        if (!q.mesh) return fail(OCL_E_ARG, "job " + std::to_string(j) + ": mesh is NULL");
+        for (size_t k = 0; k < j; ++k)
+            if (jobs[k].mesh == q.mesh)
+                return fail(OCL_E_ARG, "job " + std::to_string(j) + ": a mesh can appear once per batch");
+        if (q.n > 0 && (!q.d_cl_xyz || !q.d_z_out)) return fail(OCL_E_ARG, "job " + std::to_string(j) + ": cl/z are NULL");
+        BatchJob J;
+        J.m = q.mesh;
+        int rc = cutter_from_desc(&q.cutter, J.cd);
+        if (rc) return rc;
+        J.cl = q.d_cl_xyz;
+        J.n = (uint32_t)q.n;
+        J.base = (uint32_t)total;
+        J.o = DropOut{q.d_z_out, q.d_cc_xyz_out, q.d_cc_type_out, q.d_tri_out, q.d_feat_out};
+        J.mv = q.mesh->view();
+        J.fast = !(precision == OCL_PREC_EXACT64 || q.mesh->ntri == 0 || J.cd.c.kind == K_COMPOSITE);
+        ocl_mesh* m = q.mesh;
+        if ((rc = m->counters.ensure(sizeof(Counters))) || (rc = m->cnt2.ensure(sizeof(Counters))) ||
+            (rc = m->cnt3.ensure(sizeof(Counters))) || (rc = m->qcount.ensure(16)))
+            return rc;
+        zr.push_back({m->counters.as<uint32_t>(), (uint32_t)(sizeof(Counters) / 4)});
+        zr.push_back({m->cnt2.as<uint32_t>(), (uint32_t)(sizeof(Counters) / 4)});
+        zr.push_back({m->cnt3.as<uint32_t>(), (uint32_t)(sizeof(Counters) / 4)});
+        zr.push_back({m->qcount.as<uint32_t>(), 4u});
+        const double R = J.cd.c.radius;
+        KeyJob& k = kj[j];
+        k.cl = q.d_cl_xyz;
+        k.n = J.n;
+        k.base = J.base;
+        k.f.x0 = m->bbox[0] - R;
+        k.f.y0 = m->bbox[2] - R;
+        const double side = (double)(1u << b->hb);
+        k.f.sx = side / std::max(m->bbox[1] + R - k.f.x0, 1e-9);
+        k.f.sy = side / std::max(m->bbox[3] + R - k.f.y0, 1e-9);
+        total += q.n;
+        b->jobs.push_back(J);
+    }
+    if (total >= 0x7fffffffu) return fail(OCL_E_ARG, "too many CL points in one batch");
+    b->total = (uint32_t)total;
+    int rc;
+    const uint32_t tot = std::max<uint32_t>(b->total, 1u);
+    if ((rc = b->keys.ensure(4ull * tot)) || (rc = b->vals.ensure(4ull * tot)) ||
+        (rc = b->scratch.ensure(4ull * rs_scratch_words(tot))) || (rc = b->queue.ensure(64)))
+        return rc;
+    zr.push_back({b->queue.as<uint32_t>(), 1u});
+    // where the sort leaves the values: ping-pong parity of its passes
+    {
+        const uint32_t passes = (b->key_bits + 7) / 8;
+        const uint32_t ntiles = (tot + kRsTile - 1) / kRsTile;
+        uint32_t* v2 = b->scratch.as<uint32_t>() + (size_t)256 * ntiles + tot;
+        b->sorted_vals = (passes % 2) ? v2 : b->vals.as<uint32_t>();
+    }
+    // per-job plans (records are built and cached here, synchronously)
+    std::vector<uint32_t> sel;
+    for (BatchJob& J : b->jobs) {
+        if (!J.fast || J.n == 0) continue;
+        if ((rc = fast_prepare(J.m, J.mv, J.cd, J.cl, b->sorted_vals + J.base, J.n, precision, J.o,
+                               J.m->counters.as<Counters>(), st, J.F)))
+            return rc;
+        sel.push_back((uint32_t)(&J - b->jobs.data()));
+    }
+    std::stable_sort(sel.begin(), sel.end(), [&](uint32_t a, uint32_t c) { return b->jobs[a].n > b->jobs[c].n; });
+    b->sel_order = sel;
+    b->nsel = (uint32_t)sel.size();
+    std::vector<uint32_t> coff(sel.size() + 1, 0);
+    for (size_t i = 0; i < sel.size(); ++i) coff[i + 1] = coff[i] + (b->jobs[sel[i]].n + 63) / 64;
+    const bool f32 = precision == OCL_PREC_FP32 || precision == OCL_PREC_MIXED;
+    const size_t argsz = f32 ? sizeof(SelArgs<float>) : sizeof(SelArgs<double>);
+    if ((rc = b->seljobs.ensure(argsz * (sel.size() + 1))) || (rc = b->chunk_off.ensure(4 * coff.size())) ||
+        (rc = b->keyjobs.ensure(sizeof(KeyJob) * njobs)) || (rc = b->zeros.ensure(sizeof(ZeroRange) * zr.size())))
+        return rc;
+    if (f32) {
+        std::vector<SelArgs<float>> a;
+        for (uint32_t j : sel) a.push_back(fast_args32(b->jobs[j].F));
+        if (!a.empty()) HIP_TRY(hipMemcpy(b->seljobs.p, a.data(), argsz * a.size(), hipMemcpyHostToDevice));
+    } else {
+        std::vector<SelArgs<double>> a;
+        for (uint32_t j : sel) a.push_back(fast_args64(b->jobs[j].F));
+        if (!a.empty()) HIP_TRY(hipMemcpy(b->seljobs.p, a.data(), argsz * a.size(), hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipMemcpy(b->chunk_off.p, coff.data(), 4 * coff.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->keyjobs.p, kj.data(), sizeof(KeyJob) * njobs, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->zeros.p, zr.data(), sizeof(ZeroRange) * zr.size(), hipMemcpyHostToDevice));
+    b->nzero = (uint32_t)zr.size();
+    int dev = 0, cus = 256;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    b->grid = std::max(1u, std::min((uint32_t)cus * bpc, coff.back()));
+    for (hipEvent_t* e : {&b->e0, &b->e1, &b->s0, &b->s1}) HIP_TRY(hipEventCreate(e));
+    HIP_TRY(hipDeviceSynchronize());
+    if (b->tune.use_graph && b->total > 0) {
+        HIP_TRY(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+        HIP_TRY(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+        rc = batch_record(b.get(), b->cap, false);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(b->cap, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess) return fail(OCL_E_HIP, std::string("batch capture: ") + hipGetErrorString(ec));
+        b->graph = g;
+        HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    }
+    *out = b.release();
+    return OCL_OK;
+}
+
+int ocl_hip_batch_run(ocl_batch* b, ocl_stats* stats, void* stream_) {
+    using namespace ocl_amd;
+    if (!b) return fail(OCL_E_ARG, "batch is NULL");
+    hipStream_t st = (hipStream_t)stream_;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (b->total == 0) return OCL_OK;
+    if (b->exec) {
+        if (stats) HIP_TRY(hipEventRecord(b->e0, st));
+        HIP_TRY(hipGraphLaunch(b->exec, st));
+        if (stats) HIP_TRY(hipEventRecord(b->e1, st));
+    } else {
+        int rc = batch_record(b, st, stats != nullptr);
+        if (rc) return rc;
+    }
+    if (stats) {
+        HIP_TRY(hipEventSynchronize(b->e1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, b->e0, b->e1));
+        stats->kernel_ms = ms;
+        stats->select_ms = ms;
+        if (!b->exec) {
+            HIP_TRY(hipEventElapsedTime(&ms, b->s0, b->s1));
+            stats->select_ms = ms;
+        }
+        stats->points = b->total;
+        for (const BatchJob& J : b->jobs) {
+            Counters hc;
+            HIP_TRY(hipMemcpy(&hc, J.m->counters.p, sizeof(hc), hipMemcpyDeviceToHost));
+            stats->pairs += hc.pairs;
+            stats->escalated += hc.escalated;
+            stats->wave_evals += hc.evals;
+        }
+    }
+    return OCL_OK;
+}
+
+int ocl_hip_batch_autotune(const ocl_drop_job* jobs, size_t njobs, int precision, int trials, ocl_tune* best,
+                           double* best_ms) {
+    using namespace ocl_amd;
+    if (!best) return fail(OCL_E_ARG, "best is NULL");
+    if (trials < 1) trials = 3;
+    const int hbs[3] = {7, 8, 9}, bpcs[3] = {2, 4, 8};
+    double bt = 1e300;
+    ocl_tune pick{};
+    for (int hb : hbs)
+        for (int bpc : bpcs) {
+            ocl_tune t{};
+            t.hilbert_bits = hb;
+            t.blocks_per_cu = bpc;
+            t.use_graph = 1;
+            ocl_batch* b = nullptr;
+            int rc = ocl_hip_batch_create(jobs, njobs, precision, &t, &b);
+            if (rc) return rc;
+            double tmin = 1e300;
+            for (int r = 0; r <= trials && rc == OCL_OK; ++r) {
+                ocl_stats s;
+                rc = ocl_hip_batch_run(b, &s, nullptr);
+                if (r > 0) tmin = std::min(tmin, s.kernel_ms);  // the first run warms up
+            }
+            ocl_hip_batch_destroy(b);
+            if (rc) return rc;
+            if (tmin < bt) {
+                bt = tmin;
+                pick = t;
+            }
+        }
+    *best = pick;
+    if (best_ms) *best_ms = bt;
+    return OCL_OK;
+}
+
+}  // extern "C"

@@ -185,3 +185,44 @@ def subsample(a: np.ndarray, n: int, seed: int) -> np.ndarray:
     rng = np.random.default_rng(seed)
     idx = np.sort(rng.choice(a.shape[0], size=n, replace=False))
     return np.ascontiguousarray(a[idx])
+
+
+def config3_jobs(njobs: int = 64, seed: int = 1003, tri_range=(2_000, 500_000), cl_range=(1_000, 1_000_000)):
+    """Uneven batched jobs: per job a heightfield as config 0 (3 bumps over
+    100 x 100 mm) whose cell count is log-uniform so the mesh has
+    tri_range triangles, a cutter uniform over Cyl/Ball/Bull/Cone/Composite
+    (Cyl core d=6 + Ball rim d=10) with diameter U[3, 20] mm, and a CL grid
+    of log-uniform size in cl_range at step U[0.2, 1] mm from a random
+    origin.  Returns a list of dicts {tris, spec, cl}."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    lo, hi = math.log(tri_range[0]), math.log(tri_range[1])
+    clo, chi = math.log(cl_range[0]), math.log(cl_range[1])
+    kinds = ["cyl", "ball", "bull", "cone", "composite"]
+    for j in range(njobs):
+        ntri = math.exp(rng.uniform(lo, hi))
+        cells = max(2, int(round(math.sqrt(ntri / 2.0))))
+        tris = heightfield(cells, 100.0, bumps=3, steps=0, seed=seed * 1000 + j)
+        kind = kinds[int(rng.integers(0, 5))]
+        d = float(rng.uniform(3.0, 20.0))
+        if kind == "cyl":
+            spec = {"kind": "cyl", "d": d, "L": 20.0}
+        elif kind == "ball":
+            spec = {"kind": "ball", "d": d, "L": 20.0}
+        elif kind == "bull":
+            spec = {"kind": "bull", "d": d, "r": 0.2 * d, "L": 20.0}
+        elif kind == "cone":
+            spec = {"kind": "cone", "d": d, "angle": math.pi / 4, "L": 10.0}
+        else:
+            spec = {"kind": "composite", "L": 20.0, "subs": [
+                {"cutter": {"kind": "cyl", "d": 6.0, "L": 20.0}, "r": 3.0, "h": 0.0, "zoff": 0.0},
+                {"cutter": {"kind": "ball", "d": 10.0, "L": 20.0}, "r": 5.0, "h": 4.0, "zoff": -1.0}]}
+        n = int(math.exp(rng.uniform(clo, chi)))
+        step = float(rng.uniform(0.2, 1.0))
+        side = max(1, int(math.ceil(math.sqrt(n))))
+        span = side * step
+        ox = float(rng.uniform(0.0, max(0.0, 100.0 - span)))
+        oy = float(rng.uniform(0.0, max(0.0, 100.0 - span)))
+        cl = cl_grid(side, side, step, z0=-10.0, origin=(ox, oy))[:n]
+        jobs.append({"tris": tris, "spec": spec, "cl": np.ascontiguousarray(cl)})
+    return jobs
