@@ -12,7 +12,10 @@ and FP64 refinement of the winning contact).
 Beside it, in the same run: the FP64-only and FP32-only device paths, the
 reference-order exact FP64 device path, the accuracy of each against the FP64
 oracle on a seeded sample, and the oracle itself timed on the host cores on a
-bounded sample (the ``cpu_baseline``).  Prints one JSON line.
+bounded sample (the ``cpu_baseline``).  Unless ``--no-extra``: the backward
+pass on 1M random points of the same mesh, the push-cutter on the 80,000
+waterline fibers of the sphere scene (Bull and Cone), and 64 uneven batched
+jobs (persistent graph launch vs per-job launches).  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -106,6 +109,190 @@ def accuracy(got, ref, idx):
             "type_mismatch": int(np.sum(got["type"][idx] != ref["type"])), "sample": int(len(idx))}
 
 
+def cuda_ms(torch, fn, reps):
+    """Mean device time of fn() over reps calls, HIP events on torch's
+    current stream (where fn launches)."""
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def bench_backward(torch, capi, W, mesh, ntri, n, steps):
+    """Differentiable drop on the 524k-triangle mesh: 1M uniform-random CL
+    points, BallCutter d=10, FP32 forward (mixed) + FP32 backward with FP64
+    edges, upstream gradient N(0,1)."""
+    spec = SPECS["ball"]
+    cl = W.cl_random(n, 200.0, z0=-10.0, seed=44)
+    d_cl = torch.from_numpy(cl).cuda()
+    z = torch.empty(n, dtype=torch.float64, device="cuda")
+    cc = torch.empty((n, 3), dtype=torch.float64, device="cuda")
+    ty = torch.empty(n, dtype=torch.uint8, device="cuda")
+    tri = torch.empty(n, dtype=torch.int32, device="cuda")
+    ft = torch.empty(n, dtype=torch.uint8, device="cuda")
+    gz = torch.randn(n, dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(4))
+    dz = torch.empty((n, 3), dtype=torch.float64, device="cuda")
+    gtri = torch.zeros((ntri, 9), dtype=torch.float64, device="cuda")
+    gr = torch.zeros(1, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def fwd():
+        mesh.drop_device(spec, d_cl.data_ptr(), n, "mixed", z.data_ptr(), cc.data_ptr(), ty.data_ptr(),
+                         tri.data_ptr(), ft.data_ptr(), stream=stream)
+
+    def bwd():
+        mesh.drop_backward(spec, d_cl.data_ptr(), n, cc.data_ptr(), ty.data_ptr(), tri.data_ptr(), ft.data_ptr(),
+                           d_grad_z=gz.data_ptr(), precision="mixed", d_dz=dz.data_ptr(),
+                           d_grad_tri=gtri.data_ptr(), d_grad_r=gr.data_ptr(), stream=stream)
+
+    fwd()
+    bwd()
+    torch.cuda.synchronize()
+    f_ms = cuda_ms(torch, fwd, steps)
+    b_ms = cuda_ms(torch, bwd, steps)
+    # algorithmic HBM bytes per point of the backward kernel: CL xyz, CC xyz,
+    # type/tri/feat, upstream, 9 vertex doubles of the contact triangle, dz/dx,
+    # dz/dy, dz/dr, and the 9 read-modify-write FP64 vertex sums
+    bpp = 24 + 24 + 6 + 8 + 72 + 24 + 9 * 16
+    return {"workload": f"524,288-triangle heightfield, BallCutter d=10, {n} uniform-random CL points, "
+                        "upstream gradient N(0,1)",
+            "forward_ms": f_ms, "backward_ms": b_ms, "value": n / ((f_ms + b_ms) / 1e3),
+            "unit": "CL pts/s (forward + backward)",
+            "backward_roofline": {"bound": "hbm", "bytes_per_point": bpp,
+                                  "achieved": n * bpp / (b_ms * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
+                                  "frac": n * bpp / (b_ms * 1e-3) / 1e9 / 8000.0}}
+
+
+PUSH_SPECS = {"bull": {"kind": "bull", "d": 10.0, "r": 2.0, "L": 20.0},
+              "cone": {"kind": "cone", "d": 10.0, "angle": np.pi / 4, "L": 10.0}}
+
+
+def bench_push(capi, W, cpu_fibers, do_cpu):
+    """Waterline fibers on the 204,802-triangle sphere scene: 2,000 X and
+    2,000 Y fibers at each of 20 levels (80,000 fibers) per cutter, the
+    reference's KD-search candidate set, FP64.  Kernel time from HIP events
+    on the launch stream (ocl_stats.kernel_ms), summed over the 40 calls."""
+    tris = W.config2_mesh()
+    levels = W.config2_levels()
+    dev = capi.DeviceMesh(tris)
+    out = {"workload": f"{len(tris)}-triangle sphere scene, 20 levels x (2000 X + 2000 Y fibers)"}
+    om = None
+    for name, spec in PUSH_SPECS.items():
+        fibs = {"x": [], "y": []}
+        for z in levels:
+            xf, yf = W.waterline_fibers(tris, spec["d"] / 2, z, 2000)
+            fibs["x"].append(xf)
+            fibs["y"].append(yf)
+        fibs = {d: np.concatenate(v) for d, v in fibs.items()}
+        for d in ("x", "y"):  # warm-up: builds the direction's tree
+            dev.push(spec, fibs[d][:64], d)
+        kms = 0.0
+        nint = pairs = brent = 0
+        t0 = time.perf_counter()
+        for d in ("x", "y"):
+            r = dev.push(spec, fibs[d], d)
+            kms += r["stats"]["kernel_ms"]
+            nint += len(r["lohi"])
+            pairs += r["stats"]["pairs"]
+            brent += r["stats"]["brent_iters"]
+        wall = time.perf_counter() - t0
+        nf = len(fibs["x"]) + len(fibs["y"])
+        line = {"fibers": nf, "intervals": nint, "pairs": int(pairs), "brent_iters": int(brent),
+                "kernel_ms": kms, "fibers_per_s": nf / (kms / 1e3), "intervals_per_s": nint / (kms / 1e3),
+                "wall_s_incl_host": wall}
+        if do_cpu:
+            from oracle import oracle as O
+            if om is None:
+                om = O.Mesh(tris)
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            idx = np.sort(np.random.default_rng(7).choice(len(fibs["x"]), cpu_fibers, replace=False))
+            t0 = time.perf_counter()
+            om.batch_push(O.cutter_from_spec(spec), fibs["x"][idx], 0, search_mode=0, nthreads=threads)
+            tc = time.perf_counter() - t0
+            line["cpu_baseline"] = {"value": cpu_fibers / tc, "unit": "fibers/s", "cores": threads, "kind": "port",
+                                    "sample": f"{cpu_fibers} seeded X fibers of the same set, FP64 oracle with "
+                                              "the reference's KD-search candidates"}
+        out[name] = line
+        log(f"push {name}: {nf} fibers, {nint} intervals, kernel {kms:.1f} ms")
+    dev.close()
+    return out
+
+
+def bench_batch(torch, capi, W, steps):
+    """64 uneven jobs (2k-500k-triangle heightfields, all five cutter kinds,
+    1k-1M CL points each) in mixed precision: autotuned persistent launch in
+    a hipGraph vs one selection launch per job vs one ocl_hip_drop_device
+    call per job."""
+    t0 = time.perf_counter()
+    jobs = W.config3_jobs(64)
+    meshes = [capi.DeviceMesh(j["tris"]) for j in jobs]
+    t_build = time.perf_counter() - t0
+    cls = [torch.from_numpy(j["cl"]).cuda() for j in jobs]
+    outs = []
+    for j in jobs:
+        n = len(j["cl"])
+        outs.append([torch.empty(n, dtype=torch.float64, device="cuda"),
+                     torch.empty((n, 3), dtype=torch.float64, device="cuda"),
+                     torch.empty(n, dtype=torch.uint8, device="cuda"),
+                     torch.empty(n, dtype=torch.int32, device="cuda"),
+                     torch.empty(n, dtype=torch.uint8, device="cuda")])
+    items = [{"mesh": m, "spec": j["spec"], "cl": c.data_ptr(), "n": len(j["cl"]),
+              "z": o[0].data_ptr(), "cc": o[1].data_ptr(), "type": o[2].data_ptr(), "tri": o[3].data_ptr(),
+              "feat": o[4].data_ptr()} for m, j, c, o in zip(meshes, jobs, cls, outs)]
+    cj = capi.make_jobs(items)
+    total = sum(len(j["cl"]) for j in jobs)
+    t0 = time.perf_counter()
+    best, best_ms = capi.autotune(cj, "mixed", trials=2)
+    t_tune = time.perf_counter() - t0
+    log(f"batch: {len(jobs)} jobs, {sum(len(j['tris']) for j in jobs)} triangles, {total} CL points; "
+        f"meshes {t_build:.1f} s, autotune {t_tune:.1f} s -> {best} ({best_ms:.2f} ms)")
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def timed(batch):
+        batch.run(stream=stream, want_stats=False)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            batch.run(stream=stream, want_stats=False)
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t) / steps
+
+    b = capi.Batch(cj, "mixed", {**best, "use_graph": 1})
+    ms_graph = timed(b)
+    b.close()
+    b = capi.Batch(cj, "mixed", {"per_job_launch": 1})
+    ms_perjob = timed(b)
+    b.close()
+
+    def singles():
+        for m, j, c, o in zip(meshes, jobs, cls, outs):
+            m.drop_device(j["spec"], c.data_ptr(), len(j["cl"]), "mixed", *(t.data_ptr() for t in o),
+                          stream=stream)
+
+    singles()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        singles()
+    torch.cuda.synchronize()
+    ms_single = 1e3 * (time.perf_counter() - t) / steps
+    for m in meshes:
+        m.close()
+    kinds = {}
+    for j in jobs:
+        kinds[j["spec"]["kind"]] = kinds.get(j["spec"]["kind"], 0) + 1
+    return {"workload": f"{len(jobs)} jobs, {sum(len(j['tris']) for j in jobs)} triangles, {total} CL points, "
+                        f"cutters {kinds}",
+            "tune": best, "autotune_s": t_tune,
+            "graph_persistent": {"ms_per_step": ms_graph, "value": total / (ms_graph / 1e3)},
+            "per_job_launch": {"ms_per_step": ms_perjob, "value": total / (ms_perjob / 1e3)},
+            "call_per_job": {"ms_per_step": ms_single, "value": total / (ms_single / 1e3)},
+            "unit": "CL pts/s", "speedup_vs_call_per_job": ms_single / ms_graph}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,10 +300,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cutter", default="ball", choices=sorted(SPECS))
     ap.add_argument("--side", type=int, default=1000, help="CL grid side (1000 = 1M points)")
-    ap.add_argument("--cpu-kd-points", type=int, default=3000)
+    ap.add_argument("--cpu-kd-points", type=int, default=100_000)
     ap.add_argument("--cpu-scan-points", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64/FP32/exact device lines")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the backward, push and batched-jobs sections")
+    ap.add_argument("--cpu-push-fibers", type=int, default=200)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run 3 device copies of 256 MiB (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -218,7 +408,16 @@ def main():
             acc[mode] = accuracy(r, cb["r"], cb["idx"])
         acc["mixed_vs_kd_sample"] = accuracy(res_mixed, cb["r_kd"], cb["idx_kd"])
         out["accuracy_vs_fp64_oracle"] = acc
+    if not args.no_extra:
+        out["backward"] = bench_backward(torch, capi, W, mesh, len(tris), 1_000_000, max(3, args.steps // 2))
+        log(f"backward: forward {out['backward']['forward_ms']:.2f} ms, backward {out['backward']['backward_ms']:.3f} ms")
     mesh.close()
+    if not args.no_extra:
+        out["push"] = bench_push(capi, W, args.cpu_push_fibers, not args.no_cpu)
+        out["batch"] = bench_batch(torch, capi, W, max(3, args.steps // 2))
+        log(f"batch: graph {out['batch']['graph_persistent']['ms_per_step']:.2f} ms, per-job launch "
+            f"{out['batch']['per_job_launch']['ms_per_step']:.2f} ms, call per job "
+            f"{out['batch']['call_per_job']['ms_per_step']:.2f} ms")
     print(json.dumps(out), flush=True)
 
 

@@ -6,8 +6,9 @@
 // A pass over n pairs:
 //   k_rs_hist    one block per tile of kRsTile pairs: the tile's digit counts
 //                -> hist[digit * ntiles + tile]
-//   k_rs_scan    one block: exclusive scan of hist in place (digit-major, so
-//                a digit's tiles follow each other in input order)
+//   k_rs_scan_rows  one block per digit: exclusive scan of the digit's
+//                tile counts in place (tiles in input order) and its total;
+//   k_rs_scan_tops  exclusive scan of the 256 digit totals
 //   k_rs_scatter one block per tile, four rounds of 256 pairs in input order:
 //                rank among equal digits inside a wavefront from eight
 //                ballots, then across the block's wavefronts and rounds, so
@@ -38,12 +39,16 @@ __global__ void __launch_bounds__(256) k_rs_hist(const uint32_t* __restrict__ ke
     hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// exclusive scan of v[0..total) by one block of 1024 threads
-__global__ void __launch_bounds__(1024) k_rs_scan(uint32_t* __restrict__ v, uint32_t total) {
+// Exclusive scan of each digit's row hist[d * ntiles .. (d + 1) * ntiles)
+// in place, one block of 1024 threads per digit; the row totals go to
+// rowsum[d].
+__global__ void __launch_bounds__(1024) k_rs_scan_rows(uint32_t* __restrict__ hist, uint32_t ntiles,
+                                                       uint32_t* __restrict__ rowsum) {
     __shared__ uint32_t part[1024];
+    uint32_t* v = hist + (size_t)blockIdx.x * ntiles;
     const uint32_t t = threadIdx.x;
-    const uint32_t per = (total + 1023) / 1024;
-    const uint32_t b = min(t * per, total), e = min(b + per, total);
+    const uint32_t per = (ntiles + 1023) / 1024;
+    const uint32_t b = min(t * per, ntiles), e = min(b + per, ntiles);
     uint32_t s = 0;
     for (uint32_t i = b; i < e; ++i) s += v[i];
     part[t] = s;
@@ -60,16 +65,34 @@ __global__ void __launch_bounds__(1024) k_rs_scan(uint32_t* __restrict__ v, uint
         v[i] = run;
         run += x;
     }
+    if (t == 1023) rowsum[blockIdx.x] = part[1023];
+}
+
+// exclusive scan of the 256 row totals -> rowoff (one block of 256 threads)
+__global__ void __launch_bounds__(256) k_rs_scan_tops(const uint32_t* __restrict__ rowsum,
+                                                      uint32_t* __restrict__ rowoff) {
+    __shared__ uint32_t part[256];
+    const uint32_t t = threadIdx.x;
+    const uint32_t mine = rowsum[t];
+    part[t] = mine;
+    __syncthreads();
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+        const uint32_t a = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += a;
+        __syncthreads();
+    }
+    rowoff[t] = part[t] - mine;
 }
 
 __global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                     uint32_t n, uint32_t shift, const uint32_t* __restrict__ off,
-                                                    uint32_t ntiles, uint32_t* __restrict__ kout,
-                                                    uint32_t* __restrict__ vout) {
+                                                    const uint32_t* __restrict__ rowoff, uint32_t ntiles,
+                                                    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
     __shared__ uint32_t wcnt[4][256];  // per-wavefront digit counts of the round
     __shared__ uint32_t run[256];      // pairs of each digit placed by earlier rounds
     const uint32_t t = threadIdx.x, wv = t >> 6, ln = t & 63u;
-    run[t] = off[t * ntiles + blockIdx.x];
+    run[t] = rowoff[t] + off[t * ntiles + blockIdx.x];
     const uint32_t base = blockIdx.x * kRsTile;
     for (uint32_t r = 0; r < kRsRounds; ++r) {
         for (uint32_t w = 0; w < 4; ++w) wcnt[w][t] = 0;
@@ -101,11 +124,11 @@ __global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t* __restrict__
     }
 }
 
-// Scratch words needed by rs_sort for n pairs: hist (256 x ntiles) plus two
-// ping-pong buffers of n keys and n values.
+// Scratch words needed by rs_sort for n pairs: hist (256 x ntiles), two
+// ping-pong buffers of n keys and n values, row totals and row offsets.
 inline size_t rs_scratch_words(uint32_t n) {
     const uint32_t ntiles = (n + kRsTile - 1) / kRsTile;
-    return (size_t)256 * ntiles + 4ull * n + 64;
+    return (size_t)256 * ntiles + 2ull * n + 512 + 64;
 }
 
 // Sorts (keys, vals) by the low `bits` of the key; the result ends up in
@@ -116,11 +139,14 @@ inline void rs_sort(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t bits, u
     uint32_t* hist = scratch;
     uint32_t* k2 = scratch + (size_t)256 * ntiles;
     uint32_t* v2 = k2 + n;
+    uint32_t* rowsum = v2 + n;
+    uint32_t* rowoff = rowsum + 256;
     uint32_t *ka = keys, *va = vals, *kb = k2, *vb = v2;
     for (uint32_t shift = 0; shift < bits; shift += 8) {
         k_rs_hist<<<ntiles, 256, 0, st>>>(ka, n, shift, hist, ntiles);
-        k_rs_scan<<<1, 1024, 0, st>>>(hist, 256 * ntiles);
-        k_rs_scatter<<<ntiles, 256, 0, st>>>(ka, va, n, shift, hist, ntiles, kb, vb);
+        k_rs_scan_rows<<<256, 1024, 0, st>>>(hist, ntiles, rowsum);
+        k_rs_scan_tops<<<1, 256, 0, st>>>(rowsum, rowoff);
+        k_rs_scatter<<<ntiles, 256, 0, st>>>(ka, va, n, shift, hist, rowoff, ntiles, kb, vb);
         uint32_t* tk = ka;
         ka = kb;
         kb = tk;

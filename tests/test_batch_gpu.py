@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("z", "cc", "type", "tri", "feat")
 
 
-def small_jobs(seed=5, njobs=9):
+def small_jobs(seed=9, njobs=9):
     return W.config3_jobs(njobs, seed=seed, tri_range=(200, 20_000), cl_range=(100, 6_000))
 
 
