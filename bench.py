@@ -103,6 +103,18 @@ def traffic_from_profile(cutter):
     return None
 
 
+def valu_from_profile(cutter):
+    """VALU busy fraction and executed FP32 TFLOP/s of the selection kernel,
+    from the committed rocprofv3 SQ counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"drop_select_{cutter}_c1_valu_counters.md")
+    got = {}
+    if os.path.exists(path):
+        for line in open(path):
+            if line.startswith(("VALU_BUSY_FRACTION", "EXECUTED_FP32_TFLOPS")):
+                got[line.split()[0]] = float(line.split()[-1])
+    return got or None
+
+
 def accuracy(got, ref, idx):
     dz = np.abs(got["z"][idx] - ref["z"])
     return {"max_dz_mm": float(dz.max()), "p99_dz_mm": float(np.percentile(dz, 99)),
@@ -374,6 +386,11 @@ def main():
                 "lane_utilisation": pairs / (64.0 * st["wave_evals"]) if st["wave_evals"] else None,
                 "kernel_ms": sel_ms, "traffic": traffic_from_profile(args.cutter),
                 "traffic_source": f"profiles/drop_select_{args.cutter}_c1_hbm_bytes.md"}
+    counters = valu_from_profile(args.cutter)
+    if counters:
+        roofline["counters"] = {"valu_busy": counters.get("VALU_BUSY_FRACTION"),
+                                "executed_fp32_tflops": counters.get("EXECUTED_FP32_TFLOPS"),
+                                "source": f"profiles/drop_select_{args.cutter}_c1_valu_counters.md"}
 
     out = {"metric": METRIC, "value": n / (ms / 1e3), "unit": "CL pts/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",

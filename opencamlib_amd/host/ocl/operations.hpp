@@ -221,4 +221,32 @@ class AdaptivePathDropCutter : public PathDropCutter {
     int levels = 0;
 };
 
+// Waterline fibers (waterline.cpp:67-87, 143-178): X fibers at every y and
+// Y fibers at every x of a grid spanning the surface's bounding box grown by
+// twice the cutter radius, at height z, `sampling` apart, pushed on the
+// device (both directions on one mesh handle).  The weave that turns the
+// fibers into loops is graph work the reference does with Boost Graph and is
+// not built here: getLoops() throws.
+class Waterline : public Operation {
+  public:
+    void setZ(double z) { zh = z; }
+    double getZ() const { return zh; }
+    void setSampling(double s) { sampling = s; }
+    double getSampling() const { return sampling; }
+    void setExactFilter(bool on) { exact_filter = on; }
+    void run() override;
+    void reset() {
+        xfibers.clear();
+        yfibers.clear();
+    }
+    const std::vector<Fiber>& getXFibers() const { return xfibers; }
+    const std::vector<Fiber>& getYFibers() const { return yfibers; }
+    std::vector<std::vector<Point>> getLoops() const;
+
+  protected:
+    double zh = 0.0, sampling = 1.0;
+    bool exact_filter = false;
+    std::vector<Fiber> xfibers, yfibers;
+};
+
 }  // namespace ocl

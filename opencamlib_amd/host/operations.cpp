@@ -189,9 +189,10 @@ void PointDropCutter::run(CLPoint& cl) {
 
 // ------------------------------------------------------------ BatchPushCutter
 
-void BatchPushCutter::run() {
-    require_ready();
-    if (dir < 0) throw Error("BatchPushCutter: call setXDirection() or setYDirection() first");
+// Pushes fibers (all along one axis: dir 0 = X, 1 = Y) on the device and
+// stores each fiber's intervals, merged into any it already held.
+static void push_fibers(ocl_mesh* m, const MillingCutter& c, int dir, std::vector<Fiber>& fibers, int precision,
+                        bool exact_filter, ocl_stats* st) {
     const size_t nf = fibers.size();
     std::vector<double> fib(6 * nf);
     for (size_t i = 0; i < nf; ++i) {
@@ -199,14 +200,13 @@ void BatchPushCutter::run() {
         const double v[6] = {f.p1.x, f.p1.y, f.p1.z, f.p2.x, f.p2.y, f.p2.z};
         std::memcpy(&fib[6 * i], v, sizeof(v));
     }
-    const ocl_cutter_desc d = cutter->desc();
+    const ocl_cutter_desc d = c.desc();
     std::vector<uint64_t> offs(nf + 1, 0);
     uint64_t total = 0;
     const int prec = precision | (exact_filter ? OCL_PUSH_EXACT_FILTER : 0);
-    check(ocl_hip_push_batch(dev.handle(), &d, dir, fib.data(), nf, prec, offs.data(), &total, &stats, nullptr));
-    nCalls = stats.pairs;
+    check(ocl_hip_push_batch(m, &d, dir, fib.data(), nf, prec, offs.data(), &total, st, nullptr));
     std::vector<double> lohi(2 * total), cc(8 * total);
-    if (total) check(ocl_hip_push_fetch(dev.handle(), lohi.data(), cc.data(), nullptr));
+    if (total) check(ocl_hip_push_fetch(m, lohi.data(), cc.data(), nullptr));
     for (size_t i = 0; i < nf; ++i) {
         Fiber& f = fibers[i];
         const bool fresh = f.ints.empty();
@@ -214,15 +214,22 @@ void BatchPushCutter::run() {
             Interval iv;
             iv.lower = lohi[2 * k];
             iv.upper = lohi[2 * k + 1];
-            const double* c = &cc[8 * k];
-            iv.lower_cc = CCPoint(c[0], c[1], c[2], (CCType)(int)c[3]);
-            iv.upper_cc = CCPoint(c[4], c[5], c[6], (CCType)(int)c[7]);
+            const double* q = &cc[8 * k];
+            iv.lower_cc = CCPoint(q[0], q[1], q[2], (CCType)(int)q[3]);
+            iv.upper_cc = CCPoint(q[4], q[5], q[6], (CCType)(int)q[7]);
             if (fresh)
                 f.ints.push_back(iv);
             else
                 f.addInterval(iv);
         }
     }
+}
+
+void BatchPushCutter::run() {
+    require_ready();
+    if (dir < 0) throw Error("BatchPushCutter: call setXDirection() or setYDirection() first");
+    push_fibers(dev.handle(), *cutter, dir, fibers, precision, exact_filter, &stats);
+    nCalls = stats.pairs;
 }
 
 std::vector<CLPoint> BatchPushCutter::getCLPoints() const {
@@ -482,6 +489,49 @@ void AdaptivePathDropCutter::slope_run() {
         clpoints.push_back(done[i].b.p);
     }
     nCalls = calls;
+}
+
+// ------------------------------------------------------------ waterline
+
+// n + 1 values from start to end, accumulated (waterline.cpp:169-178)
+static std::vector<double> generate_range(double start, double end, int n) {
+    std::vector<double> out;
+    const double d = (end - start) / (double)n;
+    double v = start;
+    for (int i = 0; i <= n; ++i) {
+        out.push_back(v);
+        v = v + d;
+    }
+    return out;
+}
+
+void Waterline::run() {
+    require_ready();
+    if (!(sampling > 0)) throw Error("Waterline: sampling must be > 0");
+    if (!surf || !surf->bb.initialized) throw Error("Waterline: empty surface");
+    const double r = cutter->getRadius();
+    const double minx = surf->bb.minpt.x - 2 * r, maxx = surf->bb.maxpt.x + 2 * r;
+    const double miny = surf->bb.minpt.y - 2 * r, maxy = surf->bb.maxpt.y + 2 * r;
+    const int nx = (int)((maxx - minx) / sampling), ny = (int)((maxy - miny) / sampling);
+    xfibers.clear();
+    yfibers.clear();
+    for (double y : generate_range(miny, maxy, ny)) xfibers.emplace_back(Point(minx, y, zh), Point(maxx, y, zh));
+    for (double x : generate_range(minx, maxx, nx)) yfibers.emplace_back(Point(x, miny, zh), Point(x, maxy, zh));
+    ocl_stats sx{}, sy{};
+    push_fibers(dev.handle(), *cutter, 0, xfibers, precision, exact_filter, &sx);
+    push_fibers(dev.handle(), *cutter, 1, yfibers, precision, exact_filter, &sy);
+    stats = sx;
+    stats.points += sy.points;
+    stats.pairs += sy.pairs;
+    stats.intervals += sy.intervals;
+    stats.brent_iters += sy.brent_iters;
+    stats.kernel_ms += sy.kernel_ms;
+    nCalls = stats.pairs;
+}
+
+std::vector<std::vector<Point>> Waterline::getLoops() const {
+    throw Error("Waterline::getLoops: the weave (Boost Graph) that builds loops from the fibers is not part of "
+                "this library; use getXFibers()/getYFibers()");
 }
 
 }  // namespace ocl

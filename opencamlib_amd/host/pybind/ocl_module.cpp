@@ -221,6 +221,24 @@ PYBIND11_MODULE(_ocl, m) {
         .def("setExactFilter", &BatchPushCutter::setExactFilter)
         .def("setPrecision", &BatchPushCutter::setPrecision);
 
+    py::class_<Waterline>(m, "Waterline")
+        .def(py::init<>())
+        .def("run", &Waterline::run, py::call_guard<py::gil_scoped_release>())
+        .def("setSTL", &Waterline::setSTL, py::keep_alive<1, 2>())
+        .def("setCutter", &Waterline::setCutter, py::keep_alive<1, 2>())
+        .def("setSampling", &Waterline::setSampling)
+        .def("getSampling", &Waterline::getSampling)
+        .def("setZ", &Waterline::setZ)
+        .def("getZ", &Waterline::getZ)
+        .def("reset", &Waterline::reset)
+        .def("setThreads", &Waterline::setThreads)
+        .def("getXFibers", &Waterline::getXFibers)
+        .def("getYFibers", &Waterline::getYFibers)
+        .def("getLoops", &Waterline::getLoops)
+        .def("getCalls", &Waterline::getCalls)
+        .def("setExactFilter", &Waterline::setExactFilter)
+        .def("setPrecision", &Waterline::setPrecision);
+
     py::class_<PathDropCutter>(m, "PathDropCutter")
         .def(py::init<>())
         .def("run", &PathDropCutter::run, py::call_guard<py::gil_scoped_release>())

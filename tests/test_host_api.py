@@ -438,3 +438,38 @@ def test_slope_driven_refinement_meets_chord_tolerance(gpu, scene):
           f"uniform_points_at_min_step={int(total_len / min_step)}", flush=True)
     assert frac >= 0.995 and err[long_].max() <= 1.5 * tol
     assert npts < total_len / min_step / 5
+
+
+def test_waterline_loops_are_not_built():
+    wl = ocl.Waterline()
+    with pytest.raises(ocl.Error, match="weave"):
+        wl.getLoops()
+
+
+@pytest.mark.gpu
+def test_waterline_fibers_match_push(gpu):
+    """Waterline::init_fibers' grid (bbox +- 2R, `sampling` apart) pushed in
+    both directions equals the C-ABI push of the same fibers."""
+    tris = W.spheres_scene(3, 2, 50.0, seed=4)
+    spec = ALL["cone"]
+    s = surf_from(tris)
+    wl = ocl.Waterline()
+    wl.setSTL(s)
+    wl.setCutter(cutter_from(spec))
+    wl.setZ(3.5)
+    wl.setSampling(0.7)
+    wl.run()
+    xf = wl.getXFibers()
+    yf = wl.getYFibers()
+    r = spec["d"] / 2
+    b = s.getBounds()
+    nx, ny = int((b[1] + 2 * r - (b[0] - 2 * r)) / 0.7), int((b[3] + 2 * r - (b[2] - 2 * r)) / 0.7)
+    assert len(xf) == ny + 1 and len(yf) == nx + 1
+    dev = capi.DeviceMesh(tris)
+    for fibers, d in ((xf, "x"), (yf, "y")):
+        arr = np.array([[f.p1.x, f.p1.y, f.p1.z, f.p2.x, f.p2.y, f.p2.z] for f in fibers])
+        ref = dev.push(spec, arr, d)
+        got = np.array([[i.lower, i.upper] for f in fibers for i in f.getInts()]).reshape(-1, 2)
+        assert np.array_equal(got, ref["lohi"])
+        assert len(got) > 10
+    dev.close()
