@@ -53,23 +53,27 @@ def log(msg):
 
 
 def time_mode(mesh, spec, d_cl, n, mode, outs, steps, warmup, torch):
-    """Runs warmup + timed steps; returns (ms per step, mean select-kernel ms,
-    last stats).  The select kernel is timed with HIP events recorded on the
-    stream it is launched on, for every timed step."""
+    """Runs warmup steps, then `steps` timed steps back to back (no host
+    synchronisation inside), then `steps` more with per-call statistics, whose
+    HIP events, recorded on the stream the select kernel is launched on, give
+    the select kernel's mean time.  Returns (ms per step, mean select-kernel
+    ms, last stats)."""
     z, cc, ty, tri, ft = outs
     stream = torch.cuda.current_stream().cuda_stream
     args = (d_cl.data_ptr(), n, mode, z.data_ptr(), cc.data_ptr(), ty.data_ptr(), tri.data_ptr(), ft.data_ptr())
     for _ in range(warmup):
         mesh.drop_device(spec, *args, stream=stream)
     torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        mesh.drop_device(spec, *args, stream=stream)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
     sel = []
     st = None
-    t0 = time.perf_counter()
     for _ in range(steps):
         st = mesh.drop_device(spec, *args, stream=stream, stats=True)
         sel.append(st["select_ms"])
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
     return 1e3 * dt / steps, float(np.mean(sel)), st
 
 

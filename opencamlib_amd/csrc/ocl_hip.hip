@@ -111,6 +111,7 @@ struct ocl_mesh {
     // cached selection records (rebuilt when the cutter or precision changes)
     ocl_amd::RecSet rs32, rs64;
     uint32_t xcd_group = 1;
+    uint32_t hilbert_bits = 9;   // CL binning: Hilbert curve over 2^bits x 2^bits cells
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_s0 = nullptr, ev_s1 = nullptr;
 
     ocl_amd::MeshView view() const {
@@ -233,24 +234,25 @@ __global__ void k_box_origin(const float4* __restrict__ box, uint32_t nbox, doub
 // counting sort by key.  Order inside a key is arbitrary; per-point results
 // do not depend on it.
 struct BinFrame {
-    double x0, y0, sx, sy;  // cell = (clamp((x - x0) * sx), clamp((y - y0) * sy)) on a 256 x 256 grid
+    double x0, y0, sx, sy;  // cell = (clamp((x - x0) * sx), clamp((y - y0) * sy)) on a 2^hb x 2^hb grid
 };
 
-// Position of cell (x, y) along the Hilbert curve through the 256 x 256
+// Position of cell (x, y) along the Hilbert curve through the 2^hb x 2^hb
 // grid (the classic iterative xy -> d conversion: per level, take the
 // quadrant and rotate/reflect the remaining bits into its frame).  Successive
 // keys are always edge-adjacent cells, so 64 consecutive CL points of the
 // sorted order form one compact patch: no wavefront straddles a jump across
 // the domain, as a Morton order's quadrant boundaries would make it.
-__device__ inline uint32_t hilbert256(uint32_t x, uint32_t y) {
+__device__ inline uint32_t hilbert_key(uint32_t x, uint32_t y, uint32_t hb) {
+    const uint32_t side = 1u << hb;
     uint32_t d = 0;
-    for (uint32_t s = 128; s > 0; s >>= 1) {
+    for (uint32_t s = side >> 1; s > 0; s >>= 1) {
         const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
         d += s * s * ((3u * rx) ^ ry);
         if (ry == 0) {
             if (rx == 1) {
-                x = 255u - x;
-                y = 255u - y;
+                x = side - 1u - x;
+                y = side - 1u - y;
             }
             const uint32_t t = x;
             x = y;
@@ -260,17 +262,19 @@ __device__ inline uint32_t hilbert256(uint32_t x, uint32_t y) {
     return d;
 }
 
-__global__ void k_cl_keys(const double* __restrict__ cl, uint32_t n, BinFrame f, uint32_t* __restrict__ keys) {
+__global__ void k_cl_keys(const double* __restrict__ cl, uint32_t n, BinFrame f, uint32_t hb,
+                          uint32_t* __restrict__ keys) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const double cmax = (double)((1u << hb) - 1u);
     const double fx = (cl[3 * (size_t)i] - f.x0) * f.sx, fy = (cl[3 * (size_t)i + 1] - f.y0) * f.sy;
     // NaN coordinates land in cell 0 (fmax/fmin drop the NaN)
-    const uint32_t ix = (uint32_t)fmin(fmax(fx, 0.0), 255.0), iy = (uint32_t)fmin(fmax(fy, 0.0), 255.0);
-    keys[i] = hilbert256(ix, iy);
+    const uint32_t ix = (uint32_t)fmin(fmax(fx, 0.0), cmax), iy = (uint32_t)fmin(fmax(fy, 0.0), cmax);
+    keys[i] = hilbert_key(ix, iy, hb);
 }
 
 hipError_t cl_sort_by_key(void* temp, size_t* temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
-                          uint32_t* perm, uint32_t n, hipStream_t st);
+                          uint32_t* perm, uint32_t n, uint32_t key_bits, hipStream_t st);
 
 
 // ------------------------------------------------------------ drop outputs
@@ -1582,6 +1586,10 @@ int ocl_hip_mesh_create(const double* tri_xyz, size_t ntri, unsigned bucket, ocl
     if (!m) return fail(OCL_E_NOMEM, "out of host memory");
     HIP_TRY(hipGetDevice(&m->device));
     m->host_tri.assign(tri_xyz, tri_xyz + 9 * ntri);
+    if (const char* h = std::getenv("OCL_HILBERT_BITS")) {
+        const int v = std::atoi(h);
+        if (v >= 4 && v <= 12) m->hilbert_bits = (uint32_t)v;
+    }
     if (const char* g = std::getenv("OCL_XCD_GROUP")) {
         const int v = std::atoi(g);
         if (v >= 1 && v <= 256 && (v & (v - 1)) == 0) m->xcd_group = (uint32_t)v;
@@ -1714,16 +1722,17 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
     f.x0 = m->bbox[0] - R;
     f.y0 = m->bbox[2] - R;
     const double ex = std::max(m->bbox[1] + R - f.x0, 1e-9), ey = std::max(m->bbox[3] + R - f.y0, 1e-9);
-    f.sx = 256.0 / ex;
-    f.sy = 256.0 / ey;
+    const uint32_t hb = m->hilbert_bits;
+    f.sx = (double)(1u << hb) / ex;
+    f.sy = (double)(1u << hb) / ey;
     const uint32_t nb = (n32 + 255) / 256;
     uint32_t* keys = m->keys.as<uint32_t>();
-    k_cl_keys<<<nb, 256, 0, st>>>(d_cl, n32, f, keys);
+    k_cl_keys<<<nb, 256, 0, st>>>(d_cl, n32, f, hb, keys);
     size_t tmp_bytes = 0;
-    HIP_TRY(cl_sort_by_key(nullptr, &tmp_bytes, keys, keys + n32, m->perm.as<uint32_t>(), n32, st));
+    HIP_TRY(cl_sort_by_key(nullptr, &tmp_bytes, keys, keys + n32, m->perm.as<uint32_t>(), n32, 2 * hb, st));
     if ((rc = m->hist.ensure(tmp_bytes + 16))) return rc;
     tmp_bytes = m->hist.bytes;
-    HIP_TRY(cl_sort_by_key(m->hist.p, &tmp_bytes, keys, keys + n32, m->perm.as<uint32_t>(), n32, st));
+    HIP_TRY(cl_sort_by_key(m->hist.p, &tmp_bytes, keys, keys + n32, m->perm.as<uint32_t>(), n32, 2 * hb, st));
     if (stats) {
         if (!m->ev0) HIP_TRY(hipEventCreate(&m->ev0));
         if (!m->ev1) HIP_TRY(hipEventCreate(&m->ev1));
@@ -2142,26 +2151,6 @@ extern "C" int ocl_hip_drop_slopes(ocl_mesh* m, const ocl_cutter_desc* desc, con
 #include "radix.hpp"
 
 namespace ocl_amd {
-
-// Hilbert index of cell (x, y) on a 2^hb x 2^hb grid
-__device__ inline uint32_t hilbert_key(uint32_t x, uint32_t y, uint32_t hb) {
-    const uint32_t side = 1u << hb;
-    uint32_t d = 0;
-    for (uint32_t s = side >> 1; s > 0; s >>= 1) {
-        const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
-        d += s * s * ((3u * rx) ^ ry);
-        if (ry == 0) {
-            if (rx == 1) {
-                x = side - 1u - x;
-                y = side - 1u - y;
-            }
-            const uint32_t t = x;
-            x = y;
-            y = t;
-        }
-    }
-    return d;
-}
 
 struct KeyJob {
     const double* cl;

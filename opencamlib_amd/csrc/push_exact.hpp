@@ -412,6 +412,20 @@ OCL_HD bool bull_general_edge_push(const CutterP& c, const FibX& f, PIval& i, V3
     if (near0(vxynorm(vsub(p2, p1)))) return false;
     if (near0(p2.z - p1.z)) return false;
     const double tplane = (f.p1.z + c.r2 - p1.z) / (p2.z - p1.z);
+    {
+        // A contact is kept only if its CC, the projection of the ellipse
+        // point onto the edge line, lies on the segment (t in [0, 1],
+        // update_ifCCinEdgeAndTrue).  The ellipse lies in the plane
+        // z = f.z + r2, centred at parameter tplane, with semi-axis
+        // r2 / |sin theta| along the edge's XY direction and r2 across it,
+        // so the projection stays within r2 |e_xy| / (|e_z| |e|) of tplane.
+        // Beyond that (plus a rounding margin) no root can update the
+        // interval, and the two Brent solves are skipped.
+        const V3 ed = vsub(p2, p1);
+        const double reach = c.r2 * vxynorm(ed) / (fabs(ed.z) * vnorm(ed));
+        const double margin = 1e-9 * (1.0 + fabs(tplane) + reach);
+        if (tplane + reach < -margin || tplane - reach > 1.0 + margin) return false;
+    }
     AEll e;
     e.center = vadd(p1, vscale(tplane, vsub(p2, p1)));
     V3 major = vsub(p2, p1);
