@@ -188,7 +188,7 @@ typedef struct {
 
 /* Tunables of a batch (zero = default). */
 typedef struct {
-    int32_t hilbert_bits;   /* CL binning: Hilbert curve over 2^bits x 2^bits cells per job domain (4..10, 8) */
+    int32_t hilbert_bits;   /* CL binning: Hilbert curve over 2^bits x 2^bits cells per job domain (4..10, 9) */
     int32_t blocks_per_cu;  /* persistent selection grid: 256-thread blocks per CU (4) */
     int32_t use_graph;      /* capture in a hipGraph at create time and replay it in run */
     int32_t per_job_launch; /* one selection launch per job instead of the work queue */

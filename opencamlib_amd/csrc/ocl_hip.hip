@@ -2313,7 +2313,7 @@ int ocl_hip_batch_create(const ocl_drop_job* jobs, size_t njobs, int precision, 
     if (!b) return fail(OCL_E_NOMEM, "out of host memory");
     b->precision = precision;
     if (tune) b->tune = *tune;
-    b->hb = b->tune.hilbert_bits ? (uint32_t)b->tune.hilbert_bits : 8u;
+    b->hb = b->tune.hilbert_bits ? (uint32_t)b->tune.hilbert_bits : 9u;  // the single-call default
     if (b->hb < 4 || b->hb > 10) return fail(OCL_E_ARG, "hilbert_bits must be in 4..10");
     const uint32_t bpc = b->tune.blocks_per_cu > 0 ? (uint32_t)b->tune.blocks_per_cu : 4u;
     uint32_t jbits = 0;
