@@ -187,7 +187,11 @@ class DeviceMesh:
             self.h = None
 
     def __del__(self):
-        self.close()
+        # at interpreter exit the module's globals may already be gone
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def drop(self, spec: dict, cl_xyz, precision="exact64", want_stats=True):
         """Batch drop of host CL points (n, 3).  Returns z, cc (n, 3), type
@@ -294,7 +298,10 @@ class Batch:
             self.h = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def autotune(jobs, precision="mixed", trials=3):
