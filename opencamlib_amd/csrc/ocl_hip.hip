@@ -482,10 +482,15 @@ struct SelArgs {
 //         evaluated (kLeaf * kFastRec values)
 //   nt_k, nt_h: per-lane near-tie sets: rank indices (and upper values, wave
 //         frame) of the triangles whose contacts reach the band below the
-//         lane's known height
+//         lane's known height (in evaluation order, not rank order)
+//   bl_b, bl_key: the wavefront's list of level-0 boxes and their sort keys
+//         (kBoxList entries)
+constexpr uint32_t kBoxList = 128;  // level-0 boxes a wavefront sorts before evaluating them
+
 template <class T, int K>
 __device__ __attribute__((always_inline)) inline void select_points(const SelArgs<T>& A, uint32_t gid, T* my,
-                                                                   uint32_t (*nt_k)[64], float (*nt_h)[64]) {
+                                                                   uint32_t (*nt_k)[64], float (*nt_h)[64],
+                                                                   uint32_t* bl_b, float* bl_key) {
     typedef FM<T> F;
     const MeshView& m = A.m;
     const T* __restrict__ rec = A.rec;
@@ -567,13 +572,12 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
             pcap = boxcap[bj];
         }
     };
-    walk_boxes(m.tree, qx0, qy0, qx1, qy1, block, [&](uint32_t b, uint32_t jb) {
-        BoxOrigin bo;
-        bo.ox = readlane_d(pox, jb);
-        bo.oy = readlane_d(poy, jb);
-        bo.oz = readlane_d(poz, jb);
-        bo.hx = readlane_f(phx, jb);
-        bo.hy = readlane_f(phy, jb);
+    // Evaluates one level-0 box for every lane.  Any box order gives the
+    // same decisions (each lane keeps the strict-'>' maximum, and its
+    // near-tie set holds every triangle within the band of its final best),
+    // except for cones, whose facet-first rule makes the fold order
+    // dependent: they keep the rank order.
+    auto eval_box = [&](uint32_t b, const BoxOrigin& bo, T cap) {
         if (!framed) {
             framed = true;
             ozw = bo.oz;
@@ -583,7 +587,6 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
         const T xl = (T)(x - bo.ox), yl = (T)(y - bo.oy);
         if (K != K_CONE && P.cull) {
             // no lane can gain anything from this box
-            const T cap = (sizeof(T) == 4) ? (T)readlane_f((float)pcap, jb) : (T)readlane_d((double)pcap, jb);
             const T d2 = box_d2<T>(xl, yl, -(T)bo.hx, (T)bo.hx, -(T)bo.hy, (T)bo.hy);
             const bool rel = act & (cap + dz - prof_lower<T, K>(d2, P) >= F::max_(W.zb, zc) - P.CUT);
             if (!__any(rel)) return;
@@ -690,7 +693,81 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
         if (S.bt != bt_in) W.zb = S.zb + dz;
         for (int i = 0; i < 4; ++i) W.zt[i] = F::max_(W.zt[i], S.zt[i] + dz);
         W.za = F::max_(W.za, S.za + dz);
-    });
+    };
+    // Best first: the walk collects the level-0 boxes that may matter to
+    // some lane, keyed by the most any lane could reach from them, and each
+    // full list (and the last) is evaluated highest first, so every lane's
+    // running best rises early and the profile cull skips most of the rest.
+    // Cones evaluate their lists unsorted, in rank order.
+    uint32_t nlist = 0;
+    const double hcx = 0.5 * (hull_x0 + hull_x1), hcy = 0.5 * (hull_y0 + hull_y1);
+    const double hhx = 0.5 * (hull_x1 - hull_x0), hhy = 0.5 * (hull_y1 - hull_y0);
+    // (FP64 passes keep the walk order: the list's LDS would cost them a
+    // wavefront per SIMD)
+    if constexpr (sizeof(T) != 4) {
+        walk_boxes(m.tree, qx0, qy0, qx1, qy1, [&](uint32_t b) { eval_box(b, org[b], boxcap[b]); });
+    } else {
+    BoxWalk walk;
+    for (;;) {
+        walk_boxes_resumable(m.tree, qx0, qy0, qx1, qy1, walk, [](uint32_t) {}, [&](uint32_t b, uint32_t) -> bool {
+            if (lane_id() == 0) bl_b[nlist] = b;
+            ++nlist;
+            return nlist < kBoxList;
+        });
+        if (nlist) {
+            if (K != K_CONE) {
+                // keys, one box per lane: the most the box's triangles can
+                // give any lane, its cap less the profile height at the
+                // distance between the box and the wave's hull
+                uint32_t n2 = 1;
+                while (n2 < nlist) n2 <<= 1;
+                wave_lds_sync();
+                for (uint32_t i = lane_id(); i < n2; i += 64) {
+                    float key = -INFINITY;
+                    if (i < nlist) {
+                        const uint32_t b = bl_b[i];
+                        const BoxOrigin o = org[b];
+                        const double gx = fmax(fabs(o.ox - hcx) - (double)o.hx - hhx, 0.0);
+                        const double gy = fmax(fabs(o.oy - hcy) - (double)o.hy - hhy, 0.0);
+                        key = (float)((double)(boxcap[b] - prof_lower<T, K>((T)(gx * gx + gy * gy), P)) + o.oz);
+                    } else {
+                        bl_b[i] = 0xffffffffu;
+                    }
+                    bl_key[i] = key;
+                }
+                wave_lds_sync();
+                // sort by key, highest first (bitonic, in LDS)
+                for (uint32_t k = 2; k <= n2; k <<= 1)
+                    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                        for (uint32_t i = lane_id(); i < n2; i += 64) {
+                            const uint32_t l = i ^ j;
+                            if (l > i) {
+                                const float ki = bl_key[i], kl = bl_key[l];
+                                // descending inside blocks whose bit k is clear
+                                const bool desc = (i & k) == 0;
+                                if (desc ? (ki < kl) : (ki > kl)) {
+                                    const uint32_t bi = bl_b[i];
+                                    bl_key[i] = kl;
+                                    bl_key[l] = ki;
+                                    bl_b[i] = bl_b[l];
+                                    bl_b[l] = bi;
+                                }
+                            }
+                        }
+                        wave_lds_sync();
+                    }
+            } else {
+                wave_lds_sync();
+            }
+            for (uint32_t i = 0; i < nlist; ++i) {
+                const uint32_t b = uniform_u(bl_b[i]);
+                eval_box(b, org[b], boxcap[b]);
+            }
+            nlist = 0;
+        }
+        if (walk.done) break;
+    }
+    }
     // decision in the wave frame
     Sel<T> D = S;
     D.zb = W.zb;
@@ -753,9 +830,11 @@ k_drop_select(SelArgs<T> A) {
     __shared__ __align__(16) T srec[4][kLeaf * kFastRec];
     __shared__ uint32_t nt_k[4][kNearTie][64];
     __shared__ float nt_h[4][kNearTie][64];
+    __shared__ uint32_t bl_b[4][sizeof(T) == 4 ? kBoxList : 1];
+    __shared__ float bl_key[4][sizeof(T) == 4 ? kBoxList : 1];
     const uint32_t wv = threadIdx.x >> 6;
     const uint32_t gid = xcd_block(A.m.xcd_group) * blockDim.x + threadIdx.x;
-    select_points<T, K>(A, gid, srec[wv], nt_k[wv], nt_h[wv]);
+    select_points<T, K>(A, gid, srec[wv], nt_k[wv], nt_h[wv], bl_b[wv], bl_key[wv]);
 }
 
 // The selection passes of a batch of jobs in one persistent launch: the
@@ -770,6 +849,8 @@ k_drop_select_jobs(const SelArgs<T>* __restrict__ jobs, const uint32_t* __restri
     __shared__ __align__(16) T srec[4][kLeaf * kFastRec];
     __shared__ uint32_t nt_k[4][kNearTie][64];
     __shared__ float nt_h[4][kNearTie][64];
+    __shared__ uint32_t bl_b[4][sizeof(T) == 4 ? kBoxList : 1];
+    __shared__ float bl_key[4][sizeof(T) == 4 ? kBoxList : 1];
     const uint32_t wv = threadIdx.x >> 6;
     const uint32_t nchunks = chunk_off[njobs];
     for (;;) {
@@ -786,10 +867,10 @@ k_drop_select_jobs(const SelArgs<T>* __restrict__ jobs, const uint32_t* __restri
         const SelArgs<T>& A = jobs[lo];
         const uint32_t gid = (c - chunk_off[lo]) * 64u + lane_id();
         switch (A.P.kind) {
-        case K_CYL: select_points<T, K_CYL>(A, gid, srec[wv], nt_k[wv], nt_h[wv]); break;
-        case K_BALL: select_points<T, K_BALL>(A, gid, srec[wv], nt_k[wv], nt_h[wv]); break;
-        case K_BULL: select_points<T, K_BULL>(A, gid, srec[wv], nt_k[wv], nt_h[wv]); break;
-        default: select_points<T, K_CONE>(A, gid, srec[wv], nt_k[wv], nt_h[wv]); break;
+        case K_CYL: select_points<T, K_CYL>(A, gid, srec[wv], nt_k[wv], nt_h[wv], bl_b[wv], bl_key[wv]); break;
+        case K_BALL: select_points<T, K_BALL>(A, gid, srec[wv], nt_k[wv], nt_h[wv], bl_b[wv], bl_key[wv]); break;
+        case K_BULL: select_points<T, K_BULL>(A, gid, srec[wv], nt_k[wv], nt_h[wv], bl_b[wv], bl_key[wv]); break;
+        default: select_points<T, K_CONE>(A, gid, srec[wv], nt_k[wv], nt_h[wv], bl_b[wv], bl_key[wv]); break;
         }
     }
 }
@@ -843,8 +924,16 @@ __global__ void __launch_bounds__(256) k_drop_resolve(MeshView m, CutterDev cd, 
             }
         } else {
             DropLane E = fresh_lane(x, y, nts.zband[i]);
-            for (uint32_t j = 0; j < stt; ++j) {
-                const uint32_t k = nts.set[(size_t)i * kNearTie + j];
+            // the set holds distinct rank indices in evaluation order: fold
+            // them in rank order, each pass taking the next larger one
+            uint32_t prev = 0;
+            for (uint32_t p = 0; p < stt; ++p) {
+                uint32_t k = 0xffffffffu;
+                for (uint32_t j = 0; j < stt; ++j) {
+                    const uint32_t kk = nts.set[(size_t)i * kNearTie + j];
+                    if ((p == 0 || kk > prev) && kk < k) k = kk;
+                }
+                prev = k;
                 const TriX t = load_tri(m.tri, k);
                 E.cur_tri = k;
                 if (cutter_overlaps(cd.c.radius, x, y, t)) brent += drop_triangle_exact(cd, E, t);

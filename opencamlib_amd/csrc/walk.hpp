@@ -108,6 +108,73 @@ __device__ inline void walk_boxes(const BoxTree& T, float qx0, float qy0, float 
     }
 }
 
+// The same walk, resumable: visit(b, j) returns false to pause it; calling
+// again with the same state continues with the next box.  w.done is set when
+// the walk has ended.
+struct BoxWalk {
+    uint64_t mlo = 0, mhi = 0;
+    int l = 0;
+    uint32_t base = 0;
+    bool started = false, done = false;
+};
+
+template <class Block, class Visit>
+__device__ inline void walk_boxes_resumable(const BoxTree& T, float qx0, float qy0, float qx1, float qy1, BoxWalk& w,
+                                            Block&& block, Visit&& visit) {
+    const uint32_t lane = lane_id();
+    auto get = [&](int l) -> uint32_t {
+        return (uint32_t)(((l < 4) ? (w.mlo >> (16 * l)) : (w.mhi >> (16 * (l - 4)))) & 0xffffull);
+    };
+    auto put = [&](int l, uint32_t v) {
+        if (l < 4) w.mlo = (w.mlo & ~(0xffffull << (16 * l))) | ((uint64_t)v << (16 * l));
+        else w.mhi = (w.mhi & ~(0xffffull << (16 * (l - 4)))) | ((uint64_t)v << (16 * (l - 4)));
+    };
+    auto load_mask = [&](int l, uint32_t base) -> uint32_t {
+        const uint32_t cnt = T.lvl[2 * l], off = T.lvl[2 * l + 1];
+        bool hit = false;
+        if (lane < (uint32_t)kFan && base + lane < cnt) hit = box_hits(T.box[off + base + lane], qx0, qy0, qx1, qy1);
+        return (uint32_t)__ballot(hit) & 0xffffu;
+    };
+    const int top = T.top;
+    const uint32_t top_cnt = T.lvl[2 * top];
+    if (!w.started) {
+        w.started = true;
+        w.l = top;
+        w.base = 0;
+        put(w.l, load_mask(w.l, 0));
+        if (w.l == 0) block(0u);
+    }
+    for (;;) {
+        const uint32_t M = get(w.l);
+        if (M == 0) {
+            if (w.l == top) {
+                w.base += kFan;
+                if (w.base >= top_cnt) {
+                    w.done = true;
+                    return;
+                }
+                put(w.l, load_mask(w.l, w.base));
+                if (w.l == 0) block(w.base);
+                continue;
+            }
+            w.base = (w.base / kFan) & ~(uint32_t)(kFan - 1);
+            ++w.l;
+            continue;
+        }
+        const uint32_t j = (uint32_t)__builtin_ctz(M);
+        put(w.l, M & (M - 1));
+        const uint32_t node = w.base + j;
+        if (w.l == 0) {
+            if (!visit(node, j)) return;
+            continue;
+        }
+        --w.l;
+        w.base = node * kFan;
+        put(w.l, load_mask(w.l, w.base));
+        if (w.l == 0) block(w.base);
+    }
+}
+
 template <class Visit>
 __device__ inline void walk_boxes(const BoxTree& T, float qx0, float qy0, float qx1, float qy1, Visit&& visit) {
     walk_boxes(T, qx0, qy0, qx1, qy1, [](uint32_t) {}, [&](uint32_t b, uint32_t) { visit(b); });
