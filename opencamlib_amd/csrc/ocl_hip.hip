@@ -760,6 +760,15 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
                 wave_lds_sync();
             }
             for (uint32_t i = 0; i < nlist; ++i) {
+                if (K != K_CONE && P.cull && framed) {
+                    // keys only fall from here on: once the box's key (the
+                    // most any lane can reach from it) is below every lane's
+                    // threshold, no later box can matter either
+                    const double thr = (double)wave_min_t<T>(act ? F::max_(W.zb, zc) : F::inf()) + ozw -
+                                       (double)P.CUT;
+                    const float kb = bl_key[i];
+                    if ((double)kb + 1e-5 * (1.0 + fabs((double)kb)) < thr) break;
+                }
                 const uint32_t b = uniform_u(bl_b[i]);
                 eval_box(b, org[b], boxcap[b]);
             }
