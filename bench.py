@@ -12,7 +12,8 @@ and FP64 refinement of the winning contact).
 Beside it, in the same run: the FP64-only and FP32-only device paths, the
 reference-order exact FP64 device path, the accuracy of each against the FP64
 oracle on a seeded sample, and the oracle itself timed on the host cores on a
-bounded sample (the ``cpu_baseline``).  Unless ``--no-extra``: the backward
+bounded sample (the ``cpu_baseline``).  Unless ``--no-extra``: the same
+mesh and grid with the BullCutter (the configuration's second run), the backward
 pass on 1M random points of the same mesh, the push-cutter on the 80,000
 waterline fibers of the sphere scene (Bull and Cone), and 64 uneven batched
 jobs (persistent graph launch vs per-job launches).  Prints one JSON line.
@@ -429,6 +430,23 @@ def main():
             acc[mode] = accuracy(r, cb["r"], cb["idx"])
         acc["mixed_vs_kd_sample"] = accuracy(res_mixed, cb["r_kd"], cb["idx_kd"])
         out["accuracy_vs_fp64_oracle"] = acc
+    if not args.no_extra and args.cutter == "ball":
+        # the config's second run: BullCutter d=10 r=2 on the same mesh and grid
+        bspec = SPECS["bull"]
+        b_ms, b_sel, b_st = time_mode(mesh, bspec, d_cl, n, "mixed", outs, max(3, args.steps // 2), 1, torch)
+        b_mixed = fetch()
+        f_ms, f_sel, _ = time_mode(mesh, bspec, d_cl, n, "fp64", outs, 3, 1, torch)
+        bull = {"mixed": {"value": n / (b_ms / 1e3), "ms_per_step": b_ms, "select_ms": b_sel,
+                          "escalated": b_st["escalated"], "brent_iters": b_st["brent_iters"]},
+                "fp64_gpu": {"value": n / (f_ms / 1e3), "ms_per_step": f_ms, "select_ms": f_sel},
+                "mixed_over_fp64": f_ms / b_ms}
+        if not args.no_cpu:
+            from oracle import oracle as O
+            idx = np.sort(np.random.default_rng(100).choice(n, 20_000, replace=False))
+            rb = O.Mesh(tris).batch_drop(O.cutter_from_spec(bspec), cl[idx], search_mode=1)
+            bull["accuracy_vs_fp64_oracle"] = accuracy(b_mixed, rb, idx)
+        out["bull"] = bull
+        log(f"bull: mixed {b_ms:.2f} ms, fp64 {f_ms:.2f} ms")
     if not args.no_extra:
         out["backward"] = bench_backward(torch, capi, W, mesh, len(tris), 1_000_000, max(3, args.steps // 2))
         log(f"backward: forward {out['backward']['forward_ms']:.2f} ms, backward {out['backward']['backward_ms']:.3f} ms")
