@@ -102,7 +102,7 @@ struct ocl_mesh {
     int top = 0;
     // per-call workspace
     ocl_amd::DevBuf keys, perm, hist, cursor, cl, out_z, out_cc, out_ty, out_tri, out_ft, out_dz, counters, fastrec;
-    ocl_amd::DevBuf esc_list, esc_list2, esc_flag, blocksum, qcount, cnt2, cnt3;
+    ocl_amd::DevBuf esc_list, esc_list2, esc_list3, esc_flag, blocksum, qcount, cnt2, cnt3;
     ocl_amd::DevBuf origins;     // frame (BoxOrigin) per level-0 box
     ocl_amd::DevBuf zfloor;      // per CL point: floor of the final z handed to the escalation passes
     ocl_amd::DevBuf ntset, ntcnt, zband, selrec;  // per CL point: selection records and near-tie sets
@@ -262,9 +262,21 @@ __device__ inline uint32_t hilbert_key(uint32_t x, uint32_t y, uint32_t hb) {
     return d;
 }
 
+// Up to four small device ranges (counters, queue counts) that a call clears
+// at its start, from the first block of its first kernel: one fill kernel
+// per range would cost a launch each.
+struct ZeroSet {
+    uint32_t* p[4];
+    uint32_t words[4];
+};
+
 __global__ void k_cl_keys(const double* __restrict__ cl, uint32_t n, BinFrame f, uint32_t hb,
-                          uint32_t* __restrict__ keys) {
+                          uint32_t* __restrict__ keys, ZeroSet zs) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0) {
+        for (int r = 0; r < 4; ++r)
+            if (zs.p[r] && threadIdx.x < zs.words[r]) zs.p[r][threadIdx.x] = 0u;
+    }
     if (i >= n) return;
     const double cmax = (double)((1u << hb) - 1u);
     const double fx = (cl[3 * (size_t)i] - f.x0) * f.sx, fy = (cl[3 * (size_t)i + 1] - f.y0) * f.sy;
@@ -304,6 +316,18 @@ __device__ inline TriX load_tri(const double* __restrict__ base, uint32_t k) {
 __device__ inline void wave_add(unsigned long long* dst, unsigned long long v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
+}
+
+// Appends v of every lane with `push` to queue q (count *qn), one atomic per
+// wavefront; lanes keep their order within the wavefront.
+__device__ inline void wave_push(bool push, uint32_t v, uint32_t* q, uint32_t* qn) {
+    const unsigned long long bal = __ballot(push);
+    if (!bal) return;
+    const uint32_t lead = (uint32_t)(__ffsll((long long)bal) - 1);
+    uint32_t base = 0;
+    if ((threadIdx.x & 63) == lead) base = atomicAdd(qn, (uint32_t)__popcll(bal));
+    base = __shfl(base, (int)lead, 64);
+    if (push) q[base + __popcll(bal & ((1ull << (threadIdx.x & 63)) - 1ull))] = v;
 }
 
 __device__ inline void write_drop(const DropOut& o, const uint32_t* order, uint32_t src, const DropLane& L) {
@@ -445,6 +469,8 @@ struct NearTieSets {
     uint8_t* bf;
     double* zsel;
     float* esel;
+    uint32_t* q;     // queue of the points with a near-tie set (count *qn),
+    uint32_t* qn;    // filled by the selection pass
 };
 constexpr uint8_t kSelDecided = 0xfe;  // winner decided, refine it
 constexpr uint8_t kSelNone = 0xff;     // undecided, no usable near-tie set
@@ -807,6 +833,7 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
             if (K != K_CONE && up && !D.bamb && D.bt != 0xffffffffu) fl = F::max_(fl, D.zb - (T)2 * D.eb - P.E);
             zfloor[src] = (K == K_CONE || !framed) ? -INFINITY : ozw + (double)fl - 1e-9 * (1.0 + fabs(ozw));
         }
+        wave_push(act && up && K != K_CONE && !ovf && framed && ntn > 0, src, nts.q, nts.qn);
     } else if (act) {
         // FP32-only mode: the selection result as is (no CC point)
         DropLane L = fresh_lane(x, y, z0);
@@ -931,7 +958,37 @@ __global__ void __launch_bounds__(256) k_drop_resolve(MeshView m, CutterDev cd, 
                     done = true;
                 }
             }
-        } else {
+        }
+    }
+    // near-tie points were queued by the selection pass for k_drop_neartie,
+    // which folds them with full wavefronts
+    const bool tie = act && stt != kSelNone && stt != kSelDecided;
+    wave_push(act && !done && !tie, i, next, next_n);
+    if (cnt) wave_add(&cnt->brent, brent);
+}
+
+// Near-tie pass, one lane per queued point: the reference's fold
+// (MillingCutter::dropCutter over triangles in rank order) restricted to the
+// point's near-tie set and started at the bottom of its band.  Every other
+// triangle's contacts stay below that start, and for Cyl/Ball/Bull the
+// reference's result is the maximum of the triangles' own drops (ties to the
+// lowest rank), so when a set triangle lifts above the start the fold's
+// result is the reference's.  Points where none lifts go to `next`.
+__global__ void __launch_bounds__(256) k_drop_neartie(MeshView m, CutterDev cd, const double* __restrict__ cl,
+                                                      const uint32_t* __restrict__ ntq,
+                                                      const uint32_t* __restrict__ ntq_n, NearTieSets nts, DropOut o,
+                                                      uint32_t* __restrict__ next, uint32_t* __restrict__ next_n,
+                                                      Counters* cnt) {
+    const uint32_t n_eff = *ntq_n;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    unsigned long long brent = 0;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; (g & ~63u) < n_eff; g += stride) {
+        const bool act = g < n_eff;
+        const uint32_t i = act ? ntq[g] : 0u;
+        bool done = false;
+        if (act) {
+            const uint32_t stt = nts.cnt[i];
+            const double x = cl[3 * (size_t)i], y = cl[3 * (size_t)i + 1];
             DropLane E = fresh_lane(x, y, nts.zband[i]);
             // the set holds distinct rank indices in evaluation order: fold
             // them in rank order, each pass taking the next larger one
@@ -952,14 +1009,7 @@ __global__ void __launch_bounds__(256) k_drop_resolve(MeshView m, CutterDev cd, 
                 done = true;
             }
         }
-    }
-    const bool push = act && !done;
-    const unsigned long long bal = __ballot(push);
-    if (bal) {
-        uint32_t base = 0;
-        if ((threadIdx.x & 63) == (uint32_t)(__ffsll((long long)bal) - 1)) base = atomicAdd(next_n, (uint32_t)__popcll(bal));
-        base = __shfl(base, __ffsll((long long)bal) - 1, 64);
-        if (push) next[base + __popcll(bal & ((1ull << (threadIdx.x & 63)) - 1ull))] = i;
+        wave_push(act && !done, i, next, next_n);
     }
     if (cnt) wave_add(&cnt->brent, brent);
 }
@@ -983,9 +1033,15 @@ struct BoxCull {
     const BoxOrigin* org;
 };
 
+// The walk's level-0 boxes are tested four at a time, one triangle per lane
+// (lane = 16 x slot + triangle), so each round of dependent loads covers
+// 64 triangles; candidates are appended in rank order.  qbuf holds
+// kCoopQueue entries.
+constexpr uint32_t kCoopQueue = 128;
 template <class Thr, class Keep, class Batch>
 __device__ inline void coop_gather(const MeshView& m, uint32_t* qbuf, double x, double y, const CutterP& c,
                                    const BoxCull& bc, Thr&& thr, Keep&& keep, Batch&& batch) {
+    static_assert(4 * kLeaf == 64, "four level-0 boxes per wavefront");
     const uint32_t lane = lane_id();
     const double R = c.radius;
     const float qx0 = __double2float_rd(x - R), qy0 = __double2float_rd(y - R);
@@ -1006,44 +1062,60 @@ __device__ inline void coop_gather(const MeshView& m, uint32_t* qbuf, double x, 
             phy = o.hy;
         }
     };
-    walk_boxes(m.tree, qx0, qy0, qx1, qy1, block, [&](uint32_t b, uint32_t jb) {
-        if (bc.cap) {
-            // box-level cull: every contact in the box is below cap + oz - h(d)
-            const double ox = readlane_d(pox, jb), oy = readlane_d(poy, jb), oz = readlane_d(poz, jb);
-            const double hx = readlane_f(phx, jb), hy = readlane_f(phy, jb);
-            const double dx = fmax(fabs(x - ox) - hx, 0.0), dy = fmax(fabs(y - oy) - hy, 0.0);
-            const double d2 = dx * dx + dy * dy;
-            double h = 0.0;
-            if (c.kind == K_BALL) {
-                h = R - sqrt(fmax(R * R - d2, 0.0));
-            } else if (c.kind == K_BULL) {
-                const double w = fmax(sqrt(d2) - c.r1, 0.0);
-                h = c.r2 - sqrt(fmax(c.r2 * c.r2 - w * w, 0.0));
+    BoxWalk w;
+    for (;;) {
+        // up to four level-0 boxes that survive the box-level cull
+        uint32_t pb0 = 0, pb1 = 0, pb2 = 0, pb3 = 0, npend = 0;  // wave-uniform
+        if (!w.done)
+            walk_boxes_resumable(m.tree, qx0, qy0, qx1, qy1, w, block, [&](uint32_t b, uint32_t jb) -> bool {
+                if (bc.cap) {
+                    // every contact in the box is below cap + oz - h(d)
+                    const double ox = readlane_d(pox, jb), oy = readlane_d(poy, jb), oz = readlane_d(poz, jb);
+                    const double hx = readlane_f(phx, jb), hy = readlane_f(phy, jb);
+                    const double dx = fmax(fabs(x - ox) - hx, 0.0), dy = fmax(fabs(y - oy) - hy, 0.0);
+                    const double d2 = dx * dx + dy * dy;
+                    double h = 0.0;
+                    if (c.kind == K_BALL) {
+                        h = R - sqrt(fmax(R * R - d2, 0.0));
+                    } else if (c.kind == K_BULL) {
+                        const double wd = fmax(sqrt(d2) - c.r1, 0.0);
+                        h = c.r2 - sqrt(fmax(c.r2 * c.r2 - wd * wd, 0.0));
+                    }
+                    if (readlane_d(pcap, jb) + oz - fmax(h - 1e-12 * R, 0.0) < thr()) return true;
+                }
+                pb0 = (npend == 0) ? b : pb0;
+                pb1 = (npend == 1) ? b : pb1;
+                pb2 = (npend == 2) ? b : pb2;
+                pb3 = (npend == 3) ? b : pb3;
+                return ++npend < 4;
+            });
+        if (npend) {
+            const uint32_t slot = lane >> 4;
+            const uint32_t b = (slot == 0) ? pb0 : (slot == 1) ? pb1 : (slot == 2) ? pb2 : pb3;
+            const uint32_t k = b * kLeaf + (lane & 15u);
+            bool cand = false;
+            if (slot < npend && k < m.ntri) {
+                const TriX t = load_tri(m.tri, k);
+                cand = cutter_overlaps(R, x, y, t) && keep(k, t);
             }
-            if (readlane_d(pcap, jb) + oz - fmax(h - 1e-12 * R, 0.0) < thr()) return;
+            const unsigned long long bal = __ballot(cand);
+            const uint32_t pos = qn + __popcll(bal & ((1ull << lane) - 1ull));
+            if (cand) qbuf[pos] = k;
+            qn += __popcll(bal);
+            __builtin_amdgcn_wave_barrier();
         }
-        const uint32_t k = b * kLeaf + lane;
-        bool cand = false;
-        if (lane < (uint32_t)kLeaf && k < m.ntri) {
-            const TriX t = load_tri(m.tri, k);
-            cand = cutter_overlaps(R, x, y, t) && keep(k, t);
-        }
-        const unsigned long long bal = __ballot(cand);
-        const uint32_t pos = qn + __popcll(bal & ((1ull << lane) - 1ull));
-        if (cand) qbuf[pos] = k;
-        qn += __popcll(bal);
-        __builtin_amdgcn_wave_barrier();
-        if (qn >= 64) {
-            batch(qbuf, 64u);
-            const uint32_t rest = qn - 64;
+        if (qn >= 64 || (w.done && qn > 0)) {
+            const uint32_t nb = qn < 64 ? qn : 64u;
+            batch(qbuf, nb);
+            const uint32_t rest = qn - nb;
             const uint32_t moved = (lane < rest) ? qbuf[64 + lane] : 0u;
             __builtin_amdgcn_wave_barrier();
             if (lane < rest) qbuf[lane] = moved;
             __builtin_amdgcn_wave_barrier();
             qn = rest;
         }
-    });
-    if (qn > 0) batch(qbuf, qn);
+        if (w.done && qn == 0) break;
+    }
 }
 
 // Upper bound of every contact height of triangle t for a CL point at
@@ -1077,7 +1149,7 @@ __global__ void __launch_bounds__(256) k_drop_coop_exact(MeshView m, CutterDev c
                                                          const double* __restrict__ boxcap,
                                                          const BoxOrigin* __restrict__ org, DropOut o,
                                                          Counters* cnt) {
-    __shared__ uint32_t qmem[kCoopWaves][80];
+    __shared__ uint32_t qmem[kCoopWaves][kCoopQueue];
     const uint32_t lane = lane_id();
     const uint32_t wv = threadIdx.x >> 6;
     uint32_t* qbuf = qmem[wv];
@@ -1124,6 +1196,40 @@ __global__ void __launch_bounds__(256) k_drop_coop_exact(MeshView m, CutterDev c
                     edge_simple(c, V, t, it2);
                     brent += it2;
                 }
+            }
+            // Cyl/Ball/Bull: a triangle's own drop is its facet contact when
+            // that is valid (a valid facet contact is the triangle's highest),
+            // else its best vertex/edge contact, and the rank-ordered fold
+            // keeps the highest of these, ties to the lowest rank: a wave
+            // arg-max.  A batch where some vertex/edge contact rounds above
+            // a valid facet contact (the fold's outcome then depends on the
+            // running height) and cones take the rank-ordered fold below.
+            const bool fv = A.type != CC_NONE, vv = V.type != CC_NONE;
+            if (c.kind != K_CONE && !__any(fv && vv && V.z > A.z)) {
+                double best = fv ? A.z : (vv ? V.z : -INFINITY);
+                if (!(maxz > zr)) best = -INFINITY;  // below()
+                uint32_t bl = lane;
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double ob = __shfl_xor(best, off, 64);
+                    const uint32_t ol = (uint32_t)__shfl_xor((int)bl, off, 64);
+                    const bool take = (ob > best) || (ob == best && ol < bl);
+                    best = take ? ob : best;
+                    bl = take ? ol : bl;
+                }
+                best = __shfl(best, 0, 64);
+                bl = (uint32_t)__shfl((int)bl, 0, 64);
+                if (best > zr) {
+                    const uint32_t wl = uniform_u(bl);
+                    zr = best;
+                    win_k = __shfl(k, wl, 64);
+                    win_part = 0;
+                    const double cx = fv ? A.cc.x : V.cc.x, cy = fv ? A.cc.y : V.cc.y, cz = fv ? A.cc.z : V.cc.z;
+                    const int ty = fv ? A.type : V.type, ft = fv ? A.feat : V.feat;
+                    win_cc = v3(__shfl(cx, wl, 64), __shfl(cy, wl, 64), __shfl(cz, wl, 64));
+                    win_type = __shfl(ty, wl, 64);
+                    win_feat = __shfl(ft, wl, 64);
+                }
+                return;
             }
             // rank-ordered fold (uniform across the wave)
             for (uint32_t i = 0; i < nb; ++i) {
@@ -1181,7 +1287,7 @@ __global__ void __launch_bounds__(256) k_drop_coop_select(MeshView m, const doub
                                                           const double* __restrict__ boxcap, DropOut o,
                                                           uint32_t* __restrict__ next, uint32_t* __restrict__ next_n,
                                                           Counters* cnt) {
-    __shared__ uint32_t qmem[kCoopWaves][80];
+    __shared__ uint32_t qmem[kCoopWaves][kCoopQueue];
     const uint32_t lane = lane_id();
     const uint32_t wv = threadIdx.x >> 6;
     uint32_t* qbuf = qmem[wv];
@@ -1435,6 +1541,7 @@ static SelArgs<T> sel_args(ocl_mesh* m, const MeshView& mv, const T* rec, const 
 }
 
 constexpr uint32_t kCoopBlocks = 2048;  // persistent grid of the cooperative tiers
+constexpr uint32_t kNearTieBlocks = 512;  // persistent grid of the near-tie pass (8 waves per CU)
 
 static double env_scale(const char* name, double def) {
     const char* v = std::getenv(name);
@@ -1463,7 +1570,7 @@ struct FastPlan {
     Counters* cnt;
     NearTieSets nts;
     double* zfloor;
-    uint32_t *q1, *q2, *qcount;
+    uint32_t *q1, *q2, *qnt, *qcount;  // qcount: q1, q2, qnt entries
     SelParams<double> P64;
     SelParams<float> P32;
     double etie;
@@ -1476,6 +1583,7 @@ static int fast_prepare(ocl_mesh* m, const MeshView& mv, const CutterDev& cd, co
                         hipStream_t st, FastPlan& F) {
     int rc;
     if ((rc = m->esc_list.ensure(4ull * n + 64)) || (rc = m->esc_list2.ensure(4ull * n + 64)) ||
+        (rc = m->esc_list3.ensure(4ull * n + 64)) ||
         (rc = m->zfloor.ensure(8ull * n + 64)) || (rc = m->ntset.ensure(4ull * kNearTie * n + 64)) ||
         (rc = m->ntcnt.ensure(2ull * n + 64)) || (rc = m->zband.ensure(8ull * n + 64)) ||
         (rc = m->selrec.ensure(20ull * n + 64)))
@@ -1499,7 +1607,10 @@ static int fast_prepare(ocl_mesh* m, const MeshView& mv, const CutterDev& cd, co
     F.nts.esel = reinterpret_cast<float*>(F.nts.bt + n);
     F.q1 = m->esc_list.as<uint32_t>();
     F.q2 = m->esc_list2.as<uint32_t>();
+    F.qnt = m->esc_list3.as<uint32_t>();
     F.qcount = m->qcount.as<uint32_t>();
+    F.nts.q = F.qnt;
+    F.nts.qn = F.qcount + 2;
     const double se = env_scale("OCL_SEL_E_SCALE", 1.0);
     const SelErr er64 = make_err<double>(cd, se);
     F.P64 = make_params<double>(cd, er64);
@@ -1540,6 +1651,9 @@ static int fast_post(const FastPlan& F, hipStream_t st) {
     k_drop_resolve<<<(n + 255) / 256, 256, 0, st>>>(mv, cd, F.cl, F.perm, n, F.nts, F.etie, F.o, F.q1, F.qcount,
                                                      m->cnt2.as<Counters>());
     HIP_TRY(hipGetLastError());
+    k_drop_neartie<<<kNearTieBlocks, 256, 0, st>>>(
+        mv, cd, F.cl, F.qnt, F.qcount + 2, F.nts, F.o, F.q1, F.qcount, m->cnt2.as<Counters>());
+    HIP_TRY(hipGetLastError());
     const uint32_t* exact_list = F.q1;
     const uint32_t* exact_n = F.qcount;
     const BoxOrigin* org = m->origins.as<BoxOrigin>();
@@ -1573,7 +1687,6 @@ static int launch_drop_fast(ocl_mesh* m, const MeshView& mv, const CutterDev& cd
     FastPlan F;
     int rc = fast_prepare(m, mv, cd, d_cl, m->perm.as<uint32_t>(), n, precision, o, cnt, st, F);
     if (rc) return rc;
-    HIP_TRY(hipMemsetAsync(F.qcount, 0, 16, st));
     if (timed) HIP_TRY(hipEventRecord(m->ev_s0, st));
     const uint32_t nb = xcd_grid((n + 255) / 256, mv.xcd_group);
     const bool f32 = precision == OCL_PREC_FP32 || precision == OCL_PREC_MIXED;
@@ -1810,9 +1923,11 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
     if ((rc = m->cnt2.ensure(sizeof(Counters))) || (rc = m->cnt3.ensure(sizeof(Counters))) ||
         (rc = m->qcount.ensure(16)))
         return rc;
-    HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(Counters), st));
-    HIP_TRY(hipMemsetAsync(m->cnt2.p, 0, sizeof(Counters), st));
-    HIP_TRY(hipMemsetAsync(m->cnt3.p, 0, sizeof(Counters), st));
+    // the counters and queue counts are cleared by k_cl_keys
+    const ZeroSet zs{{reinterpret_cast<uint32_t*>(cnt), m->cnt2.as<uint32_t>(), m->cnt3.as<uint32_t>(),
+                      m->qcount.as<uint32_t>()},
+                     {(uint32_t)(sizeof(Counters) / 4), (uint32_t)(sizeof(Counters) / 4),
+                      (uint32_t)(sizeof(Counters) / 4), 4u}};
     // bin CL points
     if ((rc = m->keys.ensure(8ull * n)) || (rc = m->perm.ensure(4ull * n))) return rc;
     const double R = cd.c.radius;
@@ -1825,7 +1940,7 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
     f.sy = (double)(1u << hb) / ey;
     const uint32_t nb = (n32 + 255) / 256;
     uint32_t* keys = m->keys.as<uint32_t>();
-    k_cl_keys<<<nb, 256, 0, st>>>(d_cl, n32, f, hb, keys);
+    k_cl_keys<<<nb, 256, 0, st>>>(d_cl, n32, f, hb, keys, zs);
     size_t tmp_bytes = 0;
     HIP_TRY(cl_sort_by_key(nullptr, &tmp_bytes, keys, keys + n32, m->perm.as<uint32_t>(), n32, 2 * hb, st));
     if ((rc = m->hist.ensure(tmp_bytes + 16))) return rc;
