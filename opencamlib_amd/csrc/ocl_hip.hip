@@ -511,12 +511,25 @@ struct SelArgs {
 //         lane's known height (in evaluation order, not rank order)
 //   bl_b, bl_key: the wavefront's list of level-0 boxes and their sort keys
 //         (kBoxList entries)
+//   ptab: the wavefront's table of the (owner lane, triangle) pairs of the
+//         batch being evaluated (64 entries)
 constexpr uint32_t kBoxList = 128;  // level-0 boxes a wavefront sorts before evaluating them
+
+// Pair packing: the (point, triangle) pairs of a level-0 box are evaluated
+// 64 to a wavefront step, each lane for the pair's owner, and folded back by
+// the owners.  It pays where a pair's contact code costs far more than the
+// fold (the FP64 BullCutter edge solve); for the other selections the
+// rank-order scan, one triangle per step for the lanes that need it, is
+// faster.  Cones always scan: their fold depends on the order.
+template <class T, int K>
+constexpr bool pack_pairs() {
+    return K == K_BULL && sizeof(T) == 8;
+}
 
 template <class T, int K>
 __device__ __attribute__((always_inline)) inline void select_points(const SelArgs<T>& A, uint32_t gid, T* my,
                                                                    uint32_t (*nt_k)[64], float (*nt_h)[64],
-                                                                   uint32_t* bl_b, float* bl_key) {
+                                                                   uint32_t* bl_b, float* bl_key, uint32_t* ptab) {
     typedef FM<T> F;
     const MeshView& m = A.m;
     const T* __restrict__ rec = A.rec;
@@ -669,49 +682,141 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
             }
             tmask = (uint32_t)__ballot(keep);
         }
-        for (; tmask; tmask &= tmask - 1) {
-            const uint32_t j = (uint32_t)__builtin_ctz(tmask);
-            const uint32_t ku = uniform_u(t0 + j);
-            const T* r = my + kFastRec * j;
-            const T bx0 = r[R_BX0], bx1 = r[R_BX1], by0 = r[R_BY0], by1 = r[R_BY1];
-            bool on = act & !((bx1 < xl - P.R) | (bx0 > xl + P.R) | (by1 < yl - P.R) | (by0 > yl + P.R));
-            if (K != K_CONE && P.cull) {
-                const T d2 = box_d2<T>(xl, yl, bx0, bx1, by0, by1);
-                on = on & (r[R_ZCAP] - prof_lower<T, K>(d2, P) >= F::max_(S.zb, zc - dz) - P.CUT);
-            }
-            if (!__any(on)) continue;
-            ++evals;
-            T th;
-            sel_triangle<T, K>(r, ku, xl, yl, P, S, on, &th);
-            if (K != K_CONE && refine) {
-                // near-tie set: triangles reaching the band below the best
-                // height the lane is known to reach (running best or floor)
-                const bool keep = on & (th >= F::max_(S.zb, zc - dz) - P.CUT);
-                if (__any(keep) && keep) {
-                    if (ntn == (uint32_t)kNearTie) {
-                        // full: drop the entries that fell below the band since
-                        const float lim = (float)(F::max_(S.zb + dz, zc) - P.CUT);
-                        uint32_t w = 0;
-                        for (uint32_t i = 0; i < (uint32_t)kNearTie; ++i) {
-                            const float h = nt_h[i][ln];
-                            const uint32_t kk = nt_k[i][ln];
-                            if (h >= lim) {
-                                nt_h[w][ln] = h;
-                                nt_k[w][ln] = kk;
-                                ++w;
-                            }
+        // near-tie set: triangles reaching the band below the best height
+        // the lane is known to reach (running best or floor)
+        auto nt_add = [&](bool keep, T th, uint32_t ku) {
+            if (__any(keep) && keep) {
+                if (ntn == (uint32_t)kNearTie) {
+                    // full: drop the entries that fell below the band since
+                    const float lim = (float)(F::max_(S.zb + dz, zc) - P.CUT);
+                    uint32_t w = 0;
+                    for (uint32_t i = 0; i < (uint32_t)kNearTie; ++i) {
+                        const float h = nt_h[i][ln];
+                        const uint32_t kk = nt_k[i][ln];
+                        if (h >= lim) {
+                            nt_h[w][ln] = h;
+                            nt_k[w][ln] = kk;
+                            ++w;
                         }
-                        ntn = w;
                     }
-                    if (ntn < (uint32_t)kNearTie) {
-                        // upper value in the wave frame, rounded up
-                        const float hw = (float)(th + dz);
-                        nt_h[ntn][ln] = hw + 1e-6f * (1.0f + fabsf(hw));
-                        nt_k[ntn][ln] = ku;
-                        ++ntn;
-                    } else {
-                        ovf = true;
+                    ntn = w;
+                }
+                if (ntn < (uint32_t)kNearTie) {
+                    // upper value in the wave frame, rounded up
+                    const float hw = (float)(th + dz);
+                    nt_h[ntn][ln] = hw + 1e-6f * (1.0f + fabsf(hw));
+                    nt_k[ntn][ln] = ku;
+                    ++ntn;
+                } else {
+                    ovf = true;
+                }
+            }
+        };
+        if constexpr (!pack_pairs<T, K>()) {
+            // every lane folds its candidates among the box's triangles in
+            // rank order, one triangle per wavefront step
+            for (; tmask; tmask &= tmask - 1) {
+                const uint32_t j = (uint32_t)__builtin_ctz(tmask);
+                const uint32_t ku = uniform_u(t0 + j);
+                const T* r = my + kFastRec * j;
+                const T bx0 = r[R_BX0], bx1 = r[R_BX1], by0 = r[R_BY0], by1 = r[R_BY1];
+                bool on = act & !((bx1 < xl - P.R) | (bx0 > xl + P.R) | (by1 < yl - P.R) | (by0 > yl + P.R));
+                if (K != K_CONE && P.cull) {
+                    const T d2 = box_d2<T>(xl, yl, bx0, bx1, by0, by1);
+                    on = on & (r[R_ZCAP] - prof_lower<T, K>(d2, P) >= F::max_(S.zb, zc - dz) - P.CUT);
+                }
+                if (!__any(on)) continue;
+                ++evals;
+                T th;
+                sel_triangle<T, K>(r, ku, xl, yl, P, S, on, &th);
+                if (K != K_CONE && refine) nt_add(on & (th >= F::max_(S.zb, zc - dz) - P.CUT), th, ku);
+            }
+        } else {
+            // The lanes' (point, triangle) pairs of the box, packed 64 to a
+            // batch: a lane's candidates are the box's triangles within R of
+            // it whose cap reaches its threshold.  Lane p owns the pair
+            // slots offp .. offp + cntp - 1, its candidates in rank order.
+            uint32_t lmask = 0;
+            for (; tmask; tmask &= tmask - 1) {
+                const uint32_t j = (uint32_t)__builtin_ctz(tmask);
+                const T* r = my + kFastRec * j;
+                const T bx0 = r[R_BX0], bx1 = r[R_BX1], by0 = r[R_BY0], by1 = r[R_BY1];
+                bool on = act & !((bx1 < xl - P.R) | (bx0 > xl + P.R) | (by1 < yl - P.R) | (by0 > yl + P.R));
+                if (P.cull) {
+                    const T d2 = box_d2<T>(xl, yl, bx0, bx1, by0, by1);
+                    on = on & (r[R_ZCAP] - prof_lower<T, K>(d2, P) >= F::max_(S.zb, zc - dz) - P.CUT);
+                }
+                lmask |= on ? (1u << j) : 0u;
+            }
+            const uint32_t cntp = (uint32_t)__popc(lmask);
+            uint32_t offp = 0, total = 0;
+#pragma unroll
+            for (int bit = 0; bit < 5; ++bit) {  // cntp <= 16
+                const unsigned long long bal = __ballot((cntp >> bit) & 1u);
+                offp += (uint32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u))
+                        << bit;
+                total += (uint32_t)__popcll(bal) << bit;
+            }
+            total = uniform_u(total);
+            for (uint32_t base = 0; base < total; base += 64) {
+                // the owners write this batch's slots: (owner lane, triangle)
+                {
+                    uint32_t mm = lmask, g = offp;
+                    while (mm && g < base) {
+                        mm &= mm - 1;
+                        ++g;
                     }
+                    while (mm && g < base + 64) {
+                        ptab[g - base] = (ln << 4) | (uint32_t)__builtin_ctz(mm);
+                        mm &= mm - 1;
+                        ++g;
+                    }
+                }
+                wave_lds_sync();
+                // lane i evaluates slot base + i for its owner, from the
+                // owner's running best at the start of the batch
+                const bool has = base + ln < total;
+                const uint32_t ent = has ? ptab[ln] : 0u;
+                wave_lds_sync();
+                const uint32_t po = ent >> 4, j = ent & 15u;
+                const T px = __shfl(xl, (int)po, 64), py = __shfl(yl, (int)po, 64), pz = __shfl(S.zb, (int)po, 64);
+                Sel<T> Q;
+                sel_init(Q, pz);
+                T th;
+                sel_triangle<T, K>(my + kFastRec * j, t0 + j, px, py, P, Q, has, &th);
+                ++evals;
+                const uint32_t meta = (uint32_t)(Q.bf & 0xff) | (Q.bamb ? 0x100u : 0u) | (Q.pairs ? 0x200u : 0u) |
+                                      (Q.bt != 0xffffffffu ? 0x400u : 0u) | (j << 16);
+                // each owner folds its slots of the batch in rank order, as
+                // the lane's own scan of the box would have taken them
+                const uint32_t lo = max(offp, base), hi = min(offp + cntp, base + 64u);
+                const uint32_t nf = hi > lo ? hi - lo : 0u;
+                const uint32_t nmax = uniform_u(wave_max_u(nf));
+                for (uint32_t it = 0; it < nmax; ++it) {
+                    const bool v = it < nf;
+                    const int sl = v ? (int)(lo - base + it) : (int)ln;
+                    T qz[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) qz[i] = __shfl(Q.zt[i], sl, 64);
+                    const T qa = __shfl(Q.za, sl, 64), qb = __shfl(Q.zb, sl, 64), qe = __shfl(Q.eb, sl, 64);
+                    const T qth = __shfl(th, sl, 64);
+                    const uint32_t qm = (uint32_t)__shfl((int)meta, sl, 64);
+                    const uint32_t qk = t0 + (qm >> 16);
+                    if (v) {
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) S.zt[i] = F::max_(S.zt[i], qz[i]);
+                        S.za = F::max_(S.za, qa);
+                        S.pairs += (qm & 0x200u) ? 1u : 0u;
+                        if ((qm & 0x400u) && qb > S.zb) {
+                            S.zb = qb;
+                            S.eb = qe;
+                            S.bt = qk;
+                            S.bf = (int)(qm & 0xffu);
+                            S.bamb = (qm & 0x100u) != 0;
+                        }
+                    }
+                    if (refine) nt_add(v & (qth >= F::max_(S.zb, zc - dz) - P.CUT), qth, qk);
                 }
             }
         }
@@ -868,9 +973,10 @@ k_drop_select(SelArgs<T> A) {
     __shared__ float nt_h[4][kNearTie][64];
     __shared__ uint32_t bl_b[4][sizeof(T) == 4 ? kBoxList : 1];
     __shared__ float bl_key[4][sizeof(T) == 4 ? kBoxList : 1];
+    __shared__ uint32_t ptab[4][64];
     const uint32_t wv = threadIdx.x >> 6;
     const uint32_t gid = xcd_block(A.m.xcd_group) * blockDim.x + threadIdx.x;
-    select_points<T, K>(A, gid, srec[wv], nt_k[wv], nt_h[wv], bl_b[wv], bl_key[wv]);
+    select_points<T, K>(A, gid, srec[wv], nt_k[wv], nt_h[wv], bl_b[wv], bl_key[wv], ptab[wv]);
 }
 
 // The selection passes of a batch of jobs in one persistent launch: the
@@ -887,6 +993,7 @@ k_drop_select_jobs(const SelArgs<T>* __restrict__ jobs, const uint32_t* __restri
     __shared__ float nt_h[4][kNearTie][64];
     __shared__ uint32_t bl_b[4][sizeof(T) == 4 ? kBoxList : 1];
     __shared__ float bl_key[4][sizeof(T) == 4 ? kBoxList : 1];
+    __shared__ uint32_t ptab[4][64];
     const uint32_t wv = threadIdx.x >> 6;
     const uint32_t nchunks = chunk_off[njobs];
     for (;;) {
@@ -903,10 +1010,10 @@ k_drop_select_jobs(const SelArgs<T>* __restrict__ jobs, const uint32_t* __restri
         const SelArgs<T>& A = jobs[lo];
         const uint32_t gid = (c - chunk_off[lo]) * 64u + lane_id();
         switch (A.P.kind) {
-        case K_CYL: select_points<T, K_CYL>(A, gid, srec[wv], nt_k[wv], nt_h[wv], bl_b[wv], bl_key[wv]); break;
-        case K_BALL: select_points<T, K_BALL>(A, gid, srec[wv], nt_k[wv], nt_h[wv], bl_b[wv], bl_key[wv]); break;
-        case K_BULL: select_points<T, K_BULL>(A, gid, srec[wv], nt_k[wv], nt_h[wv], bl_b[wv], bl_key[wv]); break;
-        default: select_points<T, K_CONE>(A, gid, srec[wv], nt_k[wv], nt_h[wv], bl_b[wv], bl_key[wv]); break;
+        case K_CYL: select_points<T, K_CYL>(A, gid, srec[wv], nt_k[wv], nt_h[wv], bl_b[wv], bl_key[wv], ptab[wv]); break;
+        case K_BALL: select_points<T, K_BALL>(A, gid, srec[wv], nt_k[wv], nt_h[wv], bl_b[wv], bl_key[wv], ptab[wv]); break;
+        case K_BULL: select_points<T, K_BULL>(A, gid, srec[wv], nt_k[wv], nt_h[wv], bl_b[wv], bl_key[wv], ptab[wv]); break;
+        default: select_points<T, K_CONE>(A, gid, srec[wv], nt_k[wv], nt_h[wv], bl_b[wv], bl_key[wv], ptab[wv]); break;
         }
     }
 }
@@ -967,49 +1074,73 @@ __global__ void __launch_bounds__(256) k_drop_resolve(MeshView m, CutterDev cd, 
     if (cnt) wave_add(&cnt->brent, brent);
 }
 
-// Near-tie pass, one lane per queued point: the reference's fold
+// Near-tie pass over the queued points, 8 lanes per point (lane e takes
+// entry e of the point's near-tie set): the reference's fold
 // (MillingCutter::dropCutter over triangles in rank order) restricted to the
-// point's near-tie set and started at the bottom of its band.  Every other
-// triangle's contacts stay below that start, and for Cyl/Ball/Bull the
-// reference's result is the maximum of the triangles' own drops (ties to the
-// lowest rank), so when a set triangle lifts above the start the fold's
-// result is the reference's.  Points where none lifts go to `next`.
+// set and started at the bottom of its band.  Every other triangle's
+// contacts stay below that start.  For Cyl/Ball/Bull the fold's result is
+// the highest of the set triangles' own drops, ties to the lowest rank (a
+// triangle's own drop is its facet contact when that is valid, else its
+// best vertex/edge contact), so each lane computes one triangle's drop and
+// the 8 lanes take the arg-max.  Two rounding cases make the fold depend on
+// the running height (a vertex/edge contact above a valid facet contact of
+// the same triangle, a contact above its triangle's max z): such points, and
+// points where no set triangle lifts above the band, go to `next`.
 __global__ void __launch_bounds__(256) k_drop_neartie(MeshView m, CutterDev cd, const double* __restrict__ cl,
                                                       const uint32_t* __restrict__ ntq,
                                                       const uint32_t* __restrict__ ntq_n, NearTieSets nts, DropOut o,
                                                       uint32_t* __restrict__ next, uint32_t* __restrict__ next_n,
                                                       Counters* cnt) {
+    static_assert(kNearTie == 8, "one lane per near-tie set entry");
     const uint32_t n_eff = *ntq_n;
-    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t stride = gridDim.x * blockDim.x;  // a multiple of 64: the loop is wave-uniform
     unsigned long long brent = 0;
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; (g & ~63u) < n_eff; g += stride) {
-        const bool act = g < n_eff;
-        const uint32_t i = act ? ntq[g] : 0u;
-        bool done = false;
-        if (act) {
-            const uint32_t stt = nts.cnt[i];
-            const double x = cl[3 * (size_t)i], y = cl[3 * (size_t)i + 1];
-            DropLane E = fresh_lane(x, y, nts.zband[i]);
-            // the set holds distinct rank indices in evaluation order: fold
-            // them in rank order, each pass taking the next larger one
-            uint32_t prev = 0;
-            for (uint32_t p = 0; p < stt; ++p) {
-                uint32_t k = 0xffffffffu;
-                for (uint32_t j = 0; j < stt; ++j) {
-                    const uint32_t kk = nts.set[(size_t)i * kNearTie + j];
-                    if ((p == 0 || kk > prev) && kk < k) k = kk;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; ((g & ~63u) >> 3) < n_eff; g += stride) {
+        const uint32_t slot = g >> 3, e = g & 7u;
+        const bool act = slot < n_eff;
+        const uint32_t i = act ? ntq[slot] : 0u;
+        const uint32_t stt = act ? nts.cnt[i] : 0u;
+        const double x = act ? cl[3 * (size_t)i] : 0.0, y = act ? cl[3 * (size_t)i + 1] : 0.0;
+        const double zb = act ? nts.zband[i] : 0.0;
+        const bool has = act && e < stt;
+        const uint32_t k = has ? nts.set[(size_t)i * kNearTie + e] : 0xffffffffu;
+        DropLane A = fresh_lane(x, y, -INFINITY), V = fresh_lane(x, y, -INFINITY);
+        double maxz = -INFINITY;
+        if (has) {
+            const TriX t = load_tri(m.tri, k);
+            if (cutter_overlaps(cd.c.radius, x, y, t)) {
+                maxz = t.maxz;
+                A.cur_tri = V.cur_tri = k;
+                int it = 0;
+                exact_feature(cd.c, A, t, FT_FACET, it);
+                V.cur_feat = 0;
+                vertex_drop(cd, V, t);
+                if (V.z < t.maxz) {
+                    edge_simple(cd.c, V, t, it);
+                    brent += it;
                 }
-                prev = k;
-                const TriX t = load_tri(m.tri, k);
-                E.cur_tri = k;
-                if (cutter_overlaps(cd.c.radius, x, y, t)) brent += drop_triangle_exact(cd, E, t);
-            }
-            if (E.type != CC_NONE) {
-                write_drop(o, m.order, i, E);
-                done = true;
             }
         }
-        wave_push(act && !done, i, next, next_n);
+        const bool fv = A.type != CC_NONE, vv = V.type != CC_NONE;
+        const double val = fv ? A.z : (vv ? V.z : -INFINITY);
+        bool odd = (fv && vv && V.z > A.z) || ((fv || vv) && val > maxz);
+        double best = val;
+        uint32_t bk = k;
+        for (int off = 4; off > 0; off >>= 1) {
+            const double ob = __shfl_xor(best, off, 64);
+            const uint32_t ok = (uint32_t)__shfl_xor((int)bk, off, 64);
+            const bool take = (ob > best) || (ob == best && ok < bk);
+            best = take ? ob : best;
+            bk = take ? ok : bk;
+            odd = odd || (__shfl_xor((int)odd, off, 64) != 0);
+        }
+        const bool lift = !odd && best > zb;
+        if (lift && has && k == bk) {
+            DropLane L = fv ? A : V;
+            L.tri = k;
+            write_drop(o, m.order, i, L);
+        }
+        wave_push(act && e == 0 && !lift, i, next, next_n);
     }
     if (cnt) wave_add(&cnt->brent, brent);
 }
@@ -1201,11 +1332,12 @@ __global__ void __launch_bounds__(256) k_drop_coop_exact(MeshView m, CutterDev c
             // that is valid (a valid facet contact is the triangle's highest),
             // else its best vertex/edge contact, and the rank-ordered fold
             // keeps the highest of these, ties to the lowest rank: a wave
-            // arg-max.  A batch where some vertex/edge contact rounds above
-            // a valid facet contact (the fold's outcome then depends on the
-            // running height) and cones take the rank-ordered fold below.
+            // arg-max.  Batches where a contact rounds above a valid facet
+            // contact of its triangle or above its triangle's max z (the
+            // fold's outcome then depends on the running height) and cones
+            // take the rank-ordered fold below.
             const bool fv = A.type != CC_NONE, vv = V.type != CC_NONE;
-            if (c.kind != K_CONE && !__any(fv && vv && V.z > A.z)) {
+            if (c.kind != K_CONE && !__any((fv && vv && V.z > A.z) || ((fv || vv) && (fv ? A.z : V.z) > maxz))) {
                 double best = fv ? A.z : (vv ? V.z : -INFINITY);
                 if (!(maxz > zr)) best = -INFINITY;  // below()
                 uint32_t bl = lane;
