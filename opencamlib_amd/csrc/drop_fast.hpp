@@ -391,42 +391,55 @@ OCL_HD T prof_lower(T d2, const SelParams<T>& P) {
     return F::max_(h - P.HPAD, (T)0);
 }
 
-// Vertex j contact height (cutter base at the vertex, see sel_triangle) and
-// m = R^2 - q^2 (>= 0: the vertex is under the cutter).
+// Contact height of the vertex v = (x, y, z) (cutter base at the vertex, see
+// sel_triangle) and m = R^2 - q^2 (>= 0: the vertex is under the cutter).
 template <class T, int K>
-OCL_HD T vertex_height(const T* rec, int j, T x, T y, const SelParams<T>& P, T& m) {
+OCL_HD T vertex_height_at(const T* v, T x, T y, const SelParams<T>& P, T& m) {
     typedef FM<T> F;
-    const T dx = rec[R_P0X + 3 * j] - x, dy = rec[R_P0Y + 3 * j] - y;
+    const T dx = v[0] - x, dy = v[1] - y;
     const T q2 = F::fma_(dx, dx, dy * dy);
     m = P.R2 - q2;
     switch (K) {
-    case K_CYL: return rec[R_P0Z + 3 * j];
-    case K_BALL: return rec[R_P0Z + 3 * j] + F::sqrt_(F::max_(m, (T)0));
+    case K_CYL: return v[2];
+    case K_BALL: return v[2] + F::sqrt_(F::max_(m, (T)0));
     case K_BULL: {
         const T q = F::sqrt_(q2);
         const T w = F::max_(q - P.r1, (T)0);
         // flat part: z = p.z; torus: p.z - r2 + sqrt(r2^2 - w^2)  (base is p.z - r2)
-        return rec[R_P0Z + 3 * j] + F::sqrt_(F::max_(P.r22 - w * w, (T)0));
+        return v[2] + F::sqrt_(F::max_(P.r22 - w * w, (T)0));
     }
-    default: return rec[R_P0Z + 3 * j] - F::sqrt_(q2) * P.cot;  // cone
+    default: return v[2] - F::sqrt_(q2) * P.cot;  // cone
     }
 }
 
-// Lower bound of a lane's final height from one triangle's vertices: a vertex
-// clearly under the cutter is a contact the reference always reaches or
-// exceeds (for Cyl/Ball/Bull the facet contact it may take instead is the
-// maximum over the triangle).
+// Vertex j of a selection record.
 template <class T, int K>
-OCL_HD T vertex_floor(const T* rec, T x, T y, const SelParams<T>& P, bool on) {
+OCL_HD T vertex_height(const T* rec, int j, T x, T y, const SelParams<T>& P, T& m) {
+    static_assert(R_P0Y == R_P0X + 1 && R_P0Z == R_P0X + 2 && R_P1X == R_P0X + 3, "vertex layout");
+    return vertex_height_at<T, K>(rec + R_P0X + 3 * j, x, y, P, m);
+}
+
+// Lower bound of a lane's final height from one triangle's vertices (v: the
+// 9 coordinates p0, p1, p2): a vertex clearly under the cutter is a contact
+// the reference always reaches or exceeds (for Cyl/Ball/Bull the facet
+// contact it may take instead is the maximum over the triangle).
+template <class T, int K>
+OCL_HD T vertex_floor_at(const T* v, T x, T y, const SelParams<T>& P, bool on) {
     typedef FM<T> F;
     T zc = -F::inf();
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
         T m;
-        const T z = vertex_height<T, K>(rec, j, x, y, P, m);
+        const T z = vertex_height_at<T, K>(v + 3 * j, x, y, P, m);
         zc = F::max_(zc, (on & (m > P.S2V)) ? z - P.E : -F::inf());
     }
     return zc;
+}
+
+// The same from a selection record.
+template <class T, int K>
+OCL_HD T vertex_floor(const T* rec, T x, T y, const SelParams<T>& P, bool on) {
+    return vertex_floor_at<T, K>(rec + R_P0X, x, y, P, on);
 }
 
 // --------------------------------------------------------- lane state

@@ -588,9 +588,18 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
             }
             const T dz = (T)(bo.oz - ozw);
             const T xl = (T)(x - bo.ox), yl = (T)(y - bo.oy);
-            const uint32_t e = min(m.ntri, (b + 1) * kLeaf);
-            for (uint32_t k = b * kLeaf; k < e; ++k)
-                zc = F::max_(zc, vertex_floor<T, K>(rec + (size_t)kFastRec * uniform_u(k), xl, yl, P, act) + dz);
+            // the box's vertex coordinates (9 per triangle) through LDS, all
+            // loads in flight together
+            const uint32_t t0 = b * kLeaf;
+            const uint32_t nt = min(m.ntri - t0, (uint32_t)kLeaf);
+            const uint32_t nv = 9u * nt;
+            for (uint32_t i = lane_id(); i < 9u * kLeaf; i += 64) {
+                const uint32_t tr = i / 9u, c = i - 9u * tr;
+                if (i < nv) my[i] = rec[(size_t)kFastRec * (t0 + tr) + R_P0X + c];
+            }
+            wave_lds_sync();
+            for (uint32_t k = 0; k < nt; ++k) zc = F::max_(zc, vertex_floor_at<T, K>(my + 9u * k, xl, yl, P, act) + dz);
+            wave_lds_sync();
         });
     }
     // lane j holds the frame and cull cap of box base+j of the level-0 block
