@@ -120,6 +120,26 @@ def valu_from_profile(cutter):
     return got or None
 
 
+def overlap_pairs(tris, cl, R):
+    """Number of (CL point, triangle) pairs that pass the reference's
+    overlaps() test (millingcutter.cpp:422-433: the cutter's XY square
+    against the triangle's XY box), for a CL set that is a full product
+    grid: per triangle, the grid columns with x - R <= max x and x + R >=
+    min x times the rows likewise, in the same FP64 arithmetic.  This is
+    the pair count the reference's per-pair contact code runs on (before
+    its below() test); culling lets the selection kernel skip most of it."""
+    xs = np.unique(cl[:, 0])
+    ys = np.unique(cl[:, 1])
+    if len(xs) * len(ys) != len(cl):
+        return None
+    t = tris.reshape(-1, 3, 3)
+    mnx, mxx = t[:, :, 0].min(axis=1), t[:, :, 0].max(axis=1)
+    mny, mxy = t[:, :, 1].min(axis=1), t[:, :, 1].max(axis=1)
+    cx = np.searchsorted(xs - R, mxx, side="right") - np.searchsorted(xs + R, mnx, side="left")
+    cy = np.searchsorted(ys - R, mxy, side="right") - np.searchsorted(ys + R, mny, side="left")
+    return int(np.sum(np.maximum(cx, 0).astype(np.int64) * np.maximum(cy, 0).astype(np.int64)))
+
+
 def accuracy(got, ref, idx):
     dz = np.abs(got["z"][idx] - ref["z"])
     return {"max_dz_mm": float(dz.max()), "p99_dz_mm": float(np.percentile(dz, 99)),
@@ -391,6 +411,14 @@ def main():
                 "lane_utilisation": pairs / (64.0 * st["wave_evals"]) if st["wave_evals"] else None,
                 "kernel_ms": sel_ms, "traffic": traffic_from_profile(args.cutter),
                 "traffic_source": f"profiles/drop_select_{args.cutter}_c1_hbm_bytes.md"}
+    n_overlap = overlap_pairs(tris, cl, spec["d"] / 2.0)
+    if n_overlap is not None:
+        # the reference's work on the same input: its contact code on every
+        # overlapping pair, at this line's whole-step time (not `achieved`)
+        ref_tf = n_overlap * F_PAIR[args.cutter] / (ms * 1e-3) / 1e12
+        roofline["reference_equivalent"] = {
+            "overlap_pairs": n_overlap, "evaluated_fraction": pairs / n_overlap, "tflops": ref_tf,
+            "frac": ref_tf / PEAK_FP32, "frac_unpacked": ref_tf / PEAK_FP32_UNPACKED}
     counters = valu_from_profile(args.cutter)
     if counters:
         roofline["counters"] = {"valu_busy": counters.get("VALU_BUSY_FRACTION"),
