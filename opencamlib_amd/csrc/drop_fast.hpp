@@ -83,6 +83,7 @@ struct SelParams {
     T r1, r2, r22, ir2;  // bull
     T cot;          // cone 1/tan(angle)
     T ch;           // cone centre height
+    T LEN;          // cone length (its rim facet contact sits this far below the contact point)
     T DM;           // error bound of m = R^2 - q^2 (validity band |m| <= DM)
     T S2V;          // vertex: m below this -> height ill-conditioned
     T EAV;          // vertex: height slack of an ambiguous contact
@@ -91,8 +92,12 @@ struct SelParams {
     T E;            // z error bound of a well-conditioned contact
     T HPAD;         // rounding allowance of the profile lower bound (culling)
     T CUT;          // cull margin: 2 x the largest contact error bound of the mesh + HPAD
-    int cull;       // culling by profile bound enabled (not for cones: their
-                    // facet-first rule makes the result order dependent)
+    int cull;       // culling by profile bound enabled.  Cones cull against the
+                    // running best only (their facet-first rule makes the
+                    // result order dependent, so no floor of the final height
+                    // and no reordering): a triangle whose contacts all stay
+                    // below the running height at its turn cannot lift, and a
+                    // triangle that does not lift changes nothing
 };
 
 // Error model shared by the host launcher and the record builder.  Lane
@@ -341,6 +346,7 @@ OCL_HD SelParams<T> sel_params(const CutterDev& cd, const SelErr& er) {
     P.ir2 = (T)(c.r2 > 0 ? 1.0 / c.r2 : 0.0);
     P.cot = (T)(c.kind == K_CONE ? 1.0 / c.tan_a : 0.0);
     P.ch = (T)c.ch;
+    P.LEN = (T)c.length;
     const bool sq = (c.kind == K_BALL || c.kind == K_BULL);  // vertex height has sqrt(m)
     P.DM = (T)er.dm();
     P.S2V = (T)(sq ? er.s2band(1.0, er.E()) : er.dm());
@@ -355,7 +361,7 @@ OCL_HD SelParams<T> sel_params(const CutterDev& cd, const SelErr& er) {
     if (c.kind == K_BULL) P.E = (T)fmax((double)P.E, 2e-9);
     P.HPAD = (T)(16.0 * er.u * R);
     P.CUT = (T)INFINITY;  // set by the launcher once the records' largest error is known
-    P.cull = (c.kind == K_CYL || c.kind == K_BALL || c.kind == K_BULL) ? 1 : 0;
+    P.cull = (c.kind == K_CYL || c.kind == K_BALL || c.kind == K_BULL || (c.kind == K_CONE && c.length >= 0.0)) ? 1 : 0;
     return P;
 }
 
@@ -377,7 +383,12 @@ OCL_HD T box_d2(T x, T y, T x0, T x1, T y0, T y1) {
     return F::fma_(dx, dx, dy * dy);
 }
 
-// profile height at XY distance sqrt(d2), rounded down (0 for the cylinder)
+// profile height at XY distance sqrt(d2), rounded down (0 for the cylinder).
+// Cone: every contact at XY distance r from the axis sits r/tan(angle) below
+// its contact point (the vertex, edge and tip contacts, conecutter.cpp:
+// 109-135 and 70-105, including the hyperbola case), except the rim facet
+// contact, which sits the cutter length below a point at distance R: so
+// min(r / tan(angle), length) bounds them all.
 template <class T, int K>
 OCL_HD T prof_lower(T d2, const SelParams<T>& P) {
     typedef FM<T> F;
@@ -387,6 +398,9 @@ OCL_HD T prof_lower(T d2, const SelParams<T>& P) {
     } else if (K == K_BULL) {
         const T w = F::max_(F::sqrt_(d2) - P.r1, (T)0);
         h = P.r2 - F::sqrt_(F::max_(P.r22 - w * w, (T)0));
+    } else if (K == K_CONE) {
+        h = F::min_(F::sqrt_(d2) * P.cot, P.LEN);
+        h = h * ((T)1 - (T)(8.0 * F::eps()));
     }
     return F::max_(h - P.HPAD, (T)0);
 }

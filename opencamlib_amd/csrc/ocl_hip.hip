@@ -356,6 +356,38 @@ __device__ inline DropLane fresh_lane(double x, double y, double z) {
     return L;
 }
 
+// Profile height of the cutter at XY distance sqrt(d2) from its axis,
+// rounded down: every contact whose contact point lies that far from the
+// axis sits at least this much below the contact point (Cyl/Ball/Bull; cones
+// as prof_lower, when their length is not negative).  0 where no bound holds.
+__device__ inline double prof_lower64(const CutterP& c, double d2) {
+    double h = 0.0;
+    if (c.kind == K_BALL) {
+        h = c.radius - sqrt(fmax(c.radius * c.radius - d2, 0.0));
+    } else if (c.kind == K_BULL) {
+        const double w = fmax(sqrt(d2) - c.r1, 0.0);
+        h = c.r2 - sqrt(fmax(c.r2 * c.r2 - w * w, 0.0));
+    } else if (c.kind == K_CONE && c.length >= 0.0) {
+        h = fmin(sqrt(d2) / c.tan_a, c.length);
+    }
+    return fmax(h - 1e-12 * fmax(c.radius, h), 0.0);
+}
+
+// Cutters whose contacts are bounded by max z - prof_lower64(distance).
+__device__ inline bool has_contact_cap(const CutterP& c) {
+    return c.kind == K_CYL || c.kind == K_BALL || c.kind == K_BULL || (c.kind == K_CONE && c.length >= 0.0);
+}
+
+// Upper bound of every contact height of triangle t for a CL point at
+// (x, y): max z minus the cutter profile height at the XY distance to the
+// triangle's box.
+__device__ inline double contact_cap(const CutterP& c, const TriX& t, double x, double y) {
+    const double bx0 = fmin(fmin(t.p[0].x, t.p[1].x), t.p[2].x), bx1 = fmax(fmax(t.p[0].x, t.p[1].x), t.p[2].x);
+    const double by0 = fmin(fmin(t.p[0].y, t.p[1].y), t.p[2].y), by1 = fmax(fmax(t.p[0].y, t.p[1].y), t.p[2].y);
+    const double dx = fmax(fmax(bx0 - x, x - bx1), 0.0), dy = fmax(fmax(by0 - y, y - by1), 0.0);
+    return t.maxz - prof_lower64(c, dx * dx + dy * dy);
+}
+
 // Exact FP64 drop over a list of points (identity list or escalation list).
 // One lane per point; the wavefront walks the hierarchy with the union box.
 __global__ void __launch_bounds__(256) k_drop_exact(MeshView m, CutterDev cd, const double* __restrict__ cl,
@@ -378,6 +410,11 @@ __global__ void __launch_bounds__(256) k_drop_exact(MeshView m, CutterDev cd, co
     const float qy1 = uniform_f(__double2float_ru(wave_max(act ? y : -INFINITY) + R));
     DropLane L = fresh_lane(x, y, z);
     unsigned long long pairs = 0, brent = 0;
+    // a triangle whose contacts all stay below the running height cannot
+    // lift, and in the reference's fold a triangle that does not lift
+    // changes nothing: skipped (not for composites, whose sub-cutter offsets
+    // break the bound)
+    const bool capped = cd.c.kind != K_COMPOSITE && has_contact_cap(cd.c);
     walk_boxes(m.tree, qx0, qy0, qx1, qy1, [&](uint32_t b) {
         const uint32_t e = min(m.ntri, (b + 1) * kLeaf);
         for (uint32_t k = b * kLeaf; k < e; ++k) {
@@ -387,7 +424,8 @@ __global__ void __launch_bounds__(256) k_drop_exact(MeshView m, CutterDev cd, co
                 L.cur_tri = ku;
                 if (cutter_overlaps(R, L.x, L.y, t)) {
                     if (L.z < t.maxz) ++pairs;
-                    brent += drop_triangle_exact(cd, L, t);
+                    if (!capped || contact_cap(cd.c, t, L.x, L.y) >= L.z - 1e-9 * (1.0 + fabs(L.z)))
+                        brent += drop_triangle_exact(cd, L, t);
                 }
             }
         }
@@ -633,8 +671,8 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
         }
         const T dz = (T)(bo.oz - ozw);
         const T xl = (T)(x - bo.ox), yl = (T)(y - bo.oy);
-        if (K != K_CONE && P.cull) {
-            // no lane can gain anything from this box
+        if (P.cull) {
+            // no lane can gain anything from this box (zc is -inf for cones)
             const T d2 = box_d2<T>(xl, yl, -(T)bo.hx, (T)bo.hx, -(T)bo.hy, (T)bo.hy);
             const bool rel = act & (cap + dz - prof_lower<T, K>(d2, P) >= F::max_(W.zb, zc) - P.CUT);
             if (!__any(rel)) return;
@@ -676,12 +714,12 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
             const uint32_t jl = lane_id();
             bool keep = false;
             T thr_min = F::inf();
-            if (K != K_CONE && P.cull) thr_min = wave_min_t<T>(act ? F::max_(S.zb, zc - dz) : F::inf()) - P.CUT;
+            if (P.cull) thr_min = wave_min_t<T>(act ? F::max_(S.zb, zc - dz) : F::inf()) - P.CUT;
             if (jl < nt) {
                 const T* r = my + kFastRec * jl;
                 const T bx0 = r[R_BX0], bx1 = r[R_BX1], by0 = r[R_BY0], by1 = r[R_BY1];
                 keep = !((bx1 < lx0) | (bx0 > lx1) | (by1 < ly0) | (by0 > ly1));
-                if (K != K_CONE && P.cull) {
+                if (P.cull) {
                     const T hx0 = (T)(hull_x0 - bo.ox), hx1 = (T)(hull_x1 - bo.ox);
                     const T hy0 = (T)(hull_y0 - bo.oy), hy1 = (T)(hull_y1 - bo.oy);
                     const T ddx = F::max_(F::max_(bx0 - hx1, hx0 - bx1), (T)0);
@@ -730,7 +768,7 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
                 const T* r = my + kFastRec * j;
                 const T bx0 = r[R_BX0], bx1 = r[R_BX1], by0 = r[R_BY0], by1 = r[R_BY1];
                 bool on = act & !((bx1 < xl - P.R) | (bx0 > xl + P.R) | (by1 < yl - P.R) | (by0 > yl + P.R));
-                if (K != K_CONE && P.cull) {
+                if (P.cull) {
                     const T d2 = box_d2<T>(xl, yl, bx0, bx1, by0, by1);
                     on = on & (r[R_ZCAP] - prof_lower<T, K>(d2, P) >= F::max_(S.zb, zc - dz) - P.CUT);
                 }
@@ -1213,15 +1251,7 @@ __device__ inline void coop_gather(const MeshView& m, uint32_t* qbuf, double x, 
                     const double ox = readlane_d(pox, jb), oy = readlane_d(poy, jb), oz = readlane_d(poz, jb);
                     const double hx = readlane_f(phx, jb), hy = readlane_f(phy, jb);
                     const double dx = fmax(fabs(x - ox) - hx, 0.0), dy = fmax(fabs(y - oy) - hy, 0.0);
-                    const double d2 = dx * dx + dy * dy;
-                    double h = 0.0;
-                    if (c.kind == K_BALL) {
-                        h = R - sqrt(fmax(R * R - d2, 0.0));
-                    } else if (c.kind == K_BULL) {
-                        const double wd = fmax(sqrt(d2) - c.r1, 0.0);
-                        h = c.r2 - sqrt(fmax(c.r2 * c.r2 - wd * wd, 0.0));
-                    }
-                    if (readlane_d(pcap, jb) + oz - fmax(h - 1e-12 * R, 0.0) < thr()) return true;
+                            if (readlane_d(pcap, jb) + oz - prof_lower64(c, dx * dx + dy * dy) < thr()) return true;
                 }
                 pb0 = (npend == 0) ? b : pb0;
                 pb1 = (npend == 1) ? b : pb1;
@@ -1258,23 +1288,6 @@ __device__ inline void coop_gather(const MeshView& m, uint32_t* qbuf, double x, 
     }
 }
 
-// Upper bound of every contact height of triangle t for a CL point at
-// (x, y): max z minus the cutter profile height at the XY distance to the
-// triangle's box (Cyl/Ball/Bull; the profile grows with the distance).
-__device__ inline double contact_cap(const CutterP& c, const TriX& t, double x, double y) {
-    const double bx0 = fmin(fmin(t.p[0].x, t.p[1].x), t.p[2].x), bx1 = fmax(fmax(t.p[0].x, t.p[1].x), t.p[2].x);
-    const double by0 = fmin(fmin(t.p[0].y, t.p[1].y), t.p[2].y), by1 = fmax(fmax(t.p[0].y, t.p[1].y), t.p[2].y);
-    const double dx = fmax(fmax(bx0 - x, x - bx1), 0.0), dy = fmax(fmax(by0 - y, y - by1), 0.0);
-    const double d2 = dx * dx + dy * dy;
-    double h = 0.0;
-    if (c.kind == K_BALL) {
-        h = c.radius - sqrt(fmax(c.radius * c.radius - d2, 0.0));
-    } else if (c.kind == K_BULL) {
-        const double w = fmax(sqrt(d2) - c.r1, 0.0);
-        h = c.r2 - sqrt(fmax(c.r2 * c.r2 - w * w, 0.0));
-    }
-    return t.maxz - fmax(h - 1e-12 * c.radius, 0.0);
-}
 
 // Exact FP64 tier: reproduces the reference's sequential dropCutter fold.
 // Per triangle the lane computes, from z = -inf, the facet (cone: tip)
@@ -1308,15 +1321,18 @@ __global__ void __launch_bounds__(256) k_drop_coop_exact(MeshView m, CutterDev c
         // Cyl/Ball/Bull: the reference's result is the maximum over the
         // triangles' own drops, so a triangle whose contacts all stay below
         // a known floor of that maximum cannot lift or tie.  Cones are
-        // order dependent and never skip.
+        // order dependent: they skip only triangles whose contacts all stay
+        // below the running height (which at a triangle's turn is at least
+        // the running height when it is gathered), as those cannot lift.
         const bool cull = zfloor && (c.kind == K_CYL || c.kind == K_BALL || c.kind == K_BULL);
+        const bool crun = cull || (c.kind == K_CONE && has_contact_cap(c));
         const double zf = cull ? zfloor[src] : -INFINITY;
         auto thr = [&]() {
             const double t = fmax(zf, zr);
             return t - 1e-9 * (1.0 + fabs(t));
         };
-        auto keep = [&](uint32_t, const TriX& t) { return !cull || contact_cap(c, t, x, y) >= thr(); };
-        const BoxCull bcl{cull ? boxcap : nullptr, org};
+        auto keep = [&](uint32_t, const TriX& t) { return !crun || contact_cap(c, t, x, y) >= thr(); };
+        const BoxCull bcl{crun ? boxcap : nullptr, org};
         coop_gather(m, qbuf, x, y, c, bcl, thr, keep, [&](uint32_t* q, uint32_t nb) {
             const bool has = lane < nb;
             const uint32_t k = has ? q[lane] : 0u;

@@ -60,7 +60,7 @@ void run_points(const std::vector<TriX>& tx, const std::vector<BoxOrigin>& org, 
         const size_t nbox = (ntri + kLeafH - 1) / kLeafH;
         // floor of the final height from the vertices of the boxes under the point
         T zc = -F::inf();
-        if (P.cull)
+        if (P.cull && K != K_CONE)
             for (size_t b = 0; b < nbox; ++b) {
                 const double* bb = &bbx[4 * b];
                 if (bb[1] < x || bb[0] > x || bb[3] < y || bb[2] > y) continue;
