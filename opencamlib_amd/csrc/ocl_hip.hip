@@ -378,14 +378,46 @@ __device__ inline bool has_contact_cap(const CutterP& c) {
     return c.kind == K_CYL || c.kind == K_BALL || c.kind == K_BULL || (c.kind == K_CONE && c.length >= 0.0);
 }
 
+// Squared XY distance from (x, y) to triangle t's XY box.
+__device__ inline double tri_box_d2(const TriX& t, double x, double y) {
+    const double bx0 = fmin(fmin(t.p[0].x, t.p[1].x), t.p[2].x), bx1 = fmax(fmax(t.p[0].x, t.p[1].x), t.p[2].x);
+    const double by0 = fmin(fmin(t.p[0].y, t.p[1].y), t.p[2].y), by1 = fmax(fmax(t.p[0].y, t.p[1].y), t.p[2].y);
+    const double dx = fmax(fmax(bx0 - x, x - bx1), 0.0), dy = fmax(fmax(by0 - y, y - by1), 0.0);
+    return dx * dx + dy * dy;
+}
+
 // Upper bound of every contact height of triangle t for a CL point at
 // (x, y): max z minus the cutter profile height at the XY distance to the
 // triangle's box.
 __device__ inline double contact_cap(const CutterP& c, const TriX& t, double x, double y) {
-    const double bx0 = fmin(fmin(t.p[0].x, t.p[1].x), t.p[2].x), bx1 = fmax(fmax(t.p[0].x, t.p[1].x), t.p[2].x);
-    const double by0 = fmin(fmin(t.p[0].y, t.p[1].y), t.p[2].y), by1 = fmax(fmax(t.p[0].y, t.p[1].y), t.p[2].y);
-    const double dx = fmax(fmax(bx0 - x, x - bx1), 0.0), dy = fmax(fmax(by0 - y, y - by1), 0.0);
-    return t.maxz - prof_lower64(c, dx * dx + dy * dy);
+    return t.maxz - prof_lower64(c, tri_box_d2(t, x, y));
+}
+
+// Composite cutters (compositecutter.cpp:150-194 and the vertex drop): the
+// contact of sub-cutter n is accepted only when its contact point lies in
+// the radius window [rlim[n-1], rlim[n]] (1e-6 tolerance), and it sits
+// h_n(r) + zoff[n] below the contact point, h_n being the sub-cutter's
+// profile (non-decreasing in r).  So every accepted contact of a triangle at
+// XY box distance d sits at least min over the windows reaching d of
+// h_n(max(d, window start)) + zoff[n] below the triangle's max z; with
+// negative offsets this may be negative (contacts above max z).
+__device__ inline bool comp_has_cap(const CutterDev& cd) {
+    if (cd.c.kind != K_COMPOSITE || cd.nsub < 1) return false;
+    for (int n = 0; n < cd.nsub; ++n)
+        if (!has_contact_cap(cd.sub[n])) return false;
+    return true;
+}
+__device__ inline double comp_prof_lower64(const CutterDev& cd, double d2) {
+    const double d = sqrt(d2);
+    double h = INFINITY;
+    for (int n = 0; n < cd.nsub; ++n) {
+        const double lo = (n == 0) ? 0.0 : cd.rlim[n - 1] - 1e-6, hi = cd.rlim[n] + 1e-6;
+        if (d > hi) continue;
+        const double r = fmax(d, lo);
+        const double hn = prof_lower64(cd.sub[n], r * r) + cd.zoff[n];
+        h = fmin(h, hn - 1e-12 * (cd.c.radius + fabs(hn)));
+    }
+    return h;  // +inf: no window reaches d, the triangle has no accepted contact
 }
 
 // Exact FP64 drop over a list of points (identity list or escalation list).
@@ -412,9 +444,9 @@ __global__ void __launch_bounds__(256) k_drop_exact(MeshView m, CutterDev cd, co
     unsigned long long pairs = 0, brent = 0;
     // a triangle whose contacts all stay below the running height cannot
     // lift, and in the reference's fold a triangle that does not lift
-    // changes nothing: skipped (not for composites, whose sub-cutter offsets
-    // break the bound)
-    const bool capped = cd.c.kind != K_COMPOSITE && has_contact_cap(cd.c);
+    // changes nothing: skipped
+    const bool comp = cd.c.kind == K_COMPOSITE;
+    const bool capped = comp ? comp_has_cap(cd) : has_contact_cap(cd.c);
     walk_boxes(m.tree, qx0, qy0, qx1, qy1, [&](uint32_t b) {
         const uint32_t e = min(m.ntri, (b + 1) * kLeaf);
         for (uint32_t k = b * kLeaf; k < e; ++k) {
@@ -424,8 +456,10 @@ __global__ void __launch_bounds__(256) k_drop_exact(MeshView m, CutterDev cd, co
                 L.cur_tri = ku;
                 if (cutter_overlaps(R, L.x, L.y, t)) {
                     if (L.z < t.maxz) ++pairs;
-                    if (!capped || contact_cap(cd.c, t, L.x, L.y) >= L.z - 1e-9 * (1.0 + fabs(L.z)))
-                        brent += drop_triangle_exact(cd, L, t);
+                    const double cap = !capped ? INFINITY
+                                       : t.maxz - (comp ? comp_prof_lower64(cd, tri_box_d2(t, L.x, L.y))
+                                                        : prof_lower64(cd.c, tri_box_d2(t, L.x, L.y)));
+                    if (cap >= L.z - 1e-9 * (1.0 + fabs(L.z))) brent += drop_triangle_exact(cd, L, t);
                 }
             }
         }
