@@ -35,6 +35,7 @@ struct MeshView {
     uint32_t ntri;
     uint32_t xcd_group;     // consecutive logical blocks per XCD (see xcd_block)
     BoxTree tree;
+    const double* bzmax;    // highest vertex per level-0 box (or NULL)
 };
 
 }  // namespace ocl_amd

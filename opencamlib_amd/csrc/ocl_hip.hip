@@ -104,6 +104,7 @@ struct ocl_mesh {
     ocl_amd::DevBuf keys, perm, hist, cursor, cl, out_z, out_cc, out_ty, out_tri, out_ft, out_dz, counters, fastrec;
     ocl_amd::DevBuf esc_list, esc_list2, esc_list3, esc_flag, blocksum, qcount, cnt2, cnt3;
     ocl_amd::DevBuf origins;     // frame (BoxOrigin) per level-0 box
+    ocl_amd::DevBuf boxzmax;     // highest vertex per level-0 box
     ocl_amd::DevBuf zfloor;      // per CL point: floor of the final z handed to the escalation passes
     ocl_amd::DevBuf ntset, ntcnt, zband, selrec;  // per CL point: selection records and near-tie sets
     int exact_slot = 0;          // queue-count slot read by the exact pass in the last call
@@ -123,6 +124,7 @@ struct ocl_mesh {
         v.tree.box = boxes.as<const float4>();
         v.tree.lvl = levels.as<const uint32_t>();
         v.tree.top = top;
+        v.bzmax = boxzmax.as<const double>();
         return v;
     }
 };
@@ -149,6 +151,16 @@ __global__ void k_tri_prep(const double* __restrict__ src, double* __restrict__ 
 }
 
 // Level-0 boxes: XY box of kLeaf consecutive triangles, rounded outward.
+// Highest vertex of each level-0 box (the exact path's box-level cap).
+__global__ void k_box_zmax(const double* __restrict__ rec, uint32_t ntri, double* __restrict__ out, uint32_t nbox) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    double z = -INFINITY;
+    const uint32_t e = min(ntri, (b + 1) * kLeaf);
+    for (uint32_t i = b * kLeaf; i < e; ++i) z = fmax(z, rec[(size_t)kRec * i + 12]);
+    out[b] = z;
+}
+
 __global__ void k_box_leaf(const double* __restrict__ rec, uint32_t ntri, float4* __restrict__ out, uint32_t nbox) {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbox) return;
@@ -448,6 +460,16 @@ __global__ void __launch_bounds__(256) k_drop_exact(MeshView m, CutterDev cd, co
     const bool comp = cd.c.kind == K_COMPOSITE;
     const bool capped = comp ? comp_has_cap(cd) : has_contact_cap(cd.c);
     walk_boxes(m.tree, qx0, qy0, qx1, qy1, [&](uint32_t b) {
+        if (capped && m.bzmax) {
+            // the same bound for the whole box: its highest vertex at the
+            // distance to its (outward rounded) XY box
+            const float4 bb = m.tree.box[m.tree.lvl[1] + b];
+            const double dx = fmax(fmax((double)bb.x - x, x - (double)bb.z), 0.0);
+            const double dy = fmax(fmax((double)bb.y - y, y - (double)bb.w), 0.0);
+            const double d2 = dx * dx + dy * dy;
+            const double cap = m.bzmax[b] - (comp ? comp_prof_lower64(cd, d2) : prof_lower64(cd.c, d2));
+            if (!__any(act && cap >= L.z - 1e-9 * (1.0 + fabs(L.z)))) return;
+        }
         const uint32_t e = min(m.ntri, (b + 1) * kLeaf);
         for (uint32_t k = b * kLeaf; k < e; ++k) {
             const uint32_t ku = uniform_u(k);
@@ -2074,6 +2096,11 @@ int ocl_hip_mesh_create(const double* tri_xyz, size_t ntri, unsigned bucket, ocl
         return rc;
     }
     k_box_frames<<<(m->cnt[0] + 255) / 256, 256>>>(m->tri.as<double>(), n32, boxes, m->cnt[0], m->origins.as<BoxOrigin>());
+    if ((rc = m->boxzmax.ensure(sizeof(double) * m->cnt[0] + 16))) {
+        delete m;
+        return rc;
+    }
+    k_box_zmax<<<(m->cnt[0] + 255) / 256, 256>>>(m->tri.as<double>(), n32, m->boxzmax.as<double>(), m->cnt[0]);
     e = hipDeviceSynchronize();
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) {
@@ -2251,6 +2278,7 @@ static MeshView tree_view(const ocl_tree& T, size_t ntri) {
     v.tree.box = T.boxes.as<const float4>();
     v.tree.lvl = T.levels.as<const uint32_t>();
     v.tree.top = T.top;
+    v.bzmax = nullptr;
     return v;
 }
 
