@@ -433,13 +433,12 @@ __device__ inline double comp_prof_lower64(const CutterDev& cd, double d2) {
 }
 
 // Exact FP64 drop over a list of points (identity list or escalation list).
-// One lane per point; the wavefront walks the hierarchy with the union box.
-__global__ void __launch_bounds__(256) k_drop_exact(MeshView m, CutterDev cd, const double* __restrict__ cl,
-                                                    const uint32_t* __restrict__ list,
-                                                    const uint32_t* __restrict__ count, uint32_t n, DropOut o,
-                                                    Counters* cnt) {
-    const uint32_t n_eff = count ? *count : n;
-    const uint32_t gid = xcd_block(m.xcd_group) * blockDim.x + threadIdx.x;
+// One lane per point (gid = position in the list); the wavefront walks the
+// hierarchy with the union box.
+__device__ __attribute__((always_inline)) inline void exact_points(const MeshView& m, const CutterDev& cd,
+                                                                  const double* __restrict__ cl,
+                                                                  const uint32_t* __restrict__ list, uint32_t n_eff,
+                                                                  const DropOut& o, Counters* cnt, uint32_t gid) {
     const uint32_t wave_first = gid & ~63u;
     if (wave_first >= n_eff) return;
     const bool act = gid < n_eff;
@@ -491,6 +490,42 @@ __global__ void __launch_bounds__(256) k_drop_exact(MeshView m, CutterDev cd, co
         wave_add(&cnt->pairs, pairs);
         wave_add(&cnt->brent, brent);
     }
+}
+
+__global__ void __launch_bounds__(256) k_drop_exact(MeshView m, CutterDev cd, const double* __restrict__ cl,
+                                                    const uint32_t* __restrict__ list,
+                                                    const uint32_t* __restrict__ count, uint32_t n, DropOut o,
+                                                    Counters* cnt) {
+    exact_points(m, cd, cl, list, count ? *count : n, o, cnt, xcd_block(m.xcd_group) * blockDim.x + threadIdx.x);
+}
+
+// The exact-path jobs of a batch (composites, or every job in EXACT64
+// mode) in one launch: wavefront c takes 64-point chunk c of the
+// concatenation of the jobs' chunks (chunk_off: prefix sums, largest jobs
+// first), so small jobs run beside the large ones instead of one after
+// another.
+struct ExactJob {
+    MeshView m;
+    CutterDev cd;
+    const double* cl;
+    const uint32_t* list;
+    uint32_t n;
+    DropOut o;
+    Counters* cnt;
+};
+
+__global__ void __launch_bounds__(256) k_drop_exact_jobs(const ExactJob* __restrict__ jobs,
+                                                         const uint32_t* __restrict__ chunk_off, uint32_t njobs) {
+    const uint32_t c = uniform_u(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (c >= chunk_off[njobs]) return;
+    uint32_t lo = 0, hi = njobs;  // the job: last j with chunk_off[j] <= c
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (chunk_off[mid] <= c) lo = mid;
+        else hi = mid;
+    }
+    const ExactJob& J = jobs[lo];
+    exact_points(J.m, J.cd, J.cl, J.list, J.n, J.o, J.cnt, (c - chunk_off[lo]) * 64u + lane_id());
 }
 
 
@@ -2638,8 +2673,8 @@ struct ocl_batch {
     int precision = OCL_PREC_MIXED;
     ocl_tune tune{};
     uint32_t total = 0, hb = 8, key_bits = 16;
-    ocl_amd::DevBuf keys, vals, scratch, keyjobs, zeros, seljobs, chunk_off, queue;
-    uint32_t nzero = 0, nsel = 0, grid = 0;
+    ocl_amd::DevBuf keys, vals, scratch, keyjobs, zeros, seljobs, chunk_off, queue, exjobs, exoff;
+    uint32_t nzero = 0, nsel = 0, grid = 0, nexact = 0, exchunks = 0;
     uint32_t* sorted_vals = nullptr;
     std::vector<uint32_t> sel_order;  // job indices of the queue, largest first
     hipStream_t cap = nullptr;
@@ -2671,14 +2706,12 @@ static int batch_record(ocl_batch* b, hipStream_t st, bool timed) {
     if (vo != b->sorted_vals) return fail(OCL_E_HIP, "batch: sort buffer mismatch");
     HIP_TRY(hipGetLastError());
     for (const BatchJob& J : b->jobs) {
-        if (J.fast || J.n == 0) continue;
-        const uint32_t nb = (J.n + 255) / 256;
-        if (J.m->ntri > 0)
-            k_drop_exact<<<xcd_grid(nb, J.mv.xcd_group), 256, 0, st>>>(J.mv, J.cd, J.cl, b->sorted_vals + J.base,
-                                                                       nullptr, J.n, J.o, J.m->counters.as<Counters>());
-        else
-            k_drop_none<<<nb, 256, 0, st>>>(J.cl, J.n, J.o);
+        if (J.fast || J.n == 0 || J.m->ntri > 0) continue;
+        k_drop_none<<<(J.n + 255) / 256, 256, 0, st>>>(J.cl, J.n, J.o);
     }
+    if (b->nexact)
+        k_drop_exact_jobs<<<(b->exchunks + 3) / 4, 256, 0, st>>>(b->exjobs.as<ExactJob>(), b->exoff.as<uint32_t>(),
+                                                                 b->nexact);
     if (timed) HIP_TRY(hipEventRecord(b->s0, st));
     if (b->nsel) {
         if (b->tune.per_job_launch) {
@@ -2838,6 +2871,26 @@ int ocl_hip_batch_create(const ocl_drop_job* jobs, size_t njobs, int precision, 
         if (!a.empty()) HIP_TRY(hipMemcpy(b->seljobs.p, a.data(), argsz * a.size(), hipMemcpyHostToDevice));
     }
     HIP_TRY(hipMemcpy(b->chunk_off.p, coff.data(), 4 * coff.size(), hipMemcpyHostToDevice));
+    {
+        // exact-path jobs, largest first
+        std::vector<uint32_t> ex;
+        for (const BatchJob& J : b->jobs)
+            if (!J.fast && J.n > 0 && J.m->ntri > 0) ex.push_back((uint32_t)(&J - b->jobs.data()));
+        std::stable_sort(ex.begin(), ex.end(), [&](uint32_t a, uint32_t c) { return b->jobs[a].n > b->jobs[c].n; });
+        std::vector<ExactJob> ej;
+        std::vector<uint32_t> eoff(1, 0);
+        for (uint32_t j : ex) {
+            const BatchJob& J = b->jobs[j];
+            ej.push_back(ExactJob{J.mv, J.cd, J.cl, b->sorted_vals + J.base, J.n, J.o, J.m->counters.as<Counters>()});
+            eoff.push_back(eoff.back() + (J.n + 63) / 64);
+        }
+        b->nexact = (uint32_t)ej.size();
+        b->exchunks = eoff.back();
+        if ((rc = b->exjobs.ensure(sizeof(ExactJob) * (ej.size() + 1))) || (rc = b->exoff.ensure(4 * eoff.size())))
+            return rc;
+        if (!ej.empty()) HIP_TRY(hipMemcpy(b->exjobs.p, ej.data(), sizeof(ExactJob) * ej.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b->exoff.p, eoff.data(), 4 * eoff.size(), hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipMemcpy(b->keyjobs.p, kj.data(), sizeof(KeyJob) * njobs, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b->zeros.p, zr.data(), sizeof(ZeroRange) * zr.size(), hipMemcpyHostToDevice));
     b->nzero = (uint32_t)zr.size();
