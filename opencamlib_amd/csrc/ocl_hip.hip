@@ -1168,13 +1168,12 @@ k_drop_select_jobs(const SelArgs<T>* __restrict__ jobs, const uint32_t* __restri
 //    rank), so when a set triangle lifts above the start the fold's result
 //    is the reference's.
 // Everything else goes on to the cooperative passes.
-__global__ void __launch_bounds__(256) k_drop_resolve(MeshView m, CutterDev cd, const double* __restrict__ cl,
-                                                      const uint32_t* __restrict__ perm, uint32_t n, NearTieSets nts,
-                                                      double etie, DropOut o, uint32_t* __restrict__ next,
-                                                      uint32_t* __restrict__ next_n, Counters* cnt) {
+__device__ __attribute__((always_inline)) inline void resolve_points(
+    const MeshView& m, const CutterDev& cd, const double* __restrict__ cl, const uint32_t* __restrict__ perm,
+    uint32_t n, const NearTieSets& nts, double etie, const DropOut& o, uint32_t* __restrict__ next,
+    uint32_t* __restrict__ next_n, Counters* cnt, uint32_t g) {
     // points in the Hilbert order of the selection pass: neighbouring lanes
     // refine the same few triangles
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if ((g & ~63u) >= n) return;
     const bool act = g < n;
     const uint32_t i = act ? perm[g] : 0u;
@@ -1212,6 +1211,13 @@ __global__ void __launch_bounds__(256) k_drop_resolve(MeshView m, CutterDev cd, 
     if (cnt) wave_add(&cnt->brent, brent);
 }
 
+__global__ void __launch_bounds__(256) k_drop_resolve(MeshView m, CutterDev cd, const double* __restrict__ cl,
+                                                      const uint32_t* __restrict__ perm, uint32_t n, NearTieSets nts,
+                                                      double etie, DropOut o, uint32_t* __restrict__ next,
+                                                      uint32_t* __restrict__ next_n, Counters* cnt) {
+    resolve_points(m, cd, cl, perm, n, nts, etie, o, next, next_n, cnt, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Near-tie pass over the queued points, 8 lanes per point (lane e takes
 // entry e of the point's near-tie set): the reference's fold
 // (MillingCutter::dropCutter over triangles in rank order) restricted to the
@@ -1224,16 +1230,14 @@ __global__ void __launch_bounds__(256) k_drop_resolve(MeshView m, CutterDev cd, 
 // the running height (a vertex/edge contact above a valid facet contact of
 // the same triangle, a contact above its triangle's max z): such points, and
 // points where no set triangle lifts above the band, go to `next`.
-__global__ void __launch_bounds__(256) k_drop_neartie(MeshView m, CutterDev cd, const double* __restrict__ cl,
-                                                      const uint32_t* __restrict__ ntq,
-                                                      const uint32_t* __restrict__ ntq_n, NearTieSets nts, DropOut o,
-                                                      uint32_t* __restrict__ next, uint32_t* __restrict__ next_n,
-                                                      Counters* cnt) {
+// g0: the thread's index in the grid, stride: the grid's thread count (a
+// multiple of 64: the loop is wave-uniform).
+__device__ __attribute__((always_inline)) inline void neartie_points(
+    const MeshView& m, const CutterDev& cd, const double* __restrict__ cl, const uint32_t* __restrict__ ntq,
+    uint32_t n_eff, const NearTieSets& nts, const DropOut& o, uint32_t* __restrict__ next,
+    uint32_t* __restrict__ next_n, unsigned long long& brent, uint32_t g0, uint32_t stride) {
     static_assert(kNearTie == 8, "one lane per near-tie set entry");
-    const uint32_t n_eff = *ntq_n;
-    const uint32_t stride = gridDim.x * blockDim.x;  // a multiple of 64: the loop is wave-uniform
-    unsigned long long brent = 0;
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; ((g & ~63u) >> 3) < n_eff; g += stride) {
+    for (uint32_t g = g0; ((g & ~63u) >> 3) < n_eff; g += stride) {
         const uint32_t slot = g >> 3, e = g & 7u;
         const bool act = slot < n_eff;
         const uint32_t i = act ? ntq[slot] : 0u;
@@ -1280,6 +1284,16 @@ __global__ void __launch_bounds__(256) k_drop_neartie(MeshView m, CutterDev cd, 
         }
         wave_push(act && e == 0 && !lift, i, next, next_n);
     }
+}
+
+__global__ void __launch_bounds__(256) k_drop_neartie(MeshView m, CutterDev cd, const double* __restrict__ cl,
+                                                      const uint32_t* __restrict__ ntq,
+                                                      const uint32_t* __restrict__ ntq_n, NearTieSets nts, DropOut o,
+                                                      uint32_t* __restrict__ next, uint32_t* __restrict__ next_n,
+                                                      Counters* cnt) {
+    unsigned long long brent = 0;
+    neartie_points(m, cd, cl, ntq, *ntq_n, nts, o, next, next_n, brent, blockIdx.x * blockDim.x + threadIdx.x,
+                   gridDim.x * blockDim.x);
     if (cnt) wave_add(&cnt->brent, brent);
 }
 
@@ -1386,21 +1400,15 @@ __device__ inline void coop_gather(const MeshView& m, uint32_t* qbuf, double x, 
 // then applies them in rank order exactly as MillingCutter::dropCutter does
 // against the running z (a lifting facet skips the triangle's vertices and
 // edges; the cone rim is tried only when the tip did not lift).
-__global__ void __launch_bounds__(256) k_drop_coop_exact(MeshView m, CutterDev cd, const double* __restrict__ cl,
-                                                         const uint32_t* __restrict__ list,
-                                                         const uint32_t* __restrict__ count,
-                                                         const double* __restrict__ zfloor,
-                                                         const double* __restrict__ boxcap,
-                                                         const BoxOrigin* __restrict__ org, DropOut o,
-                                                         Counters* cnt) {
-    __shared__ uint32_t qmem[kCoopWaves][kCoopQueue];
+// w0: the wavefront's index in the grid, nwaves: the grid's wavefronts;
+// qbuf: the wavefront's kCoopQueue-entry LDS queue.
+__device__ __attribute__((always_inline)) inline void coop_exact_points(
+    const MeshView& m, const CutterDev& cd, const double* __restrict__ cl, const uint32_t* __restrict__ list,
+    uint32_t n_eff, const double* __restrict__ zfloor, const double* __restrict__ boxcap,
+    const BoxOrigin* __restrict__ org, const DropOut& o, uint32_t* qbuf, unsigned long long& brent, uint32_t w0,
+    uint32_t nwaves) {
     const uint32_t lane = lane_id();
-    const uint32_t wv = threadIdx.x >> 6;
-    uint32_t* qbuf = qmem[wv];
-    const uint32_t n_eff = *count;
-    const uint32_t nwaves = gridDim.x * kCoopWaves;
-    unsigned long long brent = 0;
-    for (uint32_t w = xcd_block(m.xcd_group) * kCoopWaves + wv; w < n_eff; w += nwaves) {
+    for (uint32_t w = w0; w < n_eff; w += nwaves) {
         const uint32_t src = uniform_u(list[w]);
         const double x = cl[3 * (size_t)src], y = cl[3 * (size_t)src + 1], z0 = cl[3 * (size_t)src + 2];
         const CutterP& c = cd.c;
@@ -1519,7 +1527,87 @@ __global__ void __launch_bounds__(256) k_drop_coop_exact(MeshView m, CutterDev c
             write_drop(o, m.order, src, L);
         }
     }
+}
+
+__global__ void __launch_bounds__(256) k_drop_coop_exact(MeshView m, CutterDev cd, const double* __restrict__ cl,
+                                                         const uint32_t* __restrict__ list,
+                                                         const uint32_t* __restrict__ count,
+                                                         const double* __restrict__ zfloor,
+                                                         const double* __restrict__ boxcap,
+                                                         const BoxOrigin* __restrict__ org, DropOut o,
+                                                         Counters* cnt) {
+    __shared__ uint32_t qmem[kCoopWaves][kCoopQueue];
+    const uint32_t wv = threadIdx.x >> 6;
+    unsigned long long brent = 0;
+    coop_exact_points(m, cd, cl, list, *count, zfloor, boxcap, org, o, qmem[wv], brent,
+                      xcd_block(m.xcd_group) * kCoopWaves + wv, gridDim.x * kCoopWaves);
     if (cnt) wave_add(&cnt->brent, brent);
+}
+
+// The passes after the selection for every fast job of a batch, one launch
+// each instead of one per job: the refine pass over the jobs' 64-point
+// chunks (chunk_off over the jobs in table order), then persistent near-tie
+// and exact passes whose wavefronts take every job's queue in turn (the
+// queue lengths are only known on the device).
+struct PostJob {
+    MeshView m;
+    CutterDev cd;
+    const double* cl;
+    const uint32_t* perm;
+    uint32_t n;
+    NearTieSets nts;
+    double etie;
+    DropOut o;
+    uint32_t* q1;      // exact-tier queue; its count is qcount[0]
+    uint32_t* qcount;  // queue counts: exact tier, (unused), near-tie
+    const double* zfloor;
+    const double* cap64;
+    const BoxOrigin* org;
+    Counters* cnt2;
+    Counters* cnt3;
+};
+
+__global__ void __launch_bounds__(256) k_drop_resolve_jobs(const PostJob* __restrict__ jobs,
+                                                           const uint32_t* __restrict__ chunk_off, uint32_t njobs) {
+    const uint32_t c = uniform_u(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (c >= chunk_off[njobs]) return;
+    uint32_t lo = 0, hi = njobs;  // the job: last j with chunk_off[j] <= c
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (chunk_off[mid] <= c) lo = mid;
+        else hi = mid;
+    }
+    const PostJob& J = jobs[lo];
+    resolve_points(J.m, J.cd, J.cl, J.perm, J.n, J.nts, J.etie, J.o, J.q1, J.qcount, J.cnt2,
+                   (c - chunk_off[lo]) * 64u + lane_id());
+}
+
+__global__ void __launch_bounds__(256) k_drop_neartie_jobs(const PostJob* __restrict__ jobs, uint32_t njobs) {
+    for (uint32_t j = 0; j < njobs; ++j) {
+        const PostJob& J = jobs[j];
+        unsigned long long brent = 0;
+        neartie_points(J.m, J.cd, J.cl, J.nts.q, J.qcount[2], J.nts, J.o, J.q1, J.qcount, brent,
+                       blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+        if (J.cnt2) wave_add(&J.cnt2->brent, brent);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_drop_coop_exact_jobs(const PostJob* __restrict__ jobs, uint32_t njobs) {
+    __shared__ uint32_t qmem[kCoopWaves][kCoopQueue];
+    const uint32_t wv = threadIdx.x >> 6;
+    const uint32_t nwaves = gridDim.x * kCoopWaves;
+    // consecutive jobs start on different wavefronts, so short queues of
+    // many jobs spread over the grid
+    uint32_t first = 0;
+    for (uint32_t j = 0; j < njobs; ++j) {
+        const PostJob& J = jobs[j];
+        const uint32_t nq = J.qcount[0];
+        const uint32_t w0 = (blockIdx.x * kCoopWaves + wv + nwaves - first % nwaves) % nwaves;
+        unsigned long long brent = 0;
+        coop_exact_points(J.m, J.cd, J.cl, J.q1, nq, J.zfloor, J.cap64, J.org, J.o, qmem[wv], brent, w0, nwaves);
+        if (J.cnt3) wave_add(&J.cnt3->brent, brent);
+        first += nq;
+    }
 }
 
 // FP64 selection tier for Cyl/Ball/Bull: per-triangle selection summaries
@@ -2673,8 +2761,8 @@ struct ocl_batch {
     int precision = OCL_PREC_MIXED;
     ocl_tune tune{};
     uint32_t total = 0, hb = 8, key_bits = 16;
-    ocl_amd::DevBuf keys, vals, scratch, keyjobs, zeros, seljobs, chunk_off, queue, exjobs, exoff;
-    uint32_t nzero = 0, nsel = 0, grid = 0, nexact = 0, exchunks = 0;
+    ocl_amd::DevBuf keys, vals, scratch, keyjobs, zeros, seljobs, chunk_off, queue, exjobs, exoff, postjobs, postoff;
+    uint32_t nzero = 0, nsel = 0, grid = 0, nexact = 0, exchunks = 0, npost = 0, postchunks = 0;
     uint32_t* sorted_vals = nullptr;
     std::vector<uint32_t> sel_order;  // job indices of the queue, largest first
     hipStream_t cap = nullptr;
@@ -2748,10 +2836,19 @@ static int batch_record(ocl_batch* b, hipStream_t st, bool timed) {
         HIP_TRY(hipGetLastError());
     }
     if (timed) HIP_TRY(hipEventRecord(b->s1, st));
-    for (const BatchJob& J : b->jobs) {
-        if (!J.fast) continue;
-        int rc = fast_post(J.F, st);
-        if (rc) return rc;
+    if (b->npost) {
+        // the passes after the selection, every job in one launch each
+        k_drop_resolve_jobs<<<(b->postchunks + 3) / 4, 256, 0, st>>>(b->postjobs.as<PostJob>(),
+                                                                      b->postoff.as<uint32_t>(), b->npost);
+        k_drop_neartie_jobs<<<kNearTieBlocks, 256, 0, st>>>(b->postjobs.as<PostJob>(), b->npost);
+        k_drop_coop_exact_jobs<<<kCoopBlocks, 256, 0, st>>>(b->postjobs.as<PostJob>(), b->npost);
+        HIP_TRY(hipGetLastError());
+    } else {
+        for (const BatchJob& J : b->jobs) {
+            if (!J.fast) continue;
+            int rc = fast_post(J.F, st);
+            if (rc) return rc;
+        }
     }
     if (timed) HIP_TRY(hipEventRecord(b->e1, st));
     return OCL_OK;
@@ -2871,6 +2968,27 @@ int ocl_hip_batch_create(const ocl_drop_job* jobs, size_t njobs, int precision, 
         if (!a.empty()) HIP_TRY(hipMemcpy(b->seljobs.p, a.data(), argsz * a.size(), hipMemcpyHostToDevice));
     }
     HIP_TRY(hipMemcpy(b->chunk_off.p, coff.data(), 4 * coff.size(), hipMemcpyHostToDevice));
+    if (precision != OCL_PREC_FP32 && env_scale("OCL_COOP_SELECT", 0.0) == 0.0) {
+        // the fast jobs' passes after the selection, in queue order
+        std::vector<PostJob> pj;
+        std::vector<uint32_t> poff(1, 0);
+        for (uint32_t j : sel) {
+            BatchJob& J = b->jobs[j];
+            const FastPlan& F = J.F;
+            ocl_mesh* m = F.m;
+            pj.push_back(PostJob{F.mv, F.cd, F.cl, F.perm, F.n, F.nts, F.etie, F.o, F.q1, F.qcount, F.zfloor,
+                                 m->rs64.cap.as<double>(), m->origins.as<BoxOrigin>(), m->cnt2.as<Counters>(),
+                                 m->cnt3.as<Counters>()});
+            poff.push_back(poff.back() + (F.n + 63) / 64);
+            m->exact_slot = 0;
+        }
+        b->npost = (uint32_t)pj.size();
+        b->postchunks = poff.back();
+        if ((rc = b->postjobs.ensure(sizeof(PostJob) * (pj.size() + 1))) || (rc = b->postoff.ensure(4 * poff.size())))
+            return rc;
+        if (!pj.empty()) HIP_TRY(hipMemcpy(b->postjobs.p, pj.data(), sizeof(PostJob) * pj.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b->postoff.p, poff.data(), 4 * poff.size(), hipMemcpyHostToDevice));
+    }
     {
         // exact-path jobs, largest first
         std::vector<uint32_t> ex;
