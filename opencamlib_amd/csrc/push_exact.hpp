@@ -406,9 +406,17 @@ OCL_HD double aell_brent(const AEll& el, double a, double b, double eps, double 
     return b;
 }
 
-// bullcutter.cpp:109-155 with AlignedEllipse::aligned_solver (ellipse.cpp:212-311)
-OCL_HD bool bull_general_edge_push(const CutterP& c, const FibX& f, PIval& i, V3 p1, V3 p2, int& iters) {
-    bool r = false;
+// bullcutter.cpp:109-155 with AlignedEllipse::aligned_solver (ellipse.cpp:212-311),
+// in three steps so that a wavefront can run the Brent solves of many
+// (fiber, triangle, edge) triples side by side:
+//   bull_edge_prep:    the offset ellipse of the edge in the plane
+//                      z = fiber z + r2 (false: no contact can reach the
+//                      edge segment, or a degenerate edge);
+//   bull_edge_bracket: the diamond-angle bracket of the two roots (false:
+//                      the error has no sign change, no contact);
+//   aell_brent twice:  d1 on (lolim, hilim), d2 on (hilim - 4, lolim);
+//   bull_edge_finish:  the two contacts and the interval updates.
+OCL_HD bool bull_edge_prep(const CutterP& c, const FibX& f, V3 p1, V3 p2, AEll& e) {
     if (near0(vxynorm(vsub(p2, p1)))) return false;
     if (near0(p2.z - p1.z)) return false;
     const double tplane = (f.p1.z + c.r2 - p1.z) / (p2.z - p1.z);
@@ -426,7 +434,6 @@ OCL_HD bool bull_general_edge_push(const CutterP& c, const FibX& f, PIval& i, V3
         const double margin = 1e-9 * (1.0 + fabs(tplane) + reach);
         if (tplane + reach < -margin || tplane - reach > 1.0 + margin) return false;
     }
-    AEll e;
     e.center = vadd(p1, vscale(tplane, vsub(p2, p1)));
     V3 major = vsub(p2, p1);
     major.z = 0;
@@ -439,6 +446,10 @@ OCL_HD bool bull_general_edge_push(const CutterP& c, const FibX& f, PIval& i, V3
     // aligned_solver
     e.edir = vperp(f.dir);
     e.target = f.p1;
+    return true;
+}
+
+OCL_HD bool bull_edge_bracket(const AEll& e, const FibX& f, double& lolim, double& hilim) {
     double t1 = 0.0;
     if (f.p1.y == f.p2.y)
         t1 = sqrt(sqr(e.b * e.minor.y) / (sqr(e.a * e.major.y) + sqr(e.b * e.minor.y)));
@@ -460,7 +471,8 @@ OCL_HD bool bull_general_edge_push(const CutterP& c, const FibX& f, PIval& i, V3
         }
     }
     if (!(fpos && fneg)) return false;
-    double lolim = 0.0, hilim = 0.0;
+    lolim = 0.0;
+    hilim = 0.0;
     if (adia > bdia) {
         lolim = bdia;
         hilim = adia;
@@ -468,8 +480,18 @@ OCL_HD bool bull_general_edge_push(const CutterP& c, const FibX& f, PIval& i, V3
         hilim = bdia;
         lolim = adia;
     }
-    const double d1 = aell_brent(e, lolim, hilim, 3E-16, 1e-10, iters);
-    const double d2 = aell_brent(e, hilim - 4.0, lolim, 3E-16, 1e-10, iters);
+    return true;
+}
+
+// root `which` (0: d1, 1: d2) of the bracket
+OCL_HD double bull_edge_solve(const AEll& e, double lolim, double hilim, int which, int& iters) {
+    return which == 0 ? aell_brent(e, lolim, hilim, 3E-16, 1e-10, iters)
+                      : aell_brent(e, hilim - 4.0, lolim, 3E-16, 1e-10, iters);
+}
+
+OCL_HD bool bull_edge_finish(const CutterP& c, const FibX& f, PIval& i, V3 p1, V3 p2, const AEll& e, double d1,
+                             double d2) {
+    bool r = false;
     double s_1, t_1, s_2, t_2;
     diangle_st(d1, s_1, t_1);
     diangle_st(d2, s_2, t_2);
@@ -480,6 +502,15 @@ OCL_HD bool bull_general_edge_push(const CutterP& c, const FibX& f, PIval& i, V3
     if (piv_update_if(i, fib_tval(f, cl), cc, CC_EDGE_POS, p1, p2, true)) r = true;
     if (piv_update_if(i, fib_tval(f, cl2), cc2, CC_EDGE_POS, p1, p2, true)) r = true;
     return r;
+}
+
+OCL_HD bool bull_general_edge_push(const CutterP& c, const FibX& f, PIval& i, V3 p1, V3 p2, int& iters) {
+    AEll e;
+    double lolim, hilim;
+    if (!bull_edge_prep(c, f, p1, p2, e) || !bull_edge_bracket(e, f, lolim, hilim)) return false;
+    const double d1 = bull_edge_solve(e, lolim, hilim, 0, iters);
+    const double d2 = bull_edge_solve(e, lolim, hilim, 1, iters);
+    return bull_edge_finish(c, f, i, p1, p2, e, d1, d2);
 }
 
 // conecutter.cpp:300-308

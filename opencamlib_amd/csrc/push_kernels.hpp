@@ -102,16 +102,28 @@ struct PushCounters {
 // (batchpushcutter.cpp:170-181).  The tree's boxes are then the KD
 // acceptance boxes, so the walk's overlap test visits every box holding such
 // a triangle.  superset = 0: only triangles whose boxes overlap the search box.
+//
+// BullCutter: a lane's pair needs up to six Brent solves (two per edge), of
+// very different lengths, so a batch runs in three steps: each lane lists the
+// solves its pair needs and their brackets; the wavefront runs the listed
+// solves 64 at a time, one per lane; each lane then replays its pair in the
+// reference's order (vertices, facet, edges) with the solved roots.  A batch
+// then costs the longest solve of each round of 64 instead of the longest
+// lane's six solves in a row.
 constexpr int kPushWaves = 4;  // wavefronts per 256-thread block
+constexpr int kBullSolves = 6; // Brent solves per pair: 3 edges x 2 roots
 
 template <int K>
 __global__ void __launch_bounds__(256) k_push_fibers(MeshView pm, const double* __restrict__ acc, int superset,
                                                      CutterDev cd, const double* __restrict__ fib,
                                                      const uint32_t* __restrict__ perm, uint32_t nf, int dir,
                                                      PushLists PL, PushCounters* cnt) {
-    __shared__ uint32_t qmem[kPushWaves][80];
+    constexpr int kS = (K == K_BULL) ? kBullSolves : 1;
+    __shared__ uint32_t qmem[kPushWaves][128];
     __shared__ PEntry emem[kPushWaves][64];
     __shared__ uint32_t fmem[kPushWaves][64];
+    __shared__ uint32_t tmem[kPushWaves][kS * 64];  // Bull: listed solves (lane << 3 | edge << 1 | root)
+    __shared__ double bmem[kPushWaves][kS][64];     // Bull: brackets (lo, hi per edge), then the roots
     const uint32_t lane = lane_id();
     const uint32_t wv = threadIdx.x >> 6;
     uint32_t* qbuf = qmem[wv];
@@ -136,11 +148,81 @@ __global__ void __launch_bounds__(256) k_push_fibers(MeshView pm, const double* 
             const uint32_t k = has ? qbuf[lane] : 0u;
             PIval iv;
             piv_init(iv);
-            if (has) {
-                const TriX t = load_tri(pm.tri, k);
-                int it = 0;
-                push_pair<K>(cd, f, t, iv, it);
-                brent += it;
+            if constexpr (K == K_BULL) {
+                CutterP c = cd.c;
+                c.kind = K_BULL;
+                TriX t;
+                if (has) t = load_tri(pm.tri, k);
+                // 1. the solves the pair needs (edges that are not horizontal,
+                //    can reach the segment and have a sign change)
+                uint32_t need = 0, ntask = 0;
+                for (int e = 0; e < 3; ++e) {
+                    bool nd = false;
+                    double lo = 0.0, hi = 0.0;
+                    if (has) {
+                        const V3 p1 = tri_vtx(t, e), p2 = tri_vtx(t, (e + 1) % 3);
+                        PIval scratch;
+                        piv_init(scratch);
+                        AEll el;
+                        nd = !horiz_edge_push(c, f, scratch, p1, p2) && bull_edge_prep(c, f, p1, p2, el) &&
+                             bull_edge_bracket(el, f, lo, hi);
+                    }
+                    bmem[wv][2 * e][lane] = lo;
+                    bmem[wv][2 * e + 1][lane] = hi;
+                    need |= nd ? (1u << e) : 0u;
+                    const unsigned long long bal = __ballot(nd);
+                    const uint32_t pos = ntask + 2u * (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+                    if (nd) {
+                        tmem[wv][pos] = (lane << 3) | ((uint32_t)e << 1);
+                        tmem[wv][pos + 1] = (lane << 3) | ((uint32_t)e << 1) | 1u;
+                    }
+                    ntask += 2u * (uint32_t)__popcll(bal);
+                }
+                wave_lds_sync();
+                // 2. the listed solves, one per lane; the roots replace the
+                //    brackets once every lane of the round has read its own
+                for (uint32_t base = 0; base < ntask; base += 64) {
+                    const bool tk = base + lane < ntask;
+                    const uint32_t td = tk ? tmem[wv][base + lane] : 0u;
+                    const uint32_t pl = td >> 3, e = (td >> 1) & 3u, which = td & 1u;
+                    double root = 0.0;
+                    if (tk) {
+                        const TriX tt = load_tri(pm.tri, qbuf[pl]);
+                        const V3 p1 = tri_vtx(tt, (int)e), p2 = tri_vtx(tt, ((int)e + 1) % 3);
+                        AEll el;
+                        bull_edge_prep(c, f, p1, p2, el);
+                        int it = 0;
+                        root = bull_edge_solve(el, bmem[wv][2 * e][pl], bmem[wv][2 * e + 1][pl], (int)which, it);
+                        brent += it;
+                    }
+                    wave_lds_sync();
+                    if (tk) bmem[wv][2 * e + which][pl] = root;
+                    wave_lds_sync();
+                }
+                // 3. the pair in the reference's order with the roots
+                //    (millingcutter.cpp:377-382, 277-301)
+                if (has) {
+                    base_vertex_push(c, f, iv, t);
+                    simple_facet_push(c, f, iv, t);
+                    for (int e = 0; e < 3; ++e) {
+                        const V3 p1 = tri_vtx(t, e), p2 = tri_vtx(t, (e + 1) % 3);
+                        if (horiz_edge_push(c, f, iv, p1, p2)) continue;
+                        shaft_edge_push(c, f, iv, p1, p2);
+                        if (need & (1u << e)) {
+                            AEll el;
+                            bull_edge_prep(c, f, p1, p2, el);
+                            bull_edge_finish(c, f, iv, p1, p2, el, bmem[wv][2 * e][lane], bmem[wv][2 * e + 1][lane]);
+                        }
+                    }
+                }
+                wave_lds_sync();
+            } else {
+                if (has) {
+                    const TriX t = load_tri(pm.tri, k);
+                    int it = 0;
+                    push_pair<K>(cd, f, t, iv, it);
+                    brent += it;
+                }
             }
             const bool keep = has && !piv_empty(iv);
             if (keep) ebuf[lane] = pentry_of(iv);
@@ -153,40 +235,47 @@ __global__ void __launch_bounds__(256) k_push_fibers(MeshView pm, const double* 
             wave_lds_sync();
             pairs += has ? 1 : 0;
         };
-        walk_boxes(pm.tree, qa0, qv0, qa1, qv1, [&](uint32_t b) {
-            const uint32_t k = b * kLeaf + lane;
-            bool cand = false;
-            if (lane < (uint32_t)kLeaf && k < pm.ntri) {
-                if (superset) {
-                    const double* q = acc + 4 * (size_t)k;  // needs max >= A and min <= B per axis
-                    cand = a1 >= q[0] && a0 <= q[1] && v1 >= q[2] && v0 <= q[3];
-                } else {
-                    const TriX t = load_tri(pm.tri, k);
-                    const double ta0 = fmin(fmin(t.p[0].x, t.p[1].x), t.p[2].x);
-                    const double ta1 = fmax(fmax(t.p[0].x, t.p[1].x), t.p[2].x);
-                    const double tb0 = fmin(fmin(t.p[0].y, t.p[1].y), t.p[2].y);
-                    const double tb1 = fmax(fmax(t.p[0].y, t.p[1].y), t.p[2].y);
-                    const double lo_a = (dir == 0) ? tb0 : ta0, hi_a = (dir == 0) ? tb1 : ta1;
-                    const double lo_z = fmin(fmin(t.p[0].z, t.p[1].z), t.p[2].z);
-                    cand = !(hi_a < a0 || lo_a > a1 || t.maxz < v0 || lo_z > v1);
+        // the walk pauses whenever 64 candidates are queued, so the batch
+        // code is emitted once
+        BoxWalk walk;
+        for (;;) {
+            walk_boxes_resumable(pm.tree, qa0, qv0, qa1, qv1, walk, [](uint32_t) {}, [&](uint32_t b, uint32_t) -> bool {
+                const uint32_t k = b * kLeaf + lane;
+                bool cand = false;
+                if (lane < (uint32_t)kLeaf && k < pm.ntri) {
+                    if (superset) {
+                        const double* q = acc + 4 * (size_t)k;  // needs max >= A and min <= B per axis
+                        cand = a1 >= q[0] && a0 <= q[1] && v1 >= q[2] && v0 <= q[3];
+                    } else {
+                        const TriX t = load_tri(pm.tri, k);
+                        const double ta0 = fmin(fmin(t.p[0].x, t.p[1].x), t.p[2].x);
+                        const double ta1 = fmax(fmax(t.p[0].x, t.p[1].x), t.p[2].x);
+                        const double tb0 = fmin(fmin(t.p[0].y, t.p[1].y), t.p[2].y);
+                        const double tb1 = fmax(fmax(t.p[0].y, t.p[1].y), t.p[2].y);
+                        const double lo_a = (dir == 0) ? tb0 : ta0, hi_a = (dir == 0) ? tb1 : ta1;
+                        const double lo_z = fmin(fmin(t.p[0].z, t.p[1].z), t.p[2].z);
+                        cand = !(hi_a < a0 || lo_a > a1 || t.maxz < v0 || lo_z > v1);
+                    }
                 }
-            }
-            const unsigned long long bal = __ballot(cand);
-            const uint32_t pos = qn + __popcll(bal & ((1ull << lane) - 1ull));
-            if (cand) qbuf[pos] = k;
-            qn += __popcll(bal);
-            wave_lds_sync();
-            if (qn >= 64) {
-                batch(64u);
-                const uint32_t rest = qn - 64;
+                const unsigned long long bal = __ballot(cand);
+                const uint32_t pos = qn + __popcll(bal & ((1ull << lane) - 1ull));
+                if (cand) qbuf[pos] = k;
+                qn += __popcll(bal);
+                wave_lds_sync();
+                return qn < 64;
+            });
+            if (qn >= 64 || (walk.done && qn > 0)) {
+                const uint32_t nb = qn < 64 ? qn : 64u;
+                batch(nb);
+                const uint32_t rest = qn - nb;
                 const uint32_t moved = (lane < rest) ? qbuf[64 + lane] : 0u;
                 wave_lds_sync();
                 if (lane < rest) qbuf[lane] = moved;
                 wave_lds_sync();
                 qn = rest;
             }
-        });
-        if (qn > 0) batch(qn);
+            if (walk.done && qn == 0) break;
+        }
         if (lane == 0) {
             PL.n[src] = n;
             if (ovf) atomicAdd(PL.overflow, 1u);
