@@ -25,3 +25,34 @@ def scaled(spec, d):
         s["r"] = min(s["r"], 0.4 * d)
     s["d"] = d
     return s
+
+
+def _cone(d, a, l=10.0):
+    return {"kind": "cone", "d": d, "angle": a, "L": l}
+
+
+def cyl_cone(d1, d2, a):
+    """CylConeCutter's sub-cutters (compositecutter.cpp:310-325): the cone
+    sits below the cylinder's plane (negative offset)."""
+    off = -(d1 / 2.0) / math.tan(a)
+    ch = (d2 / 2.0) / math.tan(a) + off
+    return {"kind": "composite", "L": ch + 10.0, "subs": [
+        {"cutter": {"kind": "cyl", "d": d1, "L": 1.0}, "r": d1 / 2.0, "h": 0.0, "zoff": 0.0},
+        {"cutter": _cone(d2, a), "r": d2 / 2.0, "h": ch, "zoff": off},
+        {"cutter": {"kind": "cyl", "d": d2, "L": 20.0}, "r": d2 / 2.0, "h": (d2 / 2.0) / math.tan(a) + 20.0,
+         "zoff": ch}]}
+
+
+def ball_cone(d1, d2, a):
+    """BallConeCutter's sub-cutters (compositecutter.cpp:328-347): the ball
+    meets the cone tangentially at r1 cos(a)."""
+    r1, r2 = d1 / 2.0, d2 / 2.0
+    rc = r1 * math.cos(a)
+    h1 = r1 - math.sqrt(r1 * r1 - rc * rc)
+    off = -(rc / math.tan(a) - h1)
+    h2 = r2 / math.tan(a) + off
+    return {"kind": "composite", "L": 30.0, "subs": [
+        {"cutter": {"kind": "ball", "d": d1, "L": 1.0}, "r": rc, "h": h1, "zoff": 0.0},
+        {"cutter": _cone(d2, a), "r": r2, "h": h2, "zoff": off},
+        {"cutter": {"kind": "cyl", "d": d2, "L": 20.0}, "r": r2, "h": h2 + 20.0, "zoff": h2}]}
+

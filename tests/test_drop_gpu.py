@@ -15,7 +15,7 @@ import math
 import numpy as np
 import pytest
 
-from cutters import ALL, BALL10, BULL10
+from cutters import ALL, BALL10, BULL10, ball_cone, cyl_cone
 from opencamlib_amd import capi, workloads as W
 from oracle import oracle as O
 
@@ -95,6 +95,26 @@ def test_config0_all_cutters(gpu, c0, name, mode):
     got = mesh.drop(spec, cl, mode)
     mesh.close()
     compare(tris, spec, cl, got, ref, mode, "C0")
+
+
+@pytest.mark.parametrize("mode", ["exact64", "mixed"])
+@pytest.mark.parametrize("name", ["cylcone", "ballcone"])
+def test_composites_with_cone_parts(gpu, name, mode):
+    """Composites whose cone part sits below the first cutter (negative
+    offsets: contacts can rise above a triangle's max z): the exact path's
+    running-height skip must leave every result bit-equal to the oracle."""
+    spec = cyl_cone(2.0, 10.0, math.pi / 4) if name == "cylcone" else ball_cone(4.0, 10.0, math.pi / 4)
+    tris = W.heightfield(32, 50.0, bumps=3, steps=3, seed=23)
+    cl = W.cl_grid(70, 70, 0.7, z0=-10.0, origin=(0.3, 0.4))
+    ref = oracle_drop(tris, spec, cl)
+    mesh = capi.DeviceMesh(tris)
+    got = mesh.drop(spec, cl, mode)
+    mesh.close()
+    assert np.sum(ref["type"] != 0) > len(cl) // 2
+    assert np.array_equal(got["z"], ref["z"])
+    assert np.array_equal(got["type"], ref["type"])
+    assert np.array_equal(got["tri"], ref["tri"])
+    assert np.array_equal(got["cc"], ref["cc"])
 
 
 @pytest.fixture(scope="module")
