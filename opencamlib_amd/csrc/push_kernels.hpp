@@ -110,25 +110,33 @@ struct PushCounters {
 // reference's order (vertices, facet, edges) with the solved roots.  A batch
 // then costs the longest solve of each round of 64 instead of the longest
 // lane's six solves in a row.
+//
+// Merging: a batch's non-empty intervals are first merged among themselves
+// in parallel (sort by lower end, components by a prefix max), so lane 0
+// inserts a few components rather than every interval; one by one, the
+// merges took 65% of the ConeCutter push.  The fiber's list lives in LDS
+// while it is built and is copied out at the end, when the list capacity
+// fits (kLdsList entries); larger capacities, after an overflow retry,
+// merge in global memory.
 constexpr int kPushWaves = 4;  // wavefronts per 256-thread block
 constexpr int kBullSolves = 6; // Brent solves per pair: 3 edges x 2 roots
+constexpr uint32_t kLdsList = 32;
 
 template <int K>
-__global__ void __launch_bounds__(256) k_push_fibers(MeshView pm, const double* __restrict__ acc, int superset,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_push_fibers(MeshView pm, const double* __restrict__ acc, int superset,
                                                      CutterDev cd, const double* __restrict__ fib,
                                                      const uint32_t* __restrict__ perm, uint32_t nf, int dir,
                                                      PushLists PL, PushCounters* cnt) {
     constexpr int kS = (K == K_BULL) ? kBullSolves : 1;
     __shared__ uint32_t qmem[kPushWaves][128];
     __shared__ PEntry emem[kPushWaves][64];
-    __shared__ uint32_t fmem[kPushWaves][64];
     __shared__ uint32_t tmem[kPushWaves][kS * 64];  // Bull: listed solves (lane << 3 | edge << 1 | root)
     __shared__ double bmem[kPushWaves][kS][64];     // Bull: brackets (lo, hi per edge), then the roots
+    __shared__ PEntry lmem[kPushWaves][kLdsList];   // the fiber's interval list
     const uint32_t lane = lane_id();
     const uint32_t wv = threadIdx.x >> 6;
     uint32_t* qbuf = qmem[wv];
     PEntry* ebuf = emem[wv];
-    uint32_t* fbuf = fmem[wv];
     const double R = cd.c.radius, L = cd.c.length;
     unsigned long long pairs = 0, brent = 0, nint = 0;
     for (uint32_t w = blockIdx.x * kPushWaves + wv; w < nf; w += gridDim.x * kPushWaves) {
@@ -140,6 +148,8 @@ __global__ void __launch_bounds__(256) k_push_fibers(MeshView pm, const double* 
         const float qa0 = __double2float_rd(a0), qa1 = __double2float_ru(a1);
         const float qv0 = __double2float_rd(v0), qv1 = __double2float_ru(v1);
         PEntry* E = PL.e + (size_t)src * PL.cap;
+        const bool in_lds = PL.cap <= kLdsList;
+        PEntry* EL = in_lds ? lmem[wv] : E;  // where the list is built
         uint32_t n = 0;  // list size (kept by lane 0, broadcast when needed)
         bool ovf = false;
         uint32_t qn = 0;
@@ -226,11 +236,59 @@ __global__ void __launch_bounds__(256) k_push_fibers(MeshView pm, const double* 
             }
             const bool keep = has && !piv_empty(iv);
             if (keep) ebuf[lane] = pentry_of(iv);
-            fbuf[lane] = keep ? 1u : 0u;
             wave_lds_sync();
-            if (lane == 0) {
-                for (uint32_t i = 0; i < nb; ++i)
-                    if (fbuf[i] && !ovf && !plist_insert(E, n, PL.cap, ebuf[i])) ovf = true;
+            // The batch's intervals merge first among themselves: sorted by
+            // (lower end, rank), a connected component starts where the lower
+            // end exceeds every upper end before it (touching ends merge, as
+            // Interval::outside is strict).  A component's lower end and CC
+            // are its first member's (the lowest, earliest on ties), its upper
+            // end and CC those of its highest member, earliest on ties: what
+            // inserting the members one by one in rank order would keep, since
+            // an existing entry keeps its extremes on ties.  Lane 0 then
+            // inserts the components (disjoint, so their order is free).
+            double key = keep ? iv.lo : INFINITY;
+            uint32_t idx = lane;
+            for (uint32_t kk = 2; kk <= 64; kk <<= 1) {
+                for (uint32_t jj = kk >> 1; jj > 0; jj >>= 1) {
+                    const double ok = __shfl_xor(key, (int)jj, 64);
+                    const uint32_t oi = (uint32_t)__shfl_xor((int)idx, (int)jj, 64);
+                    const bool up = (lane & kk) == 0, low = (lane & jj) == 0;
+                    const bool less = (ok < key) || (ok == key && oi < idx);
+                    const bool take = (up == low) ? less : !less;
+                    key = take ? ok : key;
+                    idx = take ? oi : idx;
+                }
+            }
+            const bool valid = key < INFINITY;
+            const double hi_s = __shfl(keep ? iv.hi : -INFINITY, (int)idx, 64);
+            // upper ends before each position: an exclusive prefix max
+            double pm = valid ? hi_s : -INFINITY;
+            for (int off = 1; off < 64; off <<= 1) {
+                const double o = __shfl_up(pm, off, 64);
+                if ((int)lane >= off) pm = fmax(pm, o);
+            }
+            double before = __shfl_up(pm, 1, 64);
+            if (lane == 0) before = -INFINITY;
+            const bool start = valid && key > before;
+            const unsigned long long sb = __ballot(start);
+            const uint32_t comp = (uint32_t)__popcll(sb & ((2ull << lane) - 1ull)) - 1u;
+            const uint32_t ncomp = (uint32_t)__popcll(sb);
+            unsigned long long rest = sb;
+            for (uint32_t c = 0; c < ncomp; ++c) {
+                const uint32_t first = (uint32_t)__ffsll((long long)rest) - 1u;
+                rest &= rest - 1;
+                const bool in_c = valid && comp == c;
+                const double mx = wave_max(in_c ? hi_s : -INFINITY);
+                const uint32_t top = (uint32_t)wave_min_t<double>(in_c && hi_s == mx ? (double)idx : 64.0);
+                const uint32_t lead = uniform_u((uint32_t)__shfl((int)idx, (int)first, 64));
+                if (lane == 0 && !ovf) {
+                    PEntry m = ebuf[lead];
+                    const PEntry& u = ebuf[top];
+                    m.hi = u.hi;
+                    m.ucc[0] = u.ucc[0]; m.ucc[1] = u.ucc[1]; m.ucc[2] = u.ucc[2];
+                    m.uty = u.uty;
+                    if (!plist_insert(EL, n, PL.cap, m)) ovf = true;
+                }
             }
             wave_lds_sync();
             pairs += has ? 1 : 0;
@@ -275,6 +333,13 @@ __global__ void __launch_bounds__(256) k_push_fibers(MeshView pm, const double* 
                 qn = rest;
             }
             if (walk.done && qn == 0) break;
+        }
+        if (in_lds) {
+            // copy the list out (n is lane 0's)
+            const uint32_t nn = uniform_u(__shfl(n, 0, 64));
+            wave_lds_sync();
+            for (uint32_t j = lane; j < nn; j += 64) E[j] = lmem[wv][j];
+            wave_lds_sync();
         }
         if (lane == 0) {
             PL.n[src] = n;
