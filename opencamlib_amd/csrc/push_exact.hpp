@@ -406,6 +406,89 @@ OCL_HD double aell_brent(const AEll& el, double a, double b, double eps, double 
     return b;
 }
 
+// aell_brent as a state machine that asks for one error evaluation per
+// step, so that a wavefront can keep every lane on some solve: a lane whose
+// solve ends takes the next one while the others iterate.  The arithmetic
+// is aell_brent's, in the same order, so the roots are the same.
+struct BrentState {
+    double a, b, c, d, e, fa, fb, fc;
+    int it, phase;  // phase 0: f(a) pending, 1: f(b) pending, 2: f(b) of a step pending
+};
+OCL_HD void brent_start(BrentState& B, double a, double b) {
+    B.a = a;
+    B.b = b;
+    B.it = 0;
+    B.phase = 0;
+}
+// the point whose error the machine needs next
+OCL_HD double brent_x(const BrentState& B) { return B.phase == 0 ? B.a : B.b; }
+// consumes f(brent_x); true when the solve has ended (root B.b, B.it steps)
+OCL_HD bool brent_step(BrentState& B, double fx, double eps, double tolv) {
+    if (B.phase == 0) {
+        B.fa = fx;
+        B.phase = 1;
+        return false;
+    }
+    B.fb = fx;
+    if (B.phase == 1) {
+        B.c = B.a;
+        B.fc = B.fa;
+        B.e = B.b - B.a;
+        B.d = B.e;
+    } else if (signbit(B.fb) == signbit(B.fc)) {
+        B.c = B.a;
+        B.fc = B.fa;
+        B.e = B.b - B.a;
+        B.d = B.e;
+    }
+    B.phase = 2;
+    if (B.it >= 200) return true;
+    ++B.it;
+    if (fabs(B.fc) < fabs(B.fb)) {
+        B.a = B.b;
+        B.b = B.c;
+        B.c = B.a;
+        B.fa = B.fb;
+        B.fb = B.fc;
+        B.fc = B.fa;
+    }
+    const double tol = 2.0 * eps * fabs(B.b) + tolv;
+    const double m = 0.5 * (B.c - B.b);
+    if ((fabs(m) <= tol) || (B.fb == 0.0)) return true;
+    if ((fabs(B.e) < tol) || (fabs(B.fa) <= fabs(B.fb))) {
+        B.e = m;
+        B.d = B.e;
+    } else {
+        double p, q, r;
+        double s = B.fb / B.fa;
+        if (B.a == B.c) {
+            p = 2.0 * m * s;
+            q = 1.0 - s;
+        } else {
+            q = B.fa / B.fc;
+            r = B.fb / B.fc;
+            p = s * (2.0 * m * B.a * (q - r) - (B.b - B.a) * (r - 1.0));
+            q = (q - 1.0) * (r - 1.0) * (s - 1.0);
+        }
+        if (p > 0.0) q = -q;
+        else p = -p;
+        s = B.e;
+        B.e = B.d;
+        if ((2.0 * p < (3.0 * m * q - fabs(tol * q))) && (p < fabs(0.5 * s * q))) {
+            B.d = p / q;
+        } else {
+            B.e = m;
+            B.d = B.e;
+        }
+    }
+    B.a = B.b;
+    B.fa = B.fb;
+    if (fabs(B.d) > tol) B.b = B.b + B.d;
+    else if (m > 0.0) B.b = B.b + tol;
+    else B.b = B.b - tol;
+    return false;
+}
+
 // bullcutter.cpp:109-155 with AlignedEllipse::aligned_solver (ellipse.cpp:212-311),
 // in three steps so that a wavefront can run the Brent solves of many
 // (fiber, triangle, edge) triples side by side:

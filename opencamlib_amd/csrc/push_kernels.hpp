@@ -131,7 +131,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     __shared__ uint32_t qmem[kPushWaves][128];
     __shared__ PEntry emem[kPushWaves][64];
     __shared__ uint32_t tmem[kPushWaves][kS * 64];  // Bull: listed solves (lane << 3 | edge << 1 | root)
-    __shared__ double bmem[kPushWaves][kS][64];     // Bull: brackets (lo, hi per edge), then the roots
+    __shared__ double bmem[kPushWaves][kS][64];     // Bull: brackets (lo, hi per edge)
+    __shared__ double rmem[kPushWaves][kS][64];     // Bull: roots (d1, d2 per edge)
     __shared__ PEntry lmem[kPushWaves][kLdsList];   // the fiber's interval list
     const uint32_t lane = lane_id();
     const uint32_t wv = threadIdx.x >> 6;
@@ -189,26 +190,40 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                     ntask += 2u * (uint32_t)__popcll(bal);
                 }
                 wave_lds_sync();
-                // 2. the listed solves, one per lane; the roots replace the
-                //    brackets once every lane of the round has read its own
-                for (uint32_t base = 0; base < ntask; base += 64) {
-                    const bool tk = base + lane < ntask;
-                    const uint32_t td = tk ? tmem[wv][base + lane] : 0u;
-                    const uint32_t pl = td >> 3, e = (td >> 1) & 3u, which = td & 1u;
-                    double root = 0.0;
-                    if (tk) {
-                        const TriX tt = load_tri(pm.tri, qbuf[pl]);
-                        const V3 p1 = tri_vtx(tt, (int)e), p2 = tri_vtx(tt, ((int)e + 1) % 3);
-                        AEll el;
-                        bull_edge_prep(c, f, p1, p2, el);
-                        int it = 0;
-                        root = bull_edge_solve(el, bmem[wv][2 * e][pl], bmem[wv][2 * e + 1][pl], (int)which, it);
-                        brent += it;
+                // 2. the listed solves: every lane keeps one going and takes
+                //    the next listed solve when its own ends, one error
+                //    evaluation per step (brent_step)
+                {
+                    uint32_t cursor = 0;  // next solve to hand out
+                    bool busy = false;
+                    uint32_t td = 0;
+                    BrentState B;
+                    AEll el;
+                    for (;;) {
+                        const bool want = !busy;
+                        const unsigned long long wb = __ballot(want);
+                        const uint32_t tk = cursor + (uint32_t)__popcll(wb & ((1ull << lane) - 1ull));
+                        if (want && tk < ntask) {
+                            td = tmem[wv][tk];
+                            const uint32_t pl = td >> 3, e = (td >> 1) & 3u;
+                            const TriX tt = load_tri(pm.tri, qbuf[pl]);
+                            bull_edge_prep(c, f, tri_vtx(tt, (int)e), tri_vtx(tt, ((int)e + 1) % 3), el);
+                            const double lo = bmem[wv][2 * e][pl], hi = bmem[wv][2 * e + 1][pl];
+                            if (td & 1u) brent_start(B, hi - 4.0, lo);
+                            else brent_start(B, lo, hi);
+                            busy = true;
+                        }
+                        cursor += (uint32_t)__popcll(wb);
+                        if (!__any(busy)) break;
+                        if (busy && brent_step(B, aell_error(el, brent_x(B)), 3E-16, 1e-10)) {
+                            const uint32_t pl = td >> 3, e = (td >> 1) & 3u;
+                            rmem[wv][2 * e + (td & 1u)][pl] = B.b;
+                            brent += B.it;
+                            busy = false;
+                        }
                     }
-                    wave_lds_sync();
-                    if (tk) bmem[wv][2 * e + which][pl] = root;
-                    wave_lds_sync();
                 }
+                wave_lds_sync();
                 // 3. the pair in the reference's order with the roots
                 //    (millingcutter.cpp:377-382, 277-301)
                 if (has) {
@@ -221,7 +236,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                         if (need & (1u << e)) {
                             AEll el;
                             bull_edge_prep(c, f, p1, p2, el);
-                            bull_edge_finish(c, f, iv, p1, p2, el, bmem[wv][2 * e][lane], bmem[wv][2 * e + 1][lane]);
+                            bull_edge_finish(c, f, iv, p1, p2, el, rmem[wv][2 * e][lane], rmem[wv][2 * e + 1][lane]);
                         }
                     }
                 }
