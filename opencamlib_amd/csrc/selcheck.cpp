@@ -14,6 +14,7 @@
 #include "drop_exact.hpp"
 #include "drop_fast.hpp"
 #include "kd_rank.hpp"
+#include "push_exact.hpp"
 
 using namespace ocl_amd;
 
@@ -321,3 +322,38 @@ extern "C" int ocl_selcheck_calibrate(const double* tris, size_t ntri, const ocl
     }
     return OCL_OK;
 }
+
+// The push kernel's step-wise Brent (brent_step) against aell_brent on the
+// BullCutter edge ellipses of n (fiber, edge) cases.  Input per case (14
+// doubles): fiber p1, p2; edge p1, p2; diameter, corner radius.  Output per
+// case (9 doubles): needs-solve flag, then for each root (d1, d2): the root
+// and step count of aell_brent, then of the state machine.
+extern "C" int ocl_selcheck_brent(const double* in, size_t n, double* out) {
+    for (size_t i = 0; i < n; ++i) {
+        const double* q = in + 14 * i;
+        double* o = out + 9 * i;
+        for (int k = 0; k < 9; ++k) o[k] = 0.0;
+        const CutterP c = CutterP::bull(q[12], q[13], 20.0);
+        const FibX f = make_fiber(q);
+        const V3 p1 = v3(q[6], q[7], q[8]), p2 = v3(q[9], q[10], q[11]);
+        AEll e;
+        double lo, hi;
+        if (!bull_edge_prep(c, f, p1, p2, e) || !bull_edge_bracket(e, f, lo, hi)) continue;
+        o[0] = 1.0;
+        for (int which = 0; which < 2; ++which) {
+            int it = 0;
+            const double r = bull_edge_solve(e, lo, hi, which, it);
+            BrentState B;
+            if (which == 0) brent_start(B, lo, hi);
+            else brent_start(B, hi - 4.0, lo);
+            while (!brent_step(B, aell_error(e, brent_x(B)), 3E-16, 1e-10)) {
+            }
+            o[1 + 4 * which] = r;
+            o[2 + 4 * which] = it;
+            o[3 + 4 * which] = B.b;
+            o[4 + 4 * which] = B.it;
+        }
+    }
+    return 0;
+}
+

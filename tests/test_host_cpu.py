@@ -141,3 +141,40 @@ def test_selection_decides_like_oracle(scene, name, use_float):
         alt = O.drop_triangle(cut, tris[got["tri"][i]], cl[i, 0], cl[i, 1], ref["z"][i] - 1e-7)
         assert alt["type"] == ref["type"][i]
         assert abs(alt["z"] - ref["z"][i]) < 1e-9
+
+
+def test_stepwise_brent_matches_the_solver_bit_for_bit():
+    """The push kernel pools its BullCutter Brent solves with a step-wise
+    state machine; on random fiber/edge geometry it must return the same
+    roots, bit for bit, after the same number of steps as the solver it
+    replaces (brent_zero.hpp:36-123)."""
+    import ctypes as C
+    path = os.path.join(ROOT, "opencamlib_amd", "lib", "libocl_selcheck.so")
+    L = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    L.ocl_selcheck_brent.argtypes = [dp, C.c_size_t, dp]
+    rng = np.random.default_rng(31)
+    n = 4000
+    q = np.zeros((n, 14))
+    z = rng.uniform(0.0, 5.0, n)
+    across = rng.uniform(-5.0, 5.0, n)
+    xdir = rng.random(n) < 0.5
+    # X fibers (y = across) and Y fibers (x = across) through the edge region
+    q[:, 0] = np.where(xdir, -20.0, across)
+    q[:, 1] = np.where(xdir, across, -20.0)
+    q[:, 3] = np.where(xdir, 20.0, across)
+    q[:, 4] = np.where(xdir, across, 20.0)
+    q[:, 2] = q[:, 5] = z
+    q[:, 6:9] = rng.uniform(-6.0, 6.0, (n, 3))
+    q[:, 9:12] = q[:, 6:9] + rng.normal(0.0, 3.0, (n, 3))
+    q[:, 12] = rng.uniform(4.0, 12.0, n)
+    q[:, 13] = q[:, 12] * rng.uniform(0.1, 0.45, n)
+    out = np.zeros((n, 9))
+    L.ocl_selcheck_brent(q.ctypes.data_as(dp), n, out.ctypes.data_as(dp))
+    solved = out[:, 0] == 1.0
+    assert solved.sum() > n // 10
+    for w in (0, 1):
+        ref, ref_it = out[solved, 1 + 4 * w], out[solved, 2 + 4 * w]
+        got, got_it = out[solved, 3 + 4 * w], out[solved, 4 + 4 * w]
+        assert np.array_equal(ref, got)
+        assert np.array_equal(ref_it, got_it)
