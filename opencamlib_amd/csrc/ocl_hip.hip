@@ -105,6 +105,7 @@ struct ocl_mesh {
     ocl_amd::DevBuf esc_list, esc_list2, esc_list3, esc_flag, blocksum, qcount, cnt2, cnt3;
     ocl_amd::DevBuf origins;     // frame (BoxOrigin) per level-0 box
     ocl_amd::DevBuf boxzmax;     // highest vertex per level-0 box
+    ocl_amd::DevBuf gr_part;     // backward: per-block partial sums of dz/dr
     ocl_amd::DevBuf zfloor;      // per CL point: floor of the final z handed to the escalation passes
     ocl_amd::DevBuf ntset, ntcnt, zband, selrec;  // per CL point: selection records and near-tie sets
     int exact_slot = 0;          // queue-count slot read by the exact pass in the last call
@@ -2573,13 +2574,17 @@ namespace ocl_amd {
 // One lane per CL point: derivatives of its dropped z from the recorded
 // active contact (drop_grad.hpp), the per-vertex sums accumulated with FP64
 // atomics (so their summation order, and the last bits, vary run to run).
+// dz/dr: each block writes its partial sum to gr_part[block]; k_sum_into
+// adds them up (one same-address atomic per wavefront serialised ~15 k
+// atomics per million points).
 template <class T>
 __global__ void __launch_bounds__(256) k_drop_grad(MeshView m, const uint32_t* __restrict__ rank_of, CutterDev cd,
                                                    const double* __restrict__ cl, const double* __restrict__ cc,
                                                    const uint8_t* __restrict__ type, const uint32_t* __restrict__ tri,
                                                    const uint8_t* __restrict__ feat, const double* __restrict__ gz,
                                                    uint32_t n, double* __restrict__ dz, double* __restrict__ gtri,
-                                                   double* __restrict__ gr_sum) {
+                                                   double* __restrict__ gr_part) {
+    __shared__ double wsum[4];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double acc_r = 0.0;
     if (i < n) {
@@ -2617,10 +2622,26 @@ __global__ void __launch_bounds__(256) k_drop_grad(MeshView m, const uint32_t* _
             dz[3 * (size_t)i + 2] = o.gr;
         }
     }
-    if (gr_sum) {
+    if (gr_part) {
         for (int off = 32; off > 0; off >>= 1) acc_r += __shfl_xor(acc_r, off, 64);
-        if (lane_id() == 0) atomicAdd(gr_sum, acc_r);
+        if (lane_id() == 0) wsum[threadIdx.x >> 6] = acc_r;
+        __syncthreads();
+        if (threadIdx.x == 0) gr_part[blockIdx.x] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
     }
+}
+
+// *out += the sum of part[0..n), one 256-thread block, fixed order.
+__global__ void __launch_bounds__(256) k_sum_into(const double* __restrict__ part, uint32_t n, double* out) {
+    __shared__ double s[256];
+    double a = 0.0;
+    for (uint32_t j = threadIdx.x; j < n; j += 256) a += part[j];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out += s[0];
 }
 
 }  // namespace ocl_amd
@@ -2648,13 +2669,22 @@ extern "C" int ocl_hip_drop_backward(ocl_mesh* m, const ocl_cutter_desc* desc, c
         if (d_dz) HIP_TRY(hipMemsetAsync(d_dz, 0, 24 * n, st));
         return OCL_OK;
     }
+    double* part = nullptr;
+    if (d_grad_r) {
+        if ((rc = m->gr_part.ensure(8ull * nb + 64))) return rc;
+        part = m->gr_part.as<double>();
+    }
     if (precision == OCL_PREC_FP32 || precision == OCL_PREC_MIXED)
         k_drop_grad<float><<<nb, 256, 0, st>>>(mv, rk, cd, d_cl, d_cc, d_type, d_tri, d_feat, d_grad_z, n32, d_dz,
-                                               d_grad_tri, d_grad_r);
+                                               d_grad_tri, part);
     else
         k_drop_grad<double><<<nb, 256, 0, st>>>(mv, rk, cd, d_cl, d_cc, d_type, d_tri, d_feat, d_grad_z, n32, d_dz,
-                                                d_grad_tri, d_grad_r);
+                                                d_grad_tri, part);
     HIP_TRY(hipGetLastError());
+    if (d_grad_r) {
+        k_sum_into<<<1, 256, 0, st>>>(part, nb, d_grad_r);
+        HIP_TRY(hipGetLastError());
+    }
     return OCL_OK;
 }
 
