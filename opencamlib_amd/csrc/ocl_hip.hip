@@ -2303,7 +2303,9 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
         else
             k_drop_none<<<nb, 256, 0, st>>>(d_cl, n32, o);
     } else {
-        rc = launch_drop_fast(m, mv, cd, d_cl, n32, precision, o, cnt, st, stats != nullptr);
+        // the selection's pair/escalation counters feed only the statistics:
+        // without them its wavefronts skip three same-address atomics each
+        rc = launch_drop_fast(m, mv, cd, d_cl, n32, precision, o, stats ? cnt : nullptr, st, stats != nullptr);
         fast = true;
         if (rc) return rc;
     }
