@@ -42,8 +42,7 @@ def _run(cmd, cwd=None):
 def build_hip():
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libocl_hip.so")
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", out, os.path.join(CSRC, "ocl_hip.hip"),
-          os.path.join(CSRC, "cl_sort.hip")])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", out, os.path.join(CSRC, "ocl_hip.hip")])
     return out
 
 

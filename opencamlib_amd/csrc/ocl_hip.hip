@@ -19,6 +19,7 @@
 #include "drop_grad.hpp"
 #include "kd_rank.hpp"
 #include "mesh.hpp"
+#include "radix.hpp"
 #include "walk.hpp"
 
 namespace ocl_amd {
@@ -101,7 +102,7 @@ struct ocl_mesh {
     size_t box_off[ocl_amd::kMaxLevels] = {};
     int top = 0;
     // per-call workspace
-    ocl_amd::DevBuf keys, perm, hist, cursor, cl, out_z, out_cc, out_ty, out_tri, out_ft, out_dz, counters, fastrec;
+    ocl_amd::DevBuf keys, hist, cursor, cl, out_z, out_cc, out_ty, out_tri, out_ft, out_dz, counters, fastrec;
     ocl_amd::DevBuf esc_list, esc_list2, esc_list3, esc_flag, blocksum, qcount, cnt2, cnt3;
     ocl_amd::DevBuf origins;     // frame (BoxOrigin) per level-0 box
     ocl_amd::DevBuf boxzmax;     // highest vertex per level-0 box
@@ -284,7 +285,7 @@ struct ZeroSet {
 };
 
 __global__ void k_cl_keys(const double* __restrict__ cl, uint32_t n, BinFrame f, uint32_t hb,
-                          uint32_t* __restrict__ keys, ZeroSet zs) {
+                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, ZeroSet zs) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x == 0) {
         for (int r = 0; r < 4; ++r)
@@ -296,10 +297,9 @@ __global__ void k_cl_keys(const double* __restrict__ cl, uint32_t n, BinFrame f,
     // NaN coordinates land in cell 0 (fmax/fmin drop the NaN)
     const uint32_t ix = (uint32_t)fmin(fmax(fx, 0.0), cmax), iy = (uint32_t)fmin(fmax(fy, 0.0), cmax);
     keys[i] = hilbert_key(ix, iy, hb);
+    vals[i] = i;
 }
 
-hipError_t cl_sort_by_key(void* temp, size_t* temp_bytes, const uint32_t* keys_in, uint32_t* keys_out,
-                          uint32_t* perm, uint32_t n, uint32_t key_bits, hipStream_t st);
 
 
 // ------------------------------------------------------------ drop outputs
@@ -2019,10 +2019,11 @@ static int fast_post(const FastPlan& F, hipStream_t st) {
     return OCL_OK;
 }
 
-static int launch_drop_fast(ocl_mesh* m, const MeshView& mv, const CutterDev& cd, const double* d_cl, uint32_t n,
-                            int precision, const DropOut& o, Counters* cnt, hipStream_t st, bool timed) {
+static int launch_drop_fast(ocl_mesh* m, const MeshView& mv, const CutterDev& cd, const double* d_cl,
+                            const uint32_t* perm, uint32_t n, int precision, const DropOut& o, Counters* cnt,
+                            hipStream_t st, bool timed) {
     FastPlan F;
-    int rc = fast_prepare(m, mv, cd, d_cl, m->perm.as<uint32_t>(), n, precision, o, cnt, st, F);
+    int rc = fast_prepare(m, mv, cd, d_cl, perm, n, precision, o, cnt, st, F);
     if (rc) return rc;
     if (timed) HIP_TRY(hipEventRecord(m->ev_s0, st));
     const uint32_t nb = xcd_grid((n + 255) / 256, mv.xcd_group);
@@ -2271,7 +2272,7 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
                      {(uint32_t)(sizeof(Counters) / 4), (uint32_t)(sizeof(Counters) / 4),
                       (uint32_t)(sizeof(Counters) / 4), 4u}};
     // bin CL points
-    if ((rc = m->keys.ensure(8ull * n)) || (rc = m->perm.ensure(4ull * n))) return rc;
+    if ((rc = m->keys.ensure(8ull * n)) || (rc = m->hist.ensure(4ull * rs_scratch_words((uint32_t)n)))) return rc;
     const double R = cd.c.radius;
     BinFrame f;
     f.x0 = m->bbox[0] - R;
@@ -2282,12 +2283,12 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
     f.sy = (double)(1u << hb) / ey;
     const uint32_t nb = (n32 + 255) / 256;
     uint32_t* keys = m->keys.as<uint32_t>();
-    k_cl_keys<<<nb, 256, 0, st>>>(d_cl, n32, f, hb, keys, zs);
-    size_t tmp_bytes = 0;
-    HIP_TRY(cl_sort_by_key(nullptr, &tmp_bytes, keys, keys + n32, m->perm.as<uint32_t>(), n32, 2 * hb, st));
-    if ((rc = m->hist.ensure(tmp_bytes + 16))) return rc;
-    tmp_bytes = m->hist.bytes;
-    HIP_TRY(cl_sort_by_key(m->hist.p, &tmp_bytes, keys, keys + n32, m->perm.as<uint32_t>(), n32, 2 * hb, st));
+    k_cl_keys<<<nb, 256, 0, st>>>(d_cl, n32, f, hb, keys, keys + n32, zs);
+    // stable: equal keys keep their input order, so the point -> wavefront
+    // assignment, and with it every result, is the same on every run
+    uint32_t *ko = nullptr, *perm = nullptr;
+    rs_sort(keys, keys + n32, n32, 2 * hb, m->hist.as<uint32_t>(), &ko, &perm, st);
+    HIP_TRY(hipGetLastError());
     if (stats) {
         if (!m->ev0) HIP_TRY(hipEventCreate(&m->ev0));
         if (!m->ev1) HIP_TRY(hipEventCreate(&m->ev1));
@@ -2299,13 +2300,13 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
     bool fast = false;
     if (precision == OCL_PREC_EXACT64 || m->ntri == 0 || cd.c.kind == K_COMPOSITE) {
         if (m->ntri > 0)
-            k_drop_exact<<<xcd_grid(nb, mv.xcd_group), 256, 0, st>>>(mv, cd, d_cl, m->perm.as<uint32_t>(), nullptr, n32, o, cnt);
+            k_drop_exact<<<xcd_grid(nb, mv.xcd_group), 256, 0, st>>>(mv, cd, d_cl, perm, nullptr, n32, o, cnt);
         else
             k_drop_none<<<nb, 256, 0, st>>>(d_cl, n32, o);
     } else {
         // the selection's pair/escalation counters feed only the statistics:
         // without them its wavefronts skip three same-address atomics each
-        rc = launch_drop_fast(m, mv, cd, d_cl, n32, precision, o, stats ? cnt : nullptr, st, stats != nullptr);
+        rc = launch_drop_fast(m, mv, cd, d_cl, perm, n32, precision, o, stats ? cnt : nullptr, st, stats != nullptr);
         fast = true;
         if (rc) return rc;
     }
@@ -2735,7 +2736,6 @@ extern "C" int ocl_hip_drop_slopes(ocl_mesh* m, const ocl_cutter_desc* desc, con
 // by a device work queue, then each job's resolve and exact passes.  Nothing
 // in the sequence synchronises with the host or records a memset, so it is
 // captured once in a hipGraph and replayed.
-#include "radix.hpp"
 
 namespace ocl_amd {
 
