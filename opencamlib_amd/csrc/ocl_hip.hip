@@ -113,7 +113,7 @@ struct ocl_mesh {
     double zmid = 0, zhalf = 0, xyb = 0;
     // cached selection records (rebuilt when the cutter or precision changes)
     ocl_amd::RecSet rs32, rs64;
-    uint32_t xcd_group = 1;
+    uint32_t xcd_group = 8;       // runs of 8 neighbouring CL tiles per XCD (OCL_XCD_GROUP overrides)
     uint32_t hilbert_bits = 9;   // CL binning: Hilbert curve over 2^bits x 2^bits cells
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_s0 = nullptr, ev_s1 = nullptr;
 
