@@ -58,6 +58,7 @@ enum FastRec : int {
     R_FLAGS,                   // see FL_* bits
     R_ZCAP,                    // cull cap: max z + largest error bound / slack of the contacts
     R_EMAX,                    // largest error bound of the triangle's contacts
+    R_PMAR,                    // plane cap margin (see tri_cap), +inf: no plane cap
     kFastRec = 72              // 288 B in FP32: 16-byte aligned records
 };
 // per-edge values: direction (EX,EY), line constants C and A, 1/length, slope
@@ -65,7 +66,7 @@ enum FastRec : int {
 // the contact height is ill-conditioned (S2E), the height slack of an
 // ambiguous contact (EAE) and the height error bound of the contact (EZE).
 enum EdgeVal : int { E_EX = 0, E_EY, E_C, E_A, E_INVL, E_M, E_Z1, E_K1, E_K2, E_K3, E_S2E, E_EAE, E_EZE };
-static_assert(R_EMAX < kFastRec, "record overflow");
+static_assert(R_PMAR < kFastRec, "record overflow");
 
 enum : int { FL_VERT = 1, FL_HORIZ = 2, FL_EDEG0 = 8, FL_EHOR0 = 64 };
 
@@ -328,7 +329,19 @@ OCL_HD void build_record(const TriX& t, const CutterDev& cd, double ox, double o
     // rounded up: cull tests compare against this cap
     out[R_ZCAP] = (T)((bz1 + Eabs + slk) * (1.0 + 4.0 * u) + 4.0 * u);
     out[R_EMAX] = (T)(emax * (1.0 + 4.0 * u));
-    for (int q = R_EMAX + 1; q < kFastRec; ++q) out[q] = (T)0;
+    // Plane cap.  The facet height F0 + GX x + GY y (without the inside
+    // test) is the height at which the cutter first touches the triangle's
+    // whole plane, and every point of the triangle lies on that plane, so no
+    // vertex, edge or facet contact of the triangle sits above it.  The
+    // margin adds, like R_ZCAP, the contact error bounds and slack, plus the
+    // facet's own rounding once more for the evaluation at the lane.  The
+    // BullCutter's exact edge heights come from Brent solves that may sit up
+    // to about 1e-9 above the converged contact.  Vertical triangles and
+    // cones (tip and rim facets) get no plane cap.
+    const bool pcap = !(flags & FL_VERT) && (c.kind == K_CYL || c.kind == K_BALL || c.kind == K_BULL);
+    const double pextra = (c.kind == K_BULL) ? 4e-9 : 0.0;
+    out[R_PMAR] = pcap ? (T)((2.0 * ezf + Eabs + slk + pextra) * (1.0 + 4.0 * u) + 4.0 * u) : FM<T>::inf();
+    for (int q = R_PMAR + 1; q < kFastRec; ++q) out[q] = (T)0;
 }
 
 // Selection constants for one cutter (host side, FP64 -> T).
@@ -403,6 +416,19 @@ OCL_HD T prof_lower(T d2, const SelParams<T>& P) {
         h = h * ((T)1 - (T)(8.0 * F::eps()));
     }
     return F::max_(h - P.HPAD, (T)0);
+}
+
+// Upper bound of every contact's upper value (z + error bound, or z + slack)
+// of the triangle of record r for a lane at (x, y) in the record's frame,
+// whose XY distance to the triangle's box is sqrt(d2): the lower of the
+// profile cap (R_ZCAP - h(d)) and the plane cap (facet height at the lane
+// + R_PMAR).
+template <class T, int K>
+OCL_HD T tri_cap(const T* r, T x, T y, T d2, const SelParams<T>& P) {
+    typedef FM<T> F;
+    const T zp = r[R_ZCAP] - prof_lower<T, K>(d2, P);
+    if (K == K_CONE) return zp;
+    return F::min_(zp, F::fma_(r[R_GY], y, F::fma_(r[R_GX], x, r[R_F0])) + r[R_PMAR]);
 }
 
 // Contact height of the vertex v = (x, y, z) (cutter base at the vertex, see

@@ -862,7 +862,7 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
                 bool on = act & !((bx1 < xl - P.R) | (bx0 > xl + P.R) | (by1 < yl - P.R) | (by0 > yl + P.R));
                 if (P.cull) {
                     const T d2 = box_d2<T>(xl, yl, bx0, bx1, by0, by1);
-                    on = on & (r[R_ZCAP] - prof_lower<T, K>(d2, P) >= F::max_(S.zb, zc - dz) - P.CUT);
+                    on = on & (tri_cap<T, K>(r, xl, yl, d2, P) >= F::max_(S.zb, zc - dz) - P.CUT);
                 }
                 if (!__any(on)) continue;
                 ++evals;
@@ -883,7 +883,7 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
                 bool on = act & !((bx1 < xl - P.R) | (bx0 > xl + P.R) | (by1 < yl - P.R) | (by0 > yl + P.R));
                 if (P.cull) {
                     const T d2 = box_d2<T>(xl, yl, bx0, bx1, by0, by1);
-                    on = on & (r[R_ZCAP] - prof_lower<T, K>(d2, P) >= F::max_(S.zb, zc - dz) - P.CUT);
+                    on = on & (tri_cap<T, K>(r, xl, yl, d2, P) >= F::max_(S.zb, zc - dz) - P.CUT);
                 }
                 lmask |= on ? (1u << j) : 0u;
             }

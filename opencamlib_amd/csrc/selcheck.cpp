@@ -103,7 +103,7 @@ void run_points(const std::vector<TriX>& tx, const std::vector<BoxOrigin>& org, 
                     continue;
                 if (P.cull) {
                     const T d2 = box_d2<T>(xl, yl, r[R_BX0], r[R_BX1], r[R_BY0], r[R_BY1]);
-                    if (r[R_ZCAP] - prof_lower<T, K>(d2, P) < F::max_(S.zb, zc - dz) - P.CUT) continue;
+                    if (tri_cap<T, K>(r, xl, yl, d2, P) < F::max_(S.zb, zc - dz) - P.CUT) continue;
                 }
                 sel_triangle<T, K>(r, (uint32_t)k, xl, yl, P, S, true);
             }
@@ -357,3 +357,70 @@ extern "C" int ocl_selcheck_brent(const double* in, size_t n, double* out) {
     return 0;
 }
 
+
+// The selection's triangle cap (tri_cap: profile cap and plane cap) against
+// the triangle's own contacts.  For every (point, triangle) pair of the
+// inputs, with the triangle's record built in a frame at its centroid:
+// cap_out = the cap at the point and hi_out = the largest upper value
+// (z + error bound, or z + slack) of the contacts the selection evaluates
+// from z = -inf (-inf: none), both in the global frame; n x ntri row-major.
+extern "C" int ocl_selcheck_tricap(const double* tris, size_t ntri, const ocl_cutter_desc* desc, const double* pts,
+                                   size_t n, int use_float, double* cap_out, double* hi_out) {
+    CutterDev cd;
+    if (!cutter_of(desc, cd)) return OCL_E_UNSUPPORTED;
+    auto go = [&](auto tag) {
+        typedef decltype(tag) T;
+        typedef FM<T> F;
+        SelErr er;
+        er.u = F::eps();
+        er.R = cd.c.radius;
+        er.zmag = 4.0 * cd.c.radius + 10.0;
+        const SelParams<T> P = params<T>(cd, er);
+        T r[kFastRec];
+        for (size_t k = 0; k < ntri; ++k) {
+            const double* q = tris + 9 * k;
+            TriX t;
+            t.p[0] = v3(q[0], q[1], q[2]);
+            t.p[1] = v3(q[3], q[4], q[5]);
+            t.p[2] = v3(q[6], q[7], q[8]);
+            t.n = vunit(vcross(vsub(t.p[0], t.p[1]), vsub(t.p[0], t.p[2])));
+            t.maxz = std::fmax(std::fmax(q[2], q[5]), q[8]);
+            const double ox = (q[0] + q[3] + q[6]) / 3.0, oy = (q[1] + q[4] + q[7]) / 3.0;
+            const double oz = 0.5 * (std::fmin(std::fmin(q[2], q[5]), q[8]) + t.maxz);
+            build_record<T>(t, cd, ox, oy, oz, er, r);
+            for (size_t i = 0; i < n; ++i) {
+                const T xl = (T)(pts[3 * i] - ox), yl = (T)(pts[3 * i + 1] - oy);
+                const T d2 = box_d2<T>(xl, yl, r[R_BX0], r[R_BX1], r[R_BY0], r[R_BY1]);
+                T cap = 0;
+                switch (cd.c.kind) {
+                case K_CYL: cap = tri_cap<T, K_CYL>(r, xl, yl, d2, P); break;
+                case K_BALL: cap = tri_cap<T, K_BALL>(r, xl, yl, d2, P); break;
+                case K_BULL: cap = tri_cap<T, K_BULL>(r, xl, yl, d2, P); break;
+                default: cap = tri_cap<T, K_CONE>(r, xl, yl, d2, P); break;
+                }
+                // from -inf (a valid facet contact lifts and ends the
+                // triangle), then from the facet height itself (the facet
+                // does not lift, the vertices and edges run)
+                const T zf = F::fma_(r[R_GY], yl, F::fma_(r[R_GX], xl, r[R_F0]));
+                T hmax = -F::inf();
+                for (int pass = 0; pass < 2; ++pass) {
+                    Sel<T> S;
+                    sel_init(S, pass == 0 ? -F::inf() : zf);
+                    T hi = -F::inf();
+                    switch (cd.c.kind) {
+                    case K_CYL: sel_triangle<T, K_CYL>(r, 0, xl, yl, P, S, true, &hi); break;
+                    case K_BALL: sel_triangle<T, K_BALL>(r, 0, xl, yl, P, S, true, &hi); break;
+                    case K_BULL: sel_triangle<T, K_BULL>(r, 0, xl, yl, P, S, true, &hi); break;
+                    default: sel_triangle<T, K_CONE>(r, 0, xl, yl, P, S, true, &hi); break;
+                    }
+                    hmax = F::max_(hmax, hi);
+                }
+                cap_out[i * ntri + k] = (double)cap + oz;
+                hi_out[i * ntri + k] = (double)hmax + oz;
+            }
+        }
+    };
+    if (use_float) go(0.0f);
+    else go(0.0);
+    return OCL_OK;
+}

@@ -143,6 +143,49 @@ def test_selection_decides_like_oracle(scene, name, use_float):
         assert abs(alt["z"] - ref["z"][i]) < 1e-9
 
 
+@pytest.mark.parametrize("name", ["cyl", "ball", "bull", "cone"])
+@pytest.mark.parametrize("use_float", [1, 0])
+def test_triangle_cap_bounds_every_contact(name, use_float):
+    """The selection skips a triangle when its cap (the lower of the profile
+    cap and the plane cap) is below the lane's threshold, so the cap must
+    bound both the selection's own upper values and the reference's height
+    of every contact of the triangle.  Where the oracle's contact is a facet
+    contact, the plane cap must also be tight: within its margin (the
+    contacts' error bounds and ambiguity slack: below 0.06 mm in FP32,
+    1e-5 mm in FP64) of the contact, which a plane cap with a wrong slope or
+    offset, or none at all, is not."""
+    tris = W.heightfield(10, 20.0, bumps=3, steps=2, noise=0.3, seed=7).reshape(-1, 9)
+    rng = np.random.default_rng(11)
+    pts = np.column_stack([rng.uniform(-2, 22, 150), rng.uniform(-2, 22, 150), np.full(150, -10.0)])
+    spec = ALL[name]
+    L = SelCheck().L
+    dp = C.POINTER(C.c_double)
+    L.ocl_selcheck_tricap.argtypes = [dp, C.c_size_t, C.POINTER(capi.CutterDesc), dp, C.c_size_t, C.c_int, dp, dp]
+    L.ocl_selcheck_tricap.restype = C.c_int
+    t = np.ascontiguousarray(tris, dtype=np.float64)
+    cap = np.empty((pts.shape[0], t.shape[0]))
+    hi = np.empty_like(cap)
+    d = capi.cutter_desc(spec)
+    P = lambda a: a.ctypes.data_as(dp)
+    assert L.ocl_selcheck_tricap(P(t), t.shape[0], C.byref(d), P(pts), pts.shape[0], use_float, P(cap), P(hi)) == 0
+    cut = O.cutter_from_spec(spec)
+    R = cut.radius
+    bx0, bx1 = t[:, 0::3].min(1), t[:, 0::3].max(1)
+    by0, by1 = t[:, 1::3].min(1), t[:, 1::3].max(1)
+    tight, nfacet = 0, 0
+    for i, (x, y, _) in enumerate(pts):
+        for k in np.nonzero((bx0 <= x + R) & (bx1 >= x - R) & (by0 <= y + R) & (by1 >= y - R))[0]:
+            assert hi[i, k] <= cap[i, k], (i, k, hi[i, k], cap[i, k])
+            ref = O.drop_triangle(cut, t[k], x, y)
+            if ref["lifted"]:
+                assert ref["z"] <= cap[i, k], (i, k, ref["z"], cap[i, k])
+                if ref["feat"] == 0 and name != "cone":  # facet contact
+                    nfacet += 1
+                    tight += cap[i, k] - ref["z"] < (0.06 if use_float else 1e-5)
+    if name != "cone":
+        assert nfacet > 50 and tight == nfacet, (nfacet, tight)
+
+
 def test_stepwise_brent_matches_the_solver_bit_for_bit():
     """The push kernel pools its BullCutter Brent solves with a step-wise
     state machine; on random fiber/edge geometry it must return the same
