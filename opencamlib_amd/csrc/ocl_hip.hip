@@ -1102,20 +1102,31 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
 
 
 
-// One selection pass: a grid over the point list (XCD-aware block mapping).
-// 4 wavefronts per SIMD for the FP32 pass, 3 for the FP64 one (register budget)
+// One selection pass: a grid over the point list, one wavefront per block.
+// Wavefront run times differ by up to 3x; a block's LDS and slot are only
+// freed when all its wavefronts are done, so with 4-wavefront blocks the
+// early finishers idled (13% of the wave slots on the 1M-point Ball
+// workload); single-wavefront blocks free each slot for the next wavefront
+// at once.  XCD-aware mapping in runs of xcd_group x kSelRun wavefronts
+// (the points of xcd_group 256-point tiles).  4 wavefronts per SIMD for the
+// FP32 pass, 3 for the FP64 one (register budget).
+constexpr uint32_t kSelRun = 4;
 template <class T, int K>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3)))
 k_drop_select(SelArgs<T> A) {
-    __shared__ __align__(16) T srec[4][kLeaf * kFastRec];
-    __shared__ uint32_t nt_k[4][kNearTie][64];
-    __shared__ float nt_h[4][kNearTie][64];
-    __shared__ uint32_t bl_b[4][sizeof(T) == 4 ? kBoxList : 1];
-    __shared__ float bl_key[4][sizeof(T) == 4 ? kBoxList : 1];
-    __shared__ uint32_t ptab[4][64];
-    const uint32_t wv = threadIdx.x >> 6;
-    const uint32_t gid = xcd_block(A.m.xcd_group) * blockDim.x + threadIdx.x;
-    select_points<T, K>(A, gid, srec[wv], nt_k[wv], nt_h[wv], bl_b[wv], bl_key[wv], ptab[wv]);
+    __shared__ __align__(16) T srec[kLeaf * kFastRec];
+    __shared__ uint32_t nt_k[kNearTie][64];
+    __shared__ float nt_h[kNearTie][64];
+    __shared__ uint32_t bl_b[sizeof(T) == 4 ? kBoxList : 1];
+    __shared__ float bl_key[sizeof(T) == 4 ? kBoxList : 1];
+    __shared__ uint32_t ptab[64];
+    const uint32_t gid = xcd_block(A.m.xcd_group * kSelRun) * 64u + threadIdx.x;
+    select_points<T, K>(A, gid, srec, nt_k, nt_h, bl_b, bl_key, ptab);
+}
+
+// Grid of k_drop_select over n points.
+static inline uint32_t select_grid(uint32_t n, uint32_t xcd_group) {
+    return xcd_grid((n + 63) / 64, xcd_group * kSelRun);
 }
 
 // The selection passes of a batch of jobs in one persistent launch: the
@@ -2026,15 +2037,15 @@ static int launch_drop_fast(ocl_mesh* m, const MeshView& mv, const CutterDev& cd
     int rc = fast_prepare(m, mv, cd, d_cl, perm, n, precision, o, cnt, st, F);
     if (rc) return rc;
     if (timed) HIP_TRY(hipEventRecord(m->ev_s0, st));
-    const uint32_t nb = xcd_grid((n + 255) / 256, mv.xcd_group);
+    const uint32_t nb = select_grid(n, mv.xcd_group);
     const bool f32 = precision == OCL_PREC_FP32 || precision == OCL_PREC_MIXED;
     const SelArgs<float> A32 = f32 ? fast_args32(F) : SelArgs<float>{};
     const SelArgs<double> A64 = f32 ? SelArgs<double>{} : fast_args64(F);
     switch (cd.c.kind) {
-    case K_CYL: if (f32) k_drop_select<float, K_CYL><<<nb, 256, 0, st>>>(A32); else k_drop_select<double, K_CYL><<<nb, 256, 0, st>>>(A64); break;
-    case K_BALL: if (f32) k_drop_select<float, K_BALL><<<nb, 256, 0, st>>>(A32); else k_drop_select<double, K_BALL><<<nb, 256, 0, st>>>(A64); break;
-    case K_BULL: if (f32) k_drop_select<float, K_BULL><<<nb, 256, 0, st>>>(A32); else k_drop_select<double, K_BULL><<<nb, 256, 0, st>>>(A64); break;
-    case K_CONE: if (f32) k_drop_select<float, K_CONE><<<nb, 256, 0, st>>>(A32); else k_drop_select<double, K_CONE><<<nb, 256, 0, st>>>(A64); break;
+    case K_CYL: if (f32) k_drop_select<float, K_CYL><<<nb, 64, 0, st>>>(A32); else k_drop_select<double, K_CYL><<<nb, 64, 0, st>>>(A64); break;
+    case K_BALL: if (f32) k_drop_select<float, K_BALL><<<nb, 64, 0, st>>>(A32); else k_drop_select<double, K_BALL><<<nb, 64, 0, st>>>(A64); break;
+    case K_BULL: if (f32) k_drop_select<float, K_BULL><<<nb, 64, 0, st>>>(A32); else k_drop_select<double, K_BULL><<<nb, 64, 0, st>>>(A64); break;
+    case K_CONE: if (f32) k_drop_select<float, K_CONE><<<nb, 64, 0, st>>>(A32); else k_drop_select<double, K_CONE><<<nb, 64, 0, st>>>(A64); break;
     default: return fail(OCL_E_UNSUPPORTED, "selection pass has no composite specialisation");
     }
     HIP_TRY(hipGetLastError());
@@ -2837,22 +2848,22 @@ static int batch_record(ocl_batch* b, hipStream_t st, bool timed) {
         if (b->tune.per_job_launch) {
             for (uint32_t j : b->sel_order) {
                 const BatchJob& J = b->jobs[j];
-                const uint32_t nb = xcd_grid((J.n + 255) / 256, J.mv.xcd_group);
+                const uint32_t nb = select_grid(J.n, J.mv.xcd_group);
                 if (f32) {
                     const SelArgs<float> A = fast_args32(J.F);
                     switch (J.cd.c.kind) {
-                    case K_CYL: k_drop_select<float, K_CYL><<<nb, 256, 0, st>>>(A); break;
-                    case K_BALL: k_drop_select<float, K_BALL><<<nb, 256, 0, st>>>(A); break;
-                    case K_BULL: k_drop_select<float, K_BULL><<<nb, 256, 0, st>>>(A); break;
-                    default: k_drop_select<float, K_CONE><<<nb, 256, 0, st>>>(A); break;
+                    case K_CYL: k_drop_select<float, K_CYL><<<nb, 64, 0, st>>>(A); break;
+                    case K_BALL: k_drop_select<float, K_BALL><<<nb, 64, 0, st>>>(A); break;
+                    case K_BULL: k_drop_select<float, K_BULL><<<nb, 64, 0, st>>>(A); break;
+                    default: k_drop_select<float, K_CONE><<<nb, 64, 0, st>>>(A); break;
                     }
                 } else {
                     const SelArgs<double> A = fast_args64(J.F);
                     switch (J.cd.c.kind) {
-                    case K_CYL: k_drop_select<double, K_CYL><<<nb, 256, 0, st>>>(A); break;
-                    case K_BALL: k_drop_select<double, K_BALL><<<nb, 256, 0, st>>>(A); break;
-                    case K_BULL: k_drop_select<double, K_BULL><<<nb, 256, 0, st>>>(A); break;
-                    default: k_drop_select<double, K_CONE><<<nb, 256, 0, st>>>(A); break;
+                    case K_CYL: k_drop_select<double, K_CYL><<<nb, 64, 0, st>>>(A); break;
+                    case K_BALL: k_drop_select<double, K_BALL><<<nb, 64, 0, st>>>(A); break;
+                    case K_BULL: k_drop_select<double, K_BULL><<<nb, 64, 0, st>>>(A); break;
+                    default: k_drop_select<double, K_CONE><<<nb, 64, 0, st>>>(A); break;
                     }
                 }
             }
