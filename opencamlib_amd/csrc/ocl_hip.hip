@@ -493,11 +493,14 @@ __device__ __attribute__((always_inline)) inline void exact_points(const MeshVie
     }
 }
 
-__global__ void __launch_bounds__(256) k_drop_exact(MeshView m, CutterDev cd, const double* __restrict__ cl,
-                                                    const uint32_t* __restrict__ list,
-                                                    const uint32_t* __restrict__ count, uint32_t n, DropOut o,
-                                                    Counters* cnt) {
-    exact_points(m, cd, cl, list, count ? *count : n, o, cnt, xcd_block(m.xcd_group) * blockDim.x + threadIdx.x);
+// One wavefront per block, as the selection kernel: per-point work varies,
+// and a finished wavefront frees its slot at once.  XCD runs of
+// xcd_group x 4 wavefronts.
+__global__ void __launch_bounds__(64) k_drop_exact(MeshView m, CutterDev cd, const double* __restrict__ cl,
+                                                   const uint32_t* __restrict__ list,
+                                                   const uint32_t* __restrict__ count, uint32_t n, DropOut o,
+                                                   Counters* cnt) {
+    exact_points(m, cd, cl, list, count ? *count : n, o, cnt, xcd_block(m.xcd_group * 4u) * 64u + threadIdx.x);
 }
 
 // The exact-path jobs of a batch (composites, or every job in EXACT64
@@ -515,9 +518,9 @@ struct ExactJob {
     Counters* cnt;
 };
 
-__global__ void __launch_bounds__(256) k_drop_exact_jobs(const ExactJob* __restrict__ jobs,
-                                                         const uint32_t* __restrict__ chunk_off, uint32_t njobs) {
-    const uint32_t c = uniform_u(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+__global__ void __launch_bounds__(64) k_drop_exact_jobs(const ExactJob* __restrict__ jobs,
+                                                        const uint32_t* __restrict__ chunk_off, uint32_t njobs) {
+    const uint32_t c = uniform_u(blockIdx.x);
     if (c >= chunk_off[njobs]) return;
     uint32_t lo = 0, hi = njobs;  // the job: last j with chunk_off[j] <= c
     while (hi - lo > 1) {
@@ -2311,7 +2314,8 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
     bool fast = false;
     if (precision == OCL_PREC_EXACT64 || m->ntri == 0 || cd.c.kind == K_COMPOSITE) {
         if (m->ntri > 0)
-            k_drop_exact<<<xcd_grid(nb, mv.xcd_group), 256, 0, st>>>(mv, cd, d_cl, perm, nullptr, n32, o, cnt);
+            k_drop_exact<<<xcd_grid((n32 + 63) / 64, mv.xcd_group * 4u), 64, 0, st>>>(mv, cd, d_cl, perm, nullptr,
+                                                                                    n32, o, cnt);
         else
             k_drop_none<<<nb, 256, 0, st>>>(d_cl, n32, o);
     } else {
@@ -2841,7 +2845,7 @@ static int batch_record(ocl_batch* b, hipStream_t st, bool timed) {
         k_drop_none<<<(J.n + 255) / 256, 256, 0, st>>>(J.cl, J.n, J.o);
     }
     if (b->nexact)
-        k_drop_exact_jobs<<<(b->exchunks + 3) / 4, 256, 0, st>>>(b->exjobs.as<ExactJob>(), b->exoff.as<uint32_t>(),
+        k_drop_exact_jobs<<<b->exchunks, 64, 0, st>>>(b->exjobs.as<ExactJob>(), b->exoff.as<uint32_t>(),
                                                                  b->nexact);
     if (timed) HIP_TRY(hipEventRecord(b->s0, st));
     if (b->nsel) {
