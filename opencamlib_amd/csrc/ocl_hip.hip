@@ -2485,8 +2485,9 @@ int ocl_hip_push_batch(ocl_mesh* m, const ocl_cutter_desc* desc, int dir, const 
     float ms = 0;
     for (;;) {
         if ((rc = m->push_e.ensure(sizeof(PEntry) * (size_t)m->push_cap * nf))) return rc;
-        PushLists PL{m->push_e.as<PEntry>(), m->push_n.as<uint32_t>(), m->push_cap, m->push_ovf.as<uint32_t>()};
-        HIP_TRY(hipMemsetAsync(m->push_ovf.p, 0, 4, st));
+        PushLists PL{m->push_e.as<PEntry>(), m->push_n.as<uint32_t>(), m->push_cap, m->push_ovf.as<uint32_t>(),
+                     m->push_ovf.as<uint32_t>() + 1};
+        HIP_TRY(hipMemsetAsync(m->push_ovf.p, 0, 8, st));  // overflow count and fiber queue
         HIP_TRY(hipMemsetAsync(m->push_cnt.p, 0, sizeof(PushCounters), st));
         if (!m->ev0) HIP_TRY(hipEventCreate(&m->ev0));
         if (!m->ev1) HIP_TRY(hipEventCreate(&m->ev1));

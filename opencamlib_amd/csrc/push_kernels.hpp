@@ -26,6 +26,7 @@ struct PushLists {
     uint32_t* n;        // entries per fiber
     uint32_t cap;
     uint32_t* overflow; // fibers whose list overflowed
+    uint32_t* queue;    // next fiber (sorted position) to take; 0 at launch
 };
 
 OCL_HD PEntry pentry_of(const PIval& i) {
@@ -140,7 +141,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     PEntry* ebuf = emem[wv];
     const double R = cd.c.radius, L = cd.c.length;
     unsigned long long pairs = 0, brent = 0, nint = 0;
-    for (uint32_t w = blockIdx.x * kPushWaves + wv; w < nf; w += gridDim.x * kPushWaves) {
+    // fibers cost very different amounts (a fiber through a sphere against
+    // one that misses every triangle), so each wavefront takes the next
+    // fiber of the sorted order from a device counter when it is done,
+    // instead of a fixed share
+    for (;;) {
+        uint32_t w = 0;
+        if (lane == 0) w = atomicAdd(PL.queue, 1u);
+        w = uniform_u((uint32_t)__shfl((int)w, 0, 64));
+        if (w >= nf) break;
         const uint32_t src = uniform_u(perm[w]);
         const FibX f = make_fiber(fib + 6 * (size_t)src);
         const double across = (dir == 0) ? f.p1.y : f.p1.x;
