@@ -244,9 +244,9 @@ __global__ void k_box_origin(const float4* __restrict__ box, uint32_t nbox, doub
 
 // ------------------------------------------------------------ CL binning
 // CL points are grouped so that each wavefront holds 64 points that are close
-// in XY: 16-bit Morton key over the (mesh box grown by the cutter radius),
-// counting sort by key.  Order inside a key is arbitrary; per-point results
-// do not depend on it.
+// in XY: a Hilbert key over the mesh box grown by the cutter radius, then a
+// stable radix sort by key, so equal keys keep their input order and the
+// point -> wavefront assignment is the same on every run.
 struct BinFrame {
     double x0, y0, sx, sy;  // cell = (clamp((x - x0) * sx), clamp((y - y0) * sy)) on a 2^hb x 2^hb grid
 };
