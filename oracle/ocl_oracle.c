@@ -152,6 +152,8 @@ typedef struct {
     int type;
     int tri, feat;   /* last lift, for parity diagnostics */
     int cur_tri, cur_feat;
+    int diag;        /* record quirk bits (slower; off in timed runs) */
+    int quirk;       /* OC_Q_* bits: where a reference rule discarded a contact */
 } CL;
 
 /* clpoint.cpp:63-68 */
@@ -550,6 +552,69 @@ static int comp_cc_valid_radius(const oc_cutter* c, int n, const CL* cl) {
     return 1;
 }
 
+/* ---- parity diagnostics (test infrastructure; no reference counterpart).
+   They record, in CL.quirk, where one of the reference's rules discards a
+   contact that the geometric supremum of p.z - h(|p - cl|) over the
+   triangle would keep, so that tests compare with the supremum everywhere
+   else.  None of them changes a result. */
+static int single_edge_drop(const oc_cutter* c, CL* cl, P3 p1, P3 p2, double d);
+static int vertex_drop(const oc_cutter* c, CL* cl, const Tri* t);
+static int edge_drop(const oc_cutter* c, CL* cl, const Tri* t);
+
+/* A composite sub-cutter's edge drop keeps only its highest edge contact
+   and the window test rejected it (compositecutter.cpp:175-194): did one of
+   the triangle's edges on its own give sub-cutter n an in-window contact
+   above the running height?  (Edge selection as millingcutter.cpp:79-94.) */
+static int diag_comp_edge_lost(const oc_cutter* c, int n, const CL* cl, const Tri* t) {
+    const oc_cutter* s = c->sub[n];
+    for (int k = 0; k < 3; ++k) {
+        P3 p1 = t->p[k], p2 = t->p[(k + 1) % 3];
+        if (iszero_tol(p1.x - p2.x) && iszero_tol(p1.y - p2.y)) continue;
+        CL tmp;
+        memset(&tmp, 0, sizeof(tmp));
+        tmp.x = cl->x; tmp.y = cl->y; tmp.z = cl->z + c->zoffset[n];
+        tmp.type = OC_NONE; tmp.tri = -1; tmp.feat = -1;
+        double d = xy_distance_to_line(p3(tmp.x, tmp.y, tmp.z), p1, p2);
+        if (d <= s->radius && single_edge_drop(s, &tmp, p1, p2, d) && comp_cc_valid_radius(c, n, &tmp)) return 1;
+    }
+    return 0;
+}
+
+/* A cone sub-cutter's tip contact lifted, so its rim contact was not tried
+   (conecutter.cpp:100-102): would the rim contact have been inside the
+   facet, in the window and above the composite's height after the tip (the
+   tip itself may have passed or failed the window)? */
+static int diag_cone_rim_lost(const oc_cutter* c, int n, const CL* cl, const Tri* t) {
+    const oc_cutter* s = c->sub[n];
+    P3 normal = up_normal(t);
+    if (iszero_tol(normal.x) && iszero_tol(normal.y)) return 0; /* horizontal: no rim case */
+    double a = normal.x, b = normal.y, cz = normal.z;
+    double d = -pdot(normal, t->p[0]);
+    P3 xyn = pxynormalize(normal);
+    CL tmp;
+    memset(&tmp, 0, sizeof(tmp));
+    tmp.x = cl->x; tmp.y = cl->y; tmp.z = cl->z + c->zoffset[n];
+    P3 cyl = psub(p3(tmp.x, tmp.y, tmp.z), pmul(s->radius, xyn));
+    cyl.z = (1.0 / cz) * (-d - a * cyl.x - b * cyl.y);
+    if (!inside_tri(cyl, t)) return 0;
+    if (!(cyl.z - s->length - c->zoffset[n] > cl->z)) return 0;
+    tmp.z = cyl.z - s->length;
+    tmp.cc = cyl;
+    tmp.type = OC_FACET_CYL;
+    return comp_cc_valid_radius(c, n, &tmp);
+}
+
+/* The facet lifted, so the vertex and edge tests of the triangle were
+   skipped (millingcutter.cpp:398-405): would one of them have lifted
+   higher? */
+static int diag_facet_first_hid(const oc_cutter* c, const CL* cl, const Tri* t) {
+    CL tmp = *cl;
+    tmp.diag = 0;
+    vertex_drop(c, &tmp, t);
+    edge_drop(c, &tmp, t);
+    return tmp.z > cl->z;
+}
+
 /* compositecutter.cpp:152-172 */
 static int comp_facet_drop(const oc_cutter* c, CL* cl, const Tri* t) {
     int result = 0;
@@ -566,6 +631,9 @@ static int comp_facet_drop(const oc_cutter* c, CL* cl, const Tri* t) {
                 if (cl_lift(cl, tmp.z - c->zoffset[n], tmp.cc, OC_FACET)) result = 1;
                 cl->cur_feat = save;
             }
+            if (cl->diag && c->sub[n]->kind == OC_CONE && tmp.type == OC_FACET_TIP &&
+                diag_cone_rim_lost(c, n, cl, t))
+                cl->quirk |= OC_Q_CONE_TIP;
         }
     }
     return result;
@@ -687,6 +755,26 @@ static int single_edge_drop(const oc_cutter* c, CL* cl, P3 p1, P3 p2, double d) 
     return 0;
 }
 
+/* One edge of MillingCutter::edgeDrop (millingcutter.cpp:84-91) for a
+   simple cutter: the XY-degenerate skip, the radius test, singleEdgeDrop
+   (millingcutter.cpp:121-136).  p1p2 = 6 doubles; z_io = running height in
+   and out; cc_out (3) = CC point when lifted.  Returns 1 if lifted. */
+int oc_edge_drop(const oc_cutter* c, const double* p1p2, double x, double y, double* z_io, double* cc_out) {
+    if (c->kind == OC_COMPOSITE) return 0;
+    P3 p1 = p3(p1p2[0], p1p2[1], p1p2[2]), p2 = p3(p1p2[3], p1p2[4], p1p2[5]);
+    if (iszero_tol(p1.x - p2.x) && iszero_tol(p1.y - p2.y)) return 0;
+    CL cl;
+    memset(&cl, 0, sizeof(cl));
+    cl.x = x; cl.y = y; cl.z = *z_io;
+    cl.type = OC_NONE; cl.tri = -1; cl.feat = -1;
+    double d = xy_distance_to_line(p3(cl.x, cl.y, cl.z), p1, p2);
+    if (!(d <= c->radius)) return 0;
+    int r = single_edge_drop(c, &cl, p1, p2, d);
+    *z_io = cl.z;
+    if (r && cc_out) { cc_out[0] = cl.cc.x; cc_out[1] = cl.cc.y; cc_out[2] = cl.cc.z; }
+    return r;
+}
+
 /* millingcutter.cpp:79-94 */
 static int base_edge_drop(const oc_cutter* c, CL* cl, const Tri* t) {
     int result = 0;
@@ -721,6 +809,8 @@ static int comp_edge_drop(const oc_cutter* c, CL* cl, const Tri* t) {
                 cl->cur_feat = tmp.feat;
                 if (cl_lift(cl, tmp.z - c->zoffset[n], tmp.cc, OC_EDGE)) result = 1;
                 cl->cur_feat = save;
+            } else if (cl->diag && diag_comp_edge_lost(c, n, cl, t)) {
+                cl->quirk |= OC_Q_COMP_EDGE;
             }
         }
     }
@@ -741,6 +831,8 @@ static int drop_cutter(const oc_cutter* c, CL* cl, const Tri* t) {
         if (!facet) {
             vertex = vertex_drop(c, cl, t);
             if (cl_below(cl, t)) edge = edge_drop(c, cl, t);
+        } else if (cl->diag && diag_facet_first_hid(c, cl, t)) {
+            cl->quirk |= OC_Q_FACET_FIRST;
         }
     }
     return facet || vertex || edge;
@@ -1029,26 +1121,33 @@ static void grid_free(Grid* g) {
     free(g);
 }
 
-int oc_drop_triangle(const oc_cutter* c, const double* q, double x, double y,
-                     double* z_io, double* cc_out, int* cctype_out, int* feat_out) {
+int oc_drop_triangle_diag(const oc_cutter* c, const double* q, double x, double y,
+                          double* z_io, double* cc_out, int* cctype_out, int* feat_out, int* quirk_out) {
     Tri t;
     tri_init(&t, p3(q[0], q[1], q[2]), p3(q[3], q[4], q[5]), p3(q[6], q[7], q[8]));
     CL cl;
     memset(&cl, 0, sizeof(cl));
     cl.x = x; cl.y = y; cl.z = *z_io;
     cl.type = OC_NONE; cl.tri = -1; cl.feat = -1; cl.cur_tri = 0;
+    cl.diag = quirk_out != NULL;
     int r = drop_cutter(c, &cl, &t);
     *z_io = cl.z;
     if (cc_out) { cc_out[0] = cl.cc.x; cc_out[1] = cl.cc.y; cc_out[2] = cl.cc.z; }
     if (cctype_out) *cctype_out = cl.type;
     if (feat_out) *feat_out = cl.feat;
+    if (quirk_out) *quirk_out = cl.quirk;
     return r;
 }
 
-int oc_batch_drop(oc_mesh* m, const oc_cutter* c, const double* cl_xyz, long n,
-                  int search_mode, int nthreads, unsigned bucket,
-                  double* z_out, double* cc_out, int* cctype_out,
-                  int* tri_out, int* feat_out, long* ncalls_out) {
+int oc_drop_triangle(const oc_cutter* c, const double* q, double x, double y,
+                     double* z_io, double* cc_out, int* cctype_out, int* feat_out) {
+    return oc_drop_triangle_diag(c, q, x, y, z_io, cc_out, cctype_out, feat_out, NULL);
+}
+
+int oc_batch_drop_diag(oc_mesh* m, const oc_cutter* c, const double* cl_xyz, long n,
+                       int search_mode, int nthreads, unsigned bucket,
+                       double* z_out, double* cc_out, int* cctype_out,
+                       int* tri_out, int* feat_out, long* ncalls_out, int* quirk_out) {
     if (bucket < 1) bucket = 1;
     KDT* k = NULL;
     Grid* g = NULL;
@@ -1080,6 +1179,7 @@ int oc_batch_drop(oc_mesh* m, const oc_cutter* c, const double* cl_xyz, long n,
             cl.x = cl_xyz[3 * i]; cl.y = cl_xyz[3 * i + 1]; cl.z = cl_xyz[3 * i + 2];
             cl.type = OC_NONE; cl.tri = -1; cl.feat = -1;
             cl.cc = p3(0, 0, 0);
+            cl.diag = quirk_out != NULL;
             if (m->n > 0) {
                 cand.n = 0;
                 if (search_mode == 0) {
@@ -1137,6 +1237,7 @@ int oc_batch_drop(oc_mesh* m, const oc_cutter* c, const double* cl_xyz, long n,
             if (cctype_out) cctype_out[i] = cl.type;
             if (tri_out) tri_out[i] = cl.tri;
             if (feat_out) feat_out[i] = cl.feat;
+            if (quirk_out) quirk_out[i] = cl.quirk;
         }
         free(cand.v);
         free(kr);
@@ -1146,6 +1247,14 @@ int oc_batch_drop(oc_mesh* m, const oc_cutter* c, const double* cl_xyz, long n,
     free(rank);
     if (ncalls_out) *ncalls_out = calls;
     return 0;
+}
+
+int oc_batch_drop(oc_mesh* m, const oc_cutter* c, const double* cl_xyz, long n,
+                  int search_mode, int nthreads, unsigned bucket,
+                  double* z_out, double* cc_out, int* cctype_out,
+                  int* tri_out, int* feat_out, long* ncalls_out) {
+    return oc_batch_drop_diag(m, c, cl_xyz, n, search_mode, nthreads, bucket, z_out, cc_out, cctype_out, tri_out,
+                              feat_out, ncalls_out, NULL);
 }
 
 /* ------------------------------------------------------------------- push */
@@ -1562,6 +1671,9 @@ static int base_edge_push(const oc_cutter* c, const Fib* f, Ival* i, const Tri* 
 
 /* compositecutter.cpp:196-270: run the sub-cutter method on a raised fiber,
    keep contacts whose cc height maps back to that sub-cutter */
+/* parity diagnostic of the push (OC_Q_COMP_BELOW), per thread */
+static __thread int g_push_quirk;
+
 static int comp_push(const oc_cutter* c, const Fib* f, Ival* i, const Tri* t, int which) {
     int result = 0;
     double ct[2 * OC_MAX_SUB];
@@ -1581,9 +1693,11 @@ static int comp_push(const oc_cutter* c, const Fib* f, Ival* i, const Tri* t, in
         if (ok) {
             if (n == comp_height_to_index(c, ci.ucc.z - f->p1.z)) {
                 ct[nc] = ci.upper; cp[nc] = ci.ucc; cty[nc] = ci.utype; ++nc;
+                if (ci.ucc.z - f->p1.z < -1E-6) g_push_quirk |= OC_Q_COMP_BELOW;
             }
             if (n == comp_height_to_index(c, ci.lcc.z - f->p1.z)) {
                 ct[nc] = ci.lower; cp[nc] = ci.lcc; cty[nc] = ci.ltype; ++nc;
+                if (ci.lcc.z - f->p1.z < -1E-6) g_push_quirk |= OC_Q_COMP_BELOW;
             }
         }
     }
@@ -1677,16 +1791,22 @@ static void ival_in(Ival* i, const double* o) {
     i->ucc = p3(o[6], o[7], o[8]); i->utype = (int)o[9];
 }
 
-int oc_push_triangle(const oc_cutter* c, const double* fib6, const double* q, double* out10) {
+int oc_push_triangle_diag(const oc_cutter* c, const double* fib6, const double* q, double* out10, int* quirk_out) {
     Tri t;
     tri_init(&t, p3(q[0], q[1], q[2]), p3(q[3], q[4], q[5]), p3(q[6], q[7], q[8]));
     Fib f;
     fib_init(&f, fib6);
     Ival i;
     ival_init(&i);
+    g_push_quirk = 0;
     push_cutter(c, &f, &i, &t);
     ival_out(&i, out10);
+    if (quirk_out) *quirk_out = g_push_quirk;
     return !ival_empty(&i);
+}
+
+int oc_push_triangle(const oc_cutter* c, const double* fib6, const double* q, double* out10) {
+    return oc_push_triangle_diag(c, fib6, q, out10, NULL);
 }
 
 long oc_fiber_union(const double* ints10, long n, long cap, double* out10) {

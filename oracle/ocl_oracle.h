@@ -11,19 +11,23 @@
  * Pinning (what checks this restatement independently of its own formulas):
  *   - closed forms: plane facet contacts, vertex contacts, Cyl/Ball edge
  *     contacts against lines (tests/test_oracle_closed_form.py);
- *   - brute force: drop z of every cutter against a triangle equals the
- *     supremum of p.z - height(|p - cl|_xy) over densely sampled triangle
- *     points, refined by golden-section search (tests/test_oracle_bruteforce.py);
- *   - push vs drop duality: push interval endpoints equal the fiber
- *     positions where the drop height crosses the fiber z;
- *   - KD search vs exhaustive overlap scan (candidate sets equal);
+ *   - brute force: drop z of every cutter, composites included, against a
+ *     triangle equals the supremum of p.z - height(|p - cl|_xy) over densely
+ *     sampled triangle points, refined by golden-section search
+ *     (tests/test_oracle_bruteforce.py, tests/test_oracle_composite.py);
+ *     where a reference rule discards a contact the supremum keeps, the
+ *     oracle's diagnostic entry points flag the point (OC_Q_* below) and
+ *     the check is that the result is only lower there;
+ *   - push vs drop duality, composites included: push interval endpoints
+ *     equal the fiber positions where the drop height crosses the fiber z;
+ *   - ConeCutter with l > 0: the rim facet contact is the l = 0 one moved
+ *     down by l (conecutter.cpp:92); the hyperbola edge contact
+ *     (conecutter.cpp:122) is the tangency point of the cone of height
+ *     `length` over the same radius (tests/test_oracle_composite.py);
+ *   - KD tree leaf order with ties in spread and the Fiber::addInterval CC
+ *     attribution against hand-derived cases (tests/test_oracle_hand.py);
+ *     KD search vs exhaustive overlap scan (candidate sets equal);
  *   - interval union vs an independent sort-and-merge union.
- * Parity unpinned (no independent check reproduces these reference quirks):
- *   - ConeCutter rim facet contact, which subtracts `length` instead of
- *     `center_height` (conecutter.cpp:92), and the hyperbola edge branch
- *     using square(length) (conecutter.cpp:122);
- *   - CompositeCutter edge windows, which validate only the highest of the
- *     three edge contacts per sub-cutter (compositecutter.cpp:175-194).
  */
 #ifndef OCL_ORACLE_H
 #define OCL_ORACLE_H
@@ -102,6 +106,38 @@ int oc_batch_drop(oc_mesh* m, const oc_cutter* c, const double* cl_xyz, long n,
    CL point starting at z_io; returns 1 if lifted. */
 int oc_drop_triangle(const oc_cutter* c, const double* tri9, double x, double y,
                      double* z_io, double* cc_out, int* cctype_out, int* feat_out);
+
+/* One edge of MillingCutter::edgeDrop for a simple cutter (the radius test
+   and singleEdgeDrop, millingcutter.cpp:84-91, 121-136); p1p2 = 6 doubles;
+   z_io = running height in and out; returns 1 if lifted (cc_out = CC). */
+int oc_edge_drop(const oc_cutter* c, const double* p1p2, double x, double y, double* z_io, double* cc_out);
+
+/* Parity diagnostics (no reference counterpart).  The _diag variants give
+   the same results and also OR, per CL point, the bits below wherever one of
+   the reference's rules discarded a contact that the geometric supremum of
+   p.z - h(|p - cl|_xy) over the triangles would keep.  Tests compare with
+   that supremum at every point without a bit. */
+#define OC_Q_COMP_EDGE 1   /* composite: the sub-cutter's highest edge contact
+                              failed the radius window and a single edge had
+                              an in-window contact above the running height
+                              (compositecutter.cpp:175-194) */
+#define OC_Q_CONE_TIP 2    /* composite: a cone sub-cutter's tip lift skipped
+                              its rim contact (conecutter.cpp:100-102), the
+                              tip failed the window and the rim would have
+                              lifted in the window */
+#define OC_Q_FACET_FIRST 4 /* the facet lifted, so the vertex and edge tests
+                              were skipped (millingcutter.cpp:398-405), and
+                              one of them would have lifted higher */
+#define OC_Q_COMP_BELOW 8  /* composite push: a contact below the first
+                              height window was kept for the last sub-cutter,
+                              the fall-back of height_to_index
+                              (compositecutter.cpp:71-83, 196-270) */
+int oc_push_triangle_diag(const oc_cutter* c, const double* fib6, const double* tri9, double* out10, int* quirk_out);
+int oc_drop_triangle_diag(const oc_cutter* c, const double* tri9, double x, double y, double* z_io, double* cc_out,
+                          int* cctype_out, int* feat_out, int* quirk_out);
+int oc_batch_drop_diag(oc_mesh* m, const oc_cutter* c, const double* cl_xyz, long n, int search_mode, int nthreads,
+                       unsigned bucket, double* z_out, double* cc_out, int* cctype_out, int* tri_out, int* feat_out,
+                       long* ncalls_out, int* quirk_out);
 
 /* Analytic gradient of the drop height at a CL point from its active contact
    (feature code and CC point of the forward pass).  Not a reference

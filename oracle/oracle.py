@@ -17,6 +17,8 @@ LIB_PATH = os.path.join(_HERE, "build", "libocl_oracle.so")
 
 CYL, BALL, BULL, CONE, COMPOSITE = 0, 1, 2, 3, 4
 MAX_SUB = 4
+# parity diagnostics of the _diag entry points (ocl_oracle.h)
+Q_COMP_EDGE, Q_CONE_TIP, Q_FACET_FIRST, Q_COMP_BELOW = 1, 2, 4, 8
 
 
 class OCCutter(C.Structure):
@@ -73,11 +75,20 @@ def lib():
         L.oc_batch_drop.restype = C.c_int
         L.oc_drop_triangle.argtypes = [cp, dp, C.c_double, C.c_double, dp, dp, ip, ip]
         L.oc_drop_triangle.restype = C.c_int
+        L.oc_edge_drop.argtypes = [cp, dp, C.c_double, C.c_double, dp, dp]
+        L.oc_edge_drop.restype = C.c_int
+        L.oc_drop_triangle_diag.argtypes = [cp, dp, C.c_double, C.c_double, dp, dp, ip, ip, ip]
+        L.oc_drop_triangle_diag.restype = C.c_int
+        L.oc_batch_drop_diag.argtypes = [C.c_void_p, cp, dp, C.c_long, C.c_int, C.c_int, C.c_uint,
+                                         dp, dp, ip, ip, ip, lp, ip]
+        L.oc_batch_drop_diag.restype = C.c_int
         L.oc_batch_push.argtypes = [C.c_void_p, cp, C.c_int, dp, C.c_long, C.c_int, C.c_int, C.c_uint,
                                     C.c_long, lp, dp, dp, lp]
         L.oc_batch_push.restype = C.c_long
         L.oc_push_triangle.argtypes = [cp, dp, dp, dp]
         L.oc_push_triangle.restype = C.c_int
+        L.oc_push_triangle_diag.argtypes = [cp, dp, dp, dp, ip]
+        L.oc_push_triangle_diag.restype = C.c_int
         L.oc_fiber_union.argtypes = [dp, C.c_long, C.c_long, dp]
         L.oc_fiber_union.restype = C.c_long
         L.oc_drop_grad.argtypes = [cp, dp, C.c_int, C.c_int, C.c_double, C.c_double, dp, dp]
@@ -187,7 +198,8 @@ class Mesh:
         n = lib().oc_kd_search_push(self.h, C.byref(cutter.c), direction, _dp(f), bucket, _ip(out), cap)
         return out[:n]
 
-    def batch_drop(self, cutter: Cutter, cl_xyz, search_mode=1, nthreads=0, bucket=1):
+    def batch_drop(self, cutter: Cutter, cl_xyz, search_mode=1, nthreads=0, bucket=1, diag=False):
+        """diag: also return `quirk`, the OC_Q_* bits per point (slower)."""
         cl = np.ascontiguousarray(np.asarray(cl_xyz, dtype=np.float64).reshape(-1, 3))
         n = cl.shape[0]
         z = np.empty(n)
@@ -196,6 +208,11 @@ class Mesh:
         tri = np.empty(n, dtype=np.int32)
         feat = np.empty(n, dtype=np.int32)
         calls = np.zeros(1, dtype=np.int64)
+        if diag:
+            quirk = np.zeros(n, dtype=np.int32)
+            lib().oc_batch_drop_diag(self.h, C.byref(cutter.c), _dp(cl), n, search_mode, nthreads, bucket,
+                                     _dp(z), _dp(cc), _ip(ty), _ip(tri), _ip(feat), _lp(calls), _ip(quirk))
+            return dict(z=z, cc=cc, type=ty, tri=tri, feat=feat, calls=int(calls[0]), quirk=quirk)
         lib().oc_batch_drop(self.h, C.byref(cutter.c), _dp(cl), n, search_mode, nthreads, bucket,
                             _dp(z), _dp(cc), _ip(ty), _ip(tri), _ip(feat), _lp(calls))
         return dict(z=z, cc=cc, type=ty, tri=tri, feat=feat, calls=int(calls[0]))
@@ -233,20 +250,38 @@ class Mesh:
 
 
 def drop_triangle(cutter: Cutter, tri9, x, y, z0=-1e9):
+    """One triangle's MillingCutter::dropCutter; `quirk` holds the OC_Q_*
+    bits of the parity diagnostics."""
     t = np.ascontiguousarray(np.asarray(tri9, dtype=np.float64).reshape(9))
     z = np.array([z0], dtype=np.float64)
     cc = np.zeros(3)
     ty = np.zeros(1, dtype=np.int32)
     ft = np.zeros(1, dtype=np.int32)
-    lifted = lib().oc_drop_triangle(C.byref(cutter.c), _dp(t), x, y, _dp(z), _dp(cc), _ip(ty), _ip(ft))
-    return dict(lifted=bool(lifted), z=float(z[0]), cc=cc, type=int(ty[0]), feat=int(ft[0]))
+    q = np.zeros(1, dtype=np.int32)
+    lifted = lib().oc_drop_triangle_diag(C.byref(cutter.c), _dp(t), x, y, _dp(z), _dp(cc), _ip(ty), _ip(ft),
+                                         _ip(q))
+    return dict(lifted=bool(lifted), z=float(z[0]), cc=cc, type=int(ty[0]), feat=int(ft[0]), quirk=int(q[0]))
 
 
-def push_triangle(cutter: Cutter, fib6, tri9):
+def edge_drop(cutter: Cutter, p1p2, x, y, z0=-1e9):
+    """One edge of MillingCutter::edgeDrop (simple cutters)."""
+    e = np.ascontiguousarray(np.asarray(p1p2, dtype=np.float64).reshape(6))
+    z = np.array([z0], dtype=np.float64)
+    cc = np.zeros(3)
+    lifted = lib().oc_edge_drop(C.byref(cutter.c), _dp(e), x, y, _dp(z), _dp(cc))
+    return dict(lifted=bool(lifted), z=float(z[0]), cc=cc)
+
+
+def push_triangle(cutter: Cutter, fib6, tri9, diag=False):
+    """One (fiber, triangle) MillingCutter::pushCutter: (non-empty, out10),
+    plus the OC_Q_* bits of the parity diagnostics with diag."""
     f = np.ascontiguousarray(np.asarray(fib6, dtype=np.float64).reshape(6))
     t = np.ascontiguousarray(np.asarray(tri9, dtype=np.float64).reshape(9))
     out = np.zeros(10)
-    ne = lib().oc_push_triangle(C.byref(cutter.c), _dp(f), _dp(t), _dp(out))
+    q = np.zeros(1, dtype=np.int32)
+    ne = lib().oc_push_triangle_diag(C.byref(cutter.c), _dp(f), _dp(t), _dp(out), _ip(q))
+    if diag:
+        return bool(ne), out, int(q[0])
     return bool(ne), out
 
 

@@ -14,6 +14,12 @@ an edge (1-D concave search), or at an interior stationary point of the plane
     Ball  h = R - sqrt(R^2 - q^2)
     Bull  h = 0 for q <= R-r, else r - sqrt(r^2 - (q-R+r)^2)
     Cone  h = q / tan(angle)
+    Composite  h = h_n(q) + zoffset_n on the n-th radius window
+               (R_{n-1}, R_n] (R_{-1} = 0), R = the last R_n
+
+For a composite the interior maximiser over a plane of uphill slope G sits at
+the distance q maximising G q - h(q), a concave function when h is convex:
+found by golden-section search, not by any closed form.
 """
 from __future__ import annotations
 
@@ -22,8 +28,42 @@ import math
 import numpy as np
 
 
+def _golden_max(f, lo, hi, iters=200):
+    """argmax of a concave f on [lo, hi] by golden-section search."""
+    g = (math.sqrt(5.0) - 1.0) / 2.0
+    x1 = hi - g * (hi - lo)
+    x2 = lo + g * (hi - lo)
+    f1, f2 = f(x1), f(x2)
+    for _ in range(iters):
+        if hi - lo < 1e-15:
+            break
+        if f1 < f2:
+            lo, x1, f1 = x1, x2, f2
+            x2 = lo + g * (hi - lo)
+            f2 = f(x2)
+        else:
+            hi, x2, f2 = x2, x1, f1
+            x1 = hi - g * (hi - lo)
+            f1 = f(x1)
+    return max((lo, hi, 0.5 * (lo + hi)), key=f)
+
+
 def profile(spec):
     k = spec["kind"]
+    if k == "composite":
+        parts = []
+        for s in spec["subs"]:
+            _, hs = profile(s["cutter"])
+            parts.append((s["r"], s["zoff"], hs))
+        R = parts[-1][0]
+
+        def hc(q):
+            for rn, zo, hs in parts:
+                if q <= rn:
+                    return hs(q) + zo
+            rn, zo, hs = parts[-1]
+            return hs(q) + zo
+        return R, hc
     R = spec["d"] / 2.0
     if k == "cyl":
         return R, lambda q: 0.0
@@ -108,7 +148,7 @@ def facet_stationary(spec, tri, cl):
     G = math.hypot(gx, gy)
     plane = lambda x, y: a[2] + gx * (x - a[0]) + gy * (y - a[1])
     ux, uy = (gx / G, gy / G) if G > 0 else (0.0, 0.0)
-    R = spec["d"] / 2.0
+    R = profile(spec)[0]
     k = spec["kind"]
     if k == "cyl":
         q = R if G > 0 else 0.0
@@ -128,6 +168,10 @@ def facet_stationary(spec, tri, cl):
             q, dz = R, R * cot  # rim contact
         else:
             q, dz = 0.0, 0.0    # tip contact
+    elif k == "composite":
+        _, h = profile(spec)
+        q = _golden_max(lambda u: G * u - h(u), 0.0, R) if G > 0 else 0.0
+        dz = h(q)
     else:
         raise ValueError(k)
     px, py = cl[0] + q * ux, cl[1] + q * uy

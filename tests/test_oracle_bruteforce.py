@@ -7,7 +7,8 @@ the Cone edge branches (conecutter.cpp:109-135) independently of their
 formulas.  Cone cutters are built with l=0: the reference's rim facet and
 hyperbola edge formulas use `length` where the geometry has `center_height`
 (conecutter.cpp:92, 122); with l=0 the two coincide, so the brute force pins
-the code paths exactly, and the l>0 variant stays parity-unpinned.
+the code paths exactly.  The l>0 variant is pinned against these l=0 results
+by tests/test_oracle_composite.py, which also pins composite cutters.
 
 Batch drops are checked across the three candidate searches (reference KD
 order, exhaustive overlap scan in KD rank order, all triangles), and push
@@ -164,10 +165,10 @@ PUSH_SPECS = {
 }
 
 
-def collide_interval(spec, tri, fib, fz):
+def collide_interval(spec, tri, fib, fz, samples=801):
     """Interval of fiber parameters where the drop height exceeds fz."""
     p1, p2 = np.asarray(fib[:3]), np.asarray(fib[3:])
-    ts = np.linspace(0, 1, 801)
+    ts = np.linspace(0, 1, samples)
     vals = np.array([drop_sup(spec, tri, p1 + t * (p2 - p1), geometric=True)[0] - fz for t in ts])
     inside = vals > 0
     if not inside.any():
