@@ -143,8 +143,12 @@ int ocl_hip_drop_device(ocl_mesh* mesh, const ocl_cutter_desc* cutter, const dou
  *   d_grad_r   optional, one double: ADDS sum_i grad_z[i] dz_i/dr
  * precision: OCL_PREC_FP32 / OCL_PREC_MIXED compute facet and vertex contacts
  * in FP32 (FP64 where the profile is ill-conditioned) and edge contacts in
- * FP64; OCL_PREC_FP64 / OCL_PREC_EXACT64 compute everything in FP64.  The
- * accumulated sums use FP64 atomics: their last bits vary run to run. */
+ * FP64; OCL_PREC_FP64 / OCL_PREC_EXACT64 compute everything in FP64.
+ * d_dz and d_grad_r are deterministic (d_grad_r is summed per block, then by
+ * one fixed-order reduction); d_grad_tri is accumulated with FP64 atomics,
+ * so its last bits vary run to run.  The per-block partial sums live in the
+ * mesh handle: device calls on one mesh must run on one stream, or be
+ * ordered with events, like the reference's non-re-entrant Operation. */
 int ocl_hip_drop_backward(ocl_mesh* mesh, const ocl_cutter_desc* cutter, const double* d_cl_xyz, size_t n,
                           const double* d_cc_xyz, const uint8_t* d_cc_type, const uint32_t* d_tri,
                           const uint8_t* d_feat, const double* d_grad_z, int precision, double* d_dz,
@@ -172,8 +176,12 @@ int ocl_hip_drop_slopes(ocl_mesh* mesh, const ocl_cutter_desc* cutter, const dou
  * with no host synchronisation and no memset, so with use_graph the whole
  * sequence is captured once in a hipGraph and every ocl_hip_batch_run
  * replays it (re-reading the jobs' CL arrays).  Composite cutters and
- * OCL_PREC_EXACT64 take the exact pass.  A mesh may appear in one job per
- * batch (its workspace holds that job's state). */
+ * OCL_PREC_EXACT64 take the exact pass.  Each job has its own workspace and
+ * keeps the selection records of its cutter, so a mesh may appear in several
+ * jobs (with different cutters) and other calls on a mesh, drop calls with
+ * other cutters or more points and other batches included, do not change
+ * what a batch replays.  A mesh must outlive the batches that use it:
+ * ocl_hip_mesh_destroy fails with OCL_E_ARG while one is alive. */
 typedef struct {
     ocl_mesh* mesh;
     ocl_cutter_desc cutter;
@@ -234,6 +242,25 @@ int ocl_hip_push_batch(ocl_mesh* mesh, const ocl_cutter_desc* cutter, int dir, c
  * parameters t of p1 + t (p2 - p1); cc = lower CC xyz, lower CCType,
  * upper CC xyz, upper CCType (8 doubles per row). */
 int ocl_hip_push_fetch(ocl_mesh* mesh, double* lohi, double* cc, void* stream);
+
+/* X and Y fibers in one call: replaces the two BatchPushCutter runs of
+ * Waterline::run (waterline.cpp:78-87, which notes they "could/should run
+ * in parallel").  Both directions' fibers form one device queue in one
+ * persistent launch, so the second direction fills the GPU while the first
+ * one's last fibers finish.  xfib: nx X fibers, yfib: ny Y fibers (as
+ * ocl_hip_push_batch with dir 0 and 1); the results per direction are those
+ * of ocl_hip_push_batch for that direction alone: x_offsets_out[0..nx],
+ * y_offsets_out[0..ny], totals_out[0..1] = row totals (X, Y); rows via
+ * ocl_hip_push_fetch_dir.  stats: both directions together. */
+int ocl_hip_push_xy(ocl_mesh* mesh, const ocl_cutter_desc* cutter, const double* xfib, size_t nx,
+                    const double* yfib, size_t ny, int precision, uint64_t* x_offsets_out,
+                    uint64_t* y_offsets_out, uint64_t* totals_out, ocl_stats* stats, void* stream);
+
+/* Rows of the last push in direction dir (0 = X, 1 = Y; by ocl_hip_push_batch
+ * or ocl_hip_push_xy), as ocl_hip_push_fetch.  No reference counterpart:
+ * the fibers of BatchPushCutter::getFibers (batchpushcutter.hpp:65) for one
+ * direction. */
+int ocl_hip_push_fetch_dir(ocl_mesh* mesh, int dir, double* lohi, double* cc, void* stream);
 
 /* Thread-local message of the last failure. */
 const char* ocl_hip_last_error(void);

@@ -91,6 +91,11 @@ def lib():
         L.ocl_hip_push_batch.restype = C.c_int
         L.ocl_hip_push_fetch.argtypes = [vp, dp, dp, vp]
         L.ocl_hip_push_fetch.restype = C.c_int
+        L.ocl_hip_push_xy.argtypes = [vp, C.POINTER(CutterDesc), dp, sz, dp, sz, C.c_int, u64p, u64p, u64p,
+                                      C.POINTER(Stats), vp]
+        L.ocl_hip_push_xy.restype = C.c_int
+        L.ocl_hip_push_fetch_dir.argtypes = [vp, C.c_int, dp, dp, vp]
+        L.ocl_hip_push_fetch_dir.restype = C.c_int
         L.ocl_hip_drop_backward.argtypes = [vp, C.POINTER(CutterDesc), vp, sz, vp, vp, vp, vp, vp, C.c_int, vp, vp,
                                             vp, vp]
         L.ocl_hip_drop_backward.restype = C.c_int
@@ -182,8 +187,10 @@ class DeviceMesh:
         self.ntri = t.shape[0]
 
     def close(self):
+        """Frees the handle; raises OclError (and keeps it) while a batch
+        still uses the mesh."""
         if getattr(self, "h", None):
-            lib().ocl_hip_mesh_destroy(self.h)
+            check(lib().ocl_hip_mesh_destroy(self.h))
             self.h = None
 
     def __del__(self):
@@ -237,6 +244,30 @@ class DeviceMesh:
             check(lib().ocl_hip_push_fetch(self.h, _ptr(lohi, C.c_double), _ptr(cc, C.c_double), None))
         return dict(offsets=offs.astype(np.int64), lohi=lohi, cc=cc, stats=st.as_dict())
 
+    def push_xy(self, spec: dict, xfibers, yfibers, precision="exact64", want_stats=True, exact_filter=False):
+        """X and Y fibers in one launch (ocl_hip_push_xy).  Returns {"x": ...,
+        "y": ..., "stats": ...}, each direction as push() returns it."""
+        fx = np.ascontiguousarray(np.asarray(xfibers, dtype=np.float64).reshape(-1, 6))
+        fy = np.ascontiguousarray(np.asarray(yfibers, dtype=np.float64).reshape(-1, 6))
+        ox = np.zeros(fx.shape[0] + 1, dtype=np.uint64)
+        oy = np.zeros(fy.shape[0] + 1, dtype=np.uint64)
+        tot = np.zeros(2, dtype=np.uint64)
+        st = Stats()
+        desc = cutter_desc(spec)
+        prec = PREC[precision] | (PUSH_EXACT_FILTER if exact_filter else 0)
+        check(lib().ocl_hip_push_xy(self.h, C.byref(desc), _ptr(fx, C.c_double), fx.shape[0], _ptr(fy, C.c_double),
+                                    fy.shape[0], prec, _ptr(ox, C.c_uint64), _ptr(oy, C.c_uint64),
+                                    _ptr(tot, C.c_uint64), C.byref(st) if want_stats else None, None))
+        out = {"stats": st.as_dict()}
+        for d, offs in ((0, ox), (1, oy)):
+            n = int(tot[d])
+            lohi = np.empty((n, 2))
+            cc = np.empty((n, 8))
+            if n:
+                check(lib().ocl_hip_push_fetch_dir(self.h, d, _ptr(lohi, C.c_double), _ptr(cc, C.c_double), None))
+            out["xy"[d]] = dict(offsets=offs.astype(np.int64), lohi=lohi, cc=cc)
+        return out
+
     def drop_device(self, spec: dict, d_cl: int, n: int, precision, d_z: int, d_cc=0, d_type=0, d_tri=0,
                     d_feat=0, stream=None, stats=False):
         """Asynchronous drop on device pointers (ints, e.g. torch data_ptr())."""
@@ -261,8 +292,11 @@ class DeviceMesh:
 
 def make_jobs(items):
     """ctypes array of DropJob from dicts {mesh: DeviceMesh, spec, cl, n, z, cc,
-    type, tri, feat} whose arrays are device pointers (ints)."""
+    type, tri, feat} whose arrays are device pointers (ints).  The array keeps
+    its DeviceMesh objects alive (`meshes`), and so does every Batch built
+    from it."""
     arr = (DropJob * len(items))()
+    arr.meshes = [it["mesh"] for it in items]
     for i, it in enumerate(items):
         j = arr[i]
         j.mesh = it["mesh"].h
@@ -286,6 +320,9 @@ class Batch:
         check(lib().ocl_hip_batch_create(jobs, len(jobs), PREC[precision], C.byref(t), C.byref(h)))
         self.h = h
         self.jobs = jobs
+        # a mesh must outlive every batch that uses it (ocl_hip_mesh_destroy
+        # refuses while one is alive)
+        self.meshes = list(getattr(jobs, "meshes", []))
 
     def run(self, stream=None, want_stats=True):
         st = Stats()

@@ -59,13 +59,43 @@ struct DevBuf {
 };
 
 // Selection records of one precision: per-triangle records, per-box cull
-// caps and the mesh's largest contact error bound (read back once), cached
-// for the last cutter.
+// caps and the mesh's largest contact error bound (read back once), for one
+// cutter.  A mesh caches the set of its last cutter; a batch keeps the sets
+// of its jobs alive, so a later call with another cutter builds a new set
+// instead of overwriting one a batch replays.
 struct RecSet {
     DevBuf rec, cap, emax;
     bool ok = false;
     CutterDev key{};
     double emax_h = 0.0;
+};
+
+// Workspace of the push in one fiber direction: the fibers and their order,
+// the interval lists (cap entries per fiber), the row offsets and the
+// gathered rows of the last push.
+struct PushWork {
+    DevBuf fib, perm, e, n, ovf, offs, lohi, cc;
+    uint32_t cap = 16;       // interval capacity per fiber of the last push (stride of e)
+    uint32_t cap_next = 16;  // the next push starts here: what the last one needed
+    uint32_t nf = 0;
+    uint64_t total = 0;
+};
+
+// Workspace of one drop job: the per-point selection records, near-tie
+// sets, escalation queues and floors, and the job's counters.  A mesh owns
+// one for its single calls and each batch job owns its own, so a batch's
+// captured pointers stay valid whatever other calls do on the mesh.
+struct DropWork {
+    DevBuf esc_list, esc_list2, esc_list3, zfloor, ntset, ntcnt, zband, selrec;
+    DevBuf counters, cnt2, cnt3, qcount;
+    int exact_slot = 0;  // queue-count slot read by the exact pass in the last run
+    int ensure_counters() {
+        int rc;
+        if ((rc = counters.ensure(sizeof(uint64_t) * 8)) || (rc = cnt2.ensure(sizeof(uint64_t) * 8)) ||
+            (rc = cnt3.ensure(sizeof(uint64_t) * 8)) || (rc = qcount.ensure(16)))
+            return rc;
+        return OCL_OK;
+    }
 };
 
 }  // namespace ocl_amd
@@ -86,10 +116,9 @@ struct ocl_mesh {
     int device = 0;
     std::vector<double> host_tri;  // input triangles (for the push trees, built on first use)
     ocl_tree push_tree[2];
-    ocl_amd::DevBuf push_fib, push_perm, push_e, push_n, push_ovf, push_offs, push_cnt, push_lohi, push_cc;
-    uint32_t push_cap = 16;
-    uint32_t push_nf = 0;          // fibers of the last push call (for ocl_hip_push_fetch)
-    uint64_t push_total = 0;
+    ocl_amd::PushWork pw[2];       // per fiber direction: the last push of that direction
+    int push_last = 0;             // direction of the last push (ocl_hip_push_fetch)
+    ocl_amd::DevBuf push_cnt, push_queue;
     size_t ntri = 0;
     double bbox[6] = {0, 0, 0, 0, 0, 0};
     unsigned bucket = 1;
@@ -102,17 +131,16 @@ struct ocl_mesh {
     size_t box_off[ocl_amd::kMaxLevels] = {};
     int top = 0;
     // per-call workspace
-    ocl_amd::DevBuf keys, hist, cursor, cl, out_z, out_cc, out_ty, out_tri, out_ft, out_dz, counters, fastrec;
-    ocl_amd::DevBuf esc_list, esc_list2, esc_list3, esc_flag, blocksum, qcount, cnt2, cnt3;
+    ocl_amd::DevBuf keys, hist, cl, out_z, out_cc, out_ty, out_tri, out_ft, out_dz;
+    ocl_amd::DropWork work;      // selection workspace and counters of the single calls
     ocl_amd::DevBuf origins;     // frame (BoxOrigin) per level-0 box
     ocl_amd::DevBuf boxzmax;     // highest vertex per level-0 box
     ocl_amd::DevBuf gr_part;     // backward: per-block partial sums of dz/dr
-    ocl_amd::DevBuf zfloor;      // per CL point: floor of the final z handed to the escalation passes
-    ocl_amd::DevBuf ntset, ntcnt, zband, selrec;  // per CL point: selection records and near-tie sets
-    int exact_slot = 0;          // queue-count slot read by the exact pass in the last call
     double zmid = 0, zhalf = 0, xyb = 0;
-    // cached selection records (rebuilt when the cutter or precision changes)
-    ocl_amd::RecSet rs32, rs64;
+    // selection records of the last cutter, per precision (shared with the
+    // batches that use them)
+    std::shared_ptr<ocl_amd::RecSet> rs32, rs64;
+    int batches = 0;             // live batches with a job on this mesh
     uint32_t xcd_group = 8;       // runs of 8 neighbouring CL tiles per XCD (OCL_XCD_GROUP overrides)
     uint32_t hilbert_bits = 9;   // CL binning: Hilbert curve over 2^bits x 2^bits cells
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_s0 = nullptr, ev_s1 = nullptr;
@@ -314,6 +342,7 @@ struct DropOut {
 struct Counters {
     unsigned long long pairs, escalated, brent, nesc, evals;  // evals: wavefront runs of the contact code
 };
+static_assert(sizeof(Counters) <= 64, "DropWork::ensure_counters sizes the counters at 64 bytes");
 
 __device__ inline TriX load_tri(const double* __restrict__ base, uint32_t k) {
     const double* r = base + (size_t)kRec * k;
@@ -1843,9 +1872,18 @@ static SelParams<T> make_params(const CutterDev& cd, const SelErr& er) {
     return sel_params<T>(cd, er);
 }
 
+// Makes slot hold the records of cutter cd: kept if they are already its,
+// rebuilt in place if nothing else holds them, else built in a new set (a
+// batch keeps replaying the old one).
 template <class T>
-static int ensure_records(ocl_mesh* m, const CutterDev& cd, RecSet& rs, const SelErr& er, hipStream_t st) {
-    if (rs.ok && std::memcmp(&rs.key, &cd, sizeof(cd)) == 0) return OCL_OK;
+static int ensure_records(ocl_mesh* m, const CutterDev& cd, std::shared_ptr<RecSet>& slot, const SelErr& er,
+                          hipStream_t st) {
+    if (slot && slot->ok && std::memcmp(&slot->key, &cd, sizeof(cd)) == 0) return OCL_OK;
+    if (!slot || slot.use_count() > 1) {
+        slot = std::make_shared<RecSet>();
+        if (!slot) return fail(OCL_E_NOMEM, "out of host memory");
+    }
+    RecSet& rs = *slot;
     rs.ok = false;
     const uint32_t n = (uint32_t)m->ntri, nbox = m->cnt[0];
     int rc;
@@ -1915,6 +1953,8 @@ static double env_scale(const char* name, double def) {
 // arguments of its passes.
 struct FastPlan {
     ocl_mesh* m;
+    DropWork* w;                      // the job's workspace and counters
+    std::shared_ptr<RecSet> r32, r64; // the job's selection records
     MeshView mv;
     CutterDev cd;
     const double* cl;
@@ -1933,17 +1973,18 @@ struct FastPlan {
 
 // Workspace and records of a job (ensure_records may synchronise the stream
 // once per mesh and cutter: call it before any graph capture).
-static int fast_prepare(ocl_mesh* m, const MeshView& mv, const CutterDev& cd, const double* d_cl,
+static int fast_prepare(ocl_mesh* m, DropWork& w, const MeshView& mv, const CutterDev& cd, const double* d_cl,
                         const uint32_t* perm, uint32_t n, int precision, const DropOut& o, Counters* cnt,
                         hipStream_t st, FastPlan& F) {
     int rc;
-    if ((rc = m->esc_list.ensure(4ull * n + 64)) || (rc = m->esc_list2.ensure(4ull * n + 64)) ||
-        (rc = m->esc_list3.ensure(4ull * n + 64)) ||
-        (rc = m->zfloor.ensure(8ull * n + 64)) || (rc = m->ntset.ensure(4ull * kNearTie * n + 64)) ||
-        (rc = m->ntcnt.ensure(2ull * n + 64)) || (rc = m->zband.ensure(8ull * n + 64)) ||
-        (rc = m->selrec.ensure(20ull * n + 64)))
+    if ((rc = w.esc_list.ensure(4ull * n + 64)) || (rc = w.esc_list2.ensure(4ull * n + 64)) ||
+        (rc = w.esc_list3.ensure(4ull * n + 64)) ||
+        (rc = w.zfloor.ensure(8ull * n + 64)) || (rc = w.ntset.ensure(4ull * kNearTie * n + 64)) ||
+        (rc = w.ntcnt.ensure(2ull * n + 64)) || (rc = w.zband.ensure(8ull * n + 64)) ||
+        (rc = w.selrec.ensure(20ull * n + 64)) || (rc = w.ensure_counters()))
         return rc;
     F.m = m;
+    F.w = &w;
     F.mv = mv;
     F.cd = cd;
     F.cl = d_cl;
@@ -1952,43 +1993,46 @@ static int fast_prepare(ocl_mesh* m, const MeshView& mv, const CutterDev& cd, co
     F.precision = precision;
     F.o = o;
     F.cnt = cnt;
-    F.zfloor = m->zfloor.as<double>();
-    F.nts.set = m->ntset.as<uint32_t>();
-    F.nts.cnt = m->ntcnt.as<uint8_t>();
+    F.zfloor = w.zfloor.as<double>();
+    F.nts.set = w.ntset.as<uint32_t>();
+    F.nts.cnt = w.ntcnt.as<uint8_t>();
     F.nts.bf = F.nts.cnt + n;
-    F.nts.zband = m->zband.as<double>();
-    F.nts.zsel = m->selrec.as<double>();
+    F.nts.zband = w.zband.as<double>();
+    F.nts.zsel = w.selrec.as<double>();
     F.nts.bt = reinterpret_cast<uint32_t*>(F.nts.zsel + n);
     F.nts.esel = reinterpret_cast<float*>(F.nts.bt + n);
-    F.q1 = m->esc_list.as<uint32_t>();
-    F.q2 = m->esc_list2.as<uint32_t>();
-    F.qnt = m->esc_list3.as<uint32_t>();
-    F.qcount = m->qcount.as<uint32_t>();
+    F.q1 = w.esc_list.as<uint32_t>();
+    F.q2 = w.esc_list2.as<uint32_t>();
+    F.qnt = w.esc_list3.as<uint32_t>();
+    F.qcount = w.qcount.as<uint32_t>();
     F.nts.q = F.qnt;
     F.nts.qn = F.qcount + 2;
     const double se = env_scale("OCL_SEL_E_SCALE", 1.0);
     const SelErr er64 = make_err<double>(cd, se);
     F.P64 = make_params<double>(cd, er64);
     if ((rc = ensure_records<double>(m, cd, m->rs64, er64, st))) return rc;
-    set_cut(F.P64, m->rs64);
+    F.r64 = m->rs64;
+    set_cut(F.P64, *F.r64);
     F.etie = (double)F.P64.E;
+    F.r32.reset();
     if (precision == OCL_PREC_FP32 || precision == OCL_PREC_MIXED) {
         const SelErr er32 = make_err<float>(cd, se);
         F.P32 = make_params<float>(cd, er32);
         if ((rc = ensure_records<float>(m, cd, m->rs32, er32, st))) return rc;
-        set_cut(F.P32, m->rs32);
+        F.r32 = m->rs32;
+        set_cut(F.P32, *F.r32);
         F.etie = (double)F.P32.E;
     }
     return OCL_OK;
 }
 
 static SelArgs<float> fast_args32(const FastPlan& F) {
-    return sel_args<float>(F.m, F.mv, F.m->rs32.rec.as<float>(), F.m->rs32.cap.as<float>(), F.P32, F.cd, F.cl,
+    return sel_args<float>(F.m, F.mv, F.r32->rec.as<float>(), F.r32->cap.as<float>(), F.P32, F.cd, F.cl,
                            F.perm, F.n, F.o, F.zfloor, F.nts, F.precision == OCL_PREC_MIXED ? 1 : 0, F.cnt);
 }
 
 static SelArgs<double> fast_args64(const FastPlan& F) {
-    return sel_args<double>(F.m, F.mv, F.m->rs64.rec.as<double>(), F.m->rs64.cap.as<double>(), F.P64, F.cd, F.cl,
+    return sel_args<double>(F.m, F.mv, F.r64->rec.as<double>(), F.r64->cap.as<double>(), F.P64, F.cd, F.cl,
                             F.perm, F.n, F.o, F.zfloor, F.nts, 1, F.cnt);
 }
 
@@ -1999,23 +2043,24 @@ static SelArgs<double> fast_args64(const FastPlan& F) {
 static int fast_post(const FastPlan& F, hipStream_t st) {
     if (F.precision == OCL_PREC_FP32) return OCL_OK;
     ocl_mesh* m = F.m;
+    DropWork& w = *F.w;
     const MeshView& mv = F.mv;
     const CutterDev& cd = F.cd;
     const uint32_t n = F.n;
     const bool mixed = F.precision == OCL_PREC_MIXED;
     k_drop_resolve<<<(n + 255) / 256, 256, 0, st>>>(mv, cd, F.cl, F.perm, n, F.nts, F.etie, F.o, F.q1, F.qcount,
-                                                     m->cnt2.as<Counters>());
+                                                     w.cnt2.as<Counters>());
     HIP_TRY(hipGetLastError());
     k_drop_neartie<<<kNearTieBlocks, 256, 0, st>>>(
-        mv, cd, F.cl, F.qnt, F.qcount + 2, F.nts, F.o, F.q1, F.qcount, m->cnt2.as<Counters>());
+        mv, cd, F.cl, F.qnt, F.qcount + 2, F.nts, F.o, F.q1, F.qcount, w.cnt2.as<Counters>());
     HIP_TRY(hipGetLastError());
     const uint32_t* exact_list = F.q1;
     const uint32_t* exact_n = F.qcount;
     const BoxOrigin* org = m->origins.as<BoxOrigin>();
-    const double* cap64 = m->rs64.cap.as<double>();
+    const double* cap64 = F.r64->cap.as<double>();
     if (mixed && cd.c.kind != K_CONE && env_scale("OCL_COOP_SELECT", 0.0) != 0.0) {
-        Counters* c2 = m->cnt2.as<Counters>();
-        const double* r64 = m->rs64.rec.as<double>();
+        Counters* c2 = w.cnt2.as<Counters>();
+        const double* r64 = F.r64->rec.as<double>();
         const SelParams<double>& P64 = F.P64;
         double* zfloor = F.zfloor;
         uint32_t *q1 = F.q1, *q2 = F.q2, *qcount = F.qcount;
@@ -2031,9 +2076,9 @@ static int fast_post(const FastPlan& F, hipStream_t st) {
         exact_n = F.qcount + 1;
     }
     k_drop_coop_exact<<<kCoopBlocks, 256, 0, st>>>(mv, cd, F.cl, exact_list, exact_n, F.zfloor, cap64, org, F.o,
-                                                   m->cnt3.as<Counters>());
+                                                   w.cnt3.as<Counters>());
     HIP_TRY(hipGetLastError());
-    m->exact_slot = (int)(exact_n - F.qcount);
+    w.exact_slot = (int)(exact_n - F.qcount);
     return OCL_OK;
 }
 
@@ -2041,7 +2086,7 @@ static int launch_drop_fast(ocl_mesh* m, const MeshView& mv, const CutterDev& cd
                             const uint32_t* perm, uint32_t n, int precision, const DropOut& o, Counters* cnt,
                             hipStream_t st, bool timed) {
     FastPlan F;
-    int rc = fast_prepare(m, mv, cd, d_cl, perm, n, precision, o, cnt, st, F);
+    int rc = fast_prepare(m, m->work, mv, cd, d_cl, perm, n, precision, o, cnt, st, F);
     if (rc) return rc;
     if (timed) HIP_TRY(hipEventRecord(m->ev_s0, st));
     const uint32_t nb = select_grid(n, mv.xcd_group);
@@ -2256,6 +2301,9 @@ int ocl_hip_mesh_create(const double* tri_xyz, size_t ntri, unsigned bucket, ocl
 
 int ocl_hip_mesh_destroy(ocl_mesh* mesh) {
     if (!mesh) return OCL_OK;
+    if (mesh->batches > 0)
+        return fail(OCL_E_ARG, "mesh is used by " + std::to_string(mesh->batches) +
+                                   " live batch(es): destroy them first");
     if (mesh->ev0) (void)hipEventDestroy(mesh->ev0);
     if (mesh->ev1) (void)hipEventDestroy(mesh->ev1);
     if (mesh->ev_s0) (void)hipEventDestroy(mesh->ev_s0);
@@ -2279,14 +2327,12 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
     if (n == 0) return OCL_OK;
     const uint32_t n32 = (uint32_t)n;
     DropOut o{d_z, d_cc, d_type, d_tri, d_feat};
-    if ((rc = m->counters.ensure(sizeof(Counters)))) return rc;
-    Counters* cnt = m->counters.as<Counters>();
-    if ((rc = m->cnt2.ensure(sizeof(Counters))) || (rc = m->cnt3.ensure(sizeof(Counters))) ||
-        (rc = m->qcount.ensure(16)))
-        return rc;
+    DropWork& w = m->work;
+    if ((rc = w.ensure_counters())) return rc;
+    Counters* cnt = w.counters.as<Counters>();
     // the counters and queue counts are cleared by k_cl_keys
-    const ZeroSet zs{{reinterpret_cast<uint32_t*>(cnt), m->cnt2.as<uint32_t>(), m->cnt3.as<uint32_t>(),
-                      m->qcount.as<uint32_t>()},
+    const ZeroSet zs{{reinterpret_cast<uint32_t*>(cnt), w.cnt2.as<uint32_t>(), w.cnt3.as<uint32_t>(),
+                      w.qcount.as<uint32_t>()},
                      {(uint32_t)(sizeof(Counters) / 4), (uint32_t)(sizeof(Counters) / 4),
                       (uint32_t)(sizeof(Counters) / 4), 4u}};
     // bin CL points
@@ -2337,14 +2383,14 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
         HIP_TRY(hipEventElapsedTime(&ms, m->ev0, m->ev1));
         Counters hc, h2, h3;
         HIP_TRY(hipMemcpy(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(&h2, m->cnt2.p, sizeof(h2), hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(&h3, m->cnt3.p, sizeof(h3), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&h2, w.cnt2.p, sizeof(h2), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&h3, w.cnt3.p, sizeof(h3), hipMemcpyDeviceToHost));
         uint32_t qc[4] = {0, 0, 0, 0};
-        if (fast && precision != OCL_PREC_FP32) HIP_TRY(hipMemcpy(qc, m->qcount.p, 16, hipMemcpyDeviceToHost));
+        if (fast && precision != OCL_PREC_FP32) HIP_TRY(hipMemcpy(qc, w.qcount.p, 16, hipMemcpyDeviceToHost));
         stats->points = n;
         stats->pairs = hc.pairs;
         stats->escalated = hc.escalated;
-        stats->escalated_exact = fast ? qc[m->exact_slot] : 0;
+        stats->escalated_exact = fast ? qc[w.exact_slot] : 0;
         stats->brent_iters = hc.brent + h2.brent + h3.brent;
         stats->wave_evals = hc.evals;
         stats->kernel_ms = ms;
@@ -2428,20 +2474,13 @@ static MeshView tree_view(const ocl_tree& T, size_t ntri) {
     return v;
 }
 
-extern "C" {
+namespace ocl_amd {
 
-int ocl_hip_push_batch(ocl_mesh* m, const ocl_cutter_desc* desc, int dir, const double* fib, size_t nf,
-                       int precision, uint64_t* offsets_out, uint64_t* total_out, ocl_stats* stats, void* stream_) {
-    if (!m) return fail(OCL_E_ARG, "mesh is NULL");
-    if (dir != 0 && dir != 1) return fail(OCL_E_ARG, "dir must be 0 (X fibers) or 1 (Y fibers)");
-    if (nf > 0 && (!fib || !offsets_out)) return fail(OCL_E_ARG, "fiber/offset pointers are NULL");
+// Checks one direction's fibers and uploads them, sorted by (z, across) so
+// that neighbouring wavefronts take neighbouring fibers.
+static int push_side_prepare(ocl_mesh* m, int dir, const double* fib, size_t nf, hipStream_t st) {
+    if (nf > 0 && !fib) return fail(OCL_E_ARG, "fiber pointer is NULL");
     if (nf >= 0x7fffffffu) return fail(OCL_E_ARG, "too many fibers");
-    const int superset = (precision & OCL_PUSH_EXACT_FILTER) ? 0 : 1;
-    precision &= ~OCL_PUSH_EXACT_FILTER;
-    if (precision < OCL_PREC_EXACT64 || precision > OCL_PREC_MIXED) return fail(OCL_E_ARG, "unknown precision mode");
-    CutterDev cd;
-    int rc = cutter_from_desc(desc, cd);
-    if (rc) return rc;
     // Fiber::calcDir asserts a horizontal fiber; the facet push tests
     // p1.y == p2.y (X fiber) or p1.x == p2.x (Y fiber)
     for (size_t i = 0; i < nf; ++i) {
@@ -2450,25 +2489,16 @@ int ocl_hip_push_batch(ocl_mesh* m, const ocl_cutter_desc* desc, int dir, const 
         if (dir == 0 ? (q[1] != q[4]) : (q[0] != q[3]))
             return fail(OCL_E_ARG, "fiber " + std::to_string(i) + " does not run along the requested axis");
     }
-    hipStream_t st = (hipStream_t)stream_;
-    if (stats) std::memset(stats, 0, sizeof(*stats));
-    if (total_out) *total_out = 0;
-    m->push_nf = (uint32_t)nf;
-    m->push_total = 0;
-    if (nf == 0) {
-        if (offsets_out) offsets_out[0] = 0;
-        return OCL_OK;
-    }
-    if (m->ntri == 0) {
-        for (size_t i = 0; i <= nf; ++i) offsets_out[i] = 0;
-        return OCL_OK;
-    }
+    PushWork& W = m->pw[dir];
+    W.nf = (uint32_t)nf;
+    W.total = 0;
+    if (nf == 0 || m->ntri == 0) return OCL_OK;
     ocl_tree& T = m->push_tree[dir];
     if (!T.built) {
         static const int kYZ[4] = {2, 3, 4, 5}, kXZ[4] = {0, 1, 4, 5};
-        if ((rc = build_tree(m, T, dir == 0 ? kYZ : kXZ, dir == 0 ? 1 : 0))) return rc;
+        int rc = build_tree(m, T, dir == 0 ? kYZ : kXZ, dir == 0 ? 1 : 0);
+        if (rc) return rc;
     }
-    // wavefronts of neighbouring fibers: sort by (z, across)
     std::vector<uint32_t> perm(nf);
     for (size_t i = 0; i < nf; ++i) perm[i] = (uint32_t)i;
     const int ca = dir == 0 ? 1 : 0;
@@ -2477,88 +2507,227 @@ int ocl_hip_push_batch(ocl_mesh* m, const ocl_cutter_desc* desc, int dir, const 
         if (za != zb) return za < zb;
         return fib[6 * (size_t)a + ca] < fib[6 * (size_t)b + ca];
     });
-    if ((rc = m->push_fib.ensure(48ull * nf)) || (rc = m->push_perm.ensure(4ull * nf)) ||
-        (rc = m->push_n.ensure(4ull * nf + 64)) || (rc = m->push_ovf.ensure(64)) ||
-        (rc = m->push_cnt.ensure(sizeof(PushCounters))))
+    int rc;
+    if ((rc = W.fib.ensure(48ull * nf)) || (rc = W.perm.ensure(4ull * nf)) || (rc = W.n.ensure(4ull * nf + 64)) ||
+        (rc = W.ovf.ensure(64)))
         return rc;
-    HIP_TRY(hipMemcpyAsync(m->push_fib.p, fib, 48ull * nf, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(m->push_perm.p, perm.data(), 4ull * nf, hipMemcpyHostToDevice, st));
-    MeshView pm = tree_view(T, m->ntri);
-    if (superset) pm.tree.box = T.kdboxes.as<const float4>();  // walk the KD acceptance boxes
-    PushCounters hc{};
-    float ms = 0;
-    for (;;) {
-        if ((rc = m->push_e.ensure(sizeof(PEntry) * (size_t)m->push_cap * nf))) return rc;
-        PushLists PL{m->push_e.as<PEntry>(), m->push_n.as<uint32_t>(), m->push_cap, m->push_ovf.as<uint32_t>(),
-                     m->push_ovf.as<uint32_t>() + 1};
-        HIP_TRY(hipMemsetAsync(m->push_ovf.p, 0, 8, st));  // overflow count and fiber queue
-        HIP_TRY(hipMemsetAsync(m->push_cnt.p, 0, sizeof(PushCounters), st));
-        if (!m->ev0) HIP_TRY(hipEventCreate(&m->ev0));
-        if (!m->ev1) HIP_TRY(hipEventCreate(&m->ev1));
+    HIP_TRY(hipMemcpyAsync(W.fib.p, fib, 48ull * nf, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(W.perm.p, perm.data(), 4ull * nf, hipMemcpyHostToDevice, st));
+    // capacity per fiber: what the previous push in this direction needed
+    // (16 unless it had a longer list), times 4 on each overflow
+    W.cap = W.cap_next;
+    return OCL_OK;
+}
+
+static PushSide push_side(ocl_mesh* m, int dir, int superset) {
+    PushWork& W = m->pw[dir];
+    const ocl_tree& T = m->push_tree[dir];
+    PushSide S;
+    S.pm = tree_view(T, m->ntri);
+    if (superset) S.pm.tree.box = T.kdboxes.as<const float4>();  // walk the KD acceptance boxes
+    S.acc = T.acc.as<const double>();
+    S.fib = W.fib.as<const double>();
+    S.perm = W.perm.as<const uint32_t>();
+    S.nf = W.nf;
+    S.dir = dir;
+    S.PL = PushLists{W.e.as<PEntry>(), W.n.as<uint32_t>(), W.cap, W.ovf.as<uint32_t>()};
+    return S;
+}
+
+// Pushes the directions dirs[0..nd) in one launch per attempt; a direction
+// whose lists overflowed runs again with 4x the capacity.  Adds the device
+// time of every attempt to *ms and the counters to *hc.
+static int push_run(ocl_mesh* m, const CutterDev& cd, int superset, const int* dirs, int nd, hipStream_t st,
+                    float* ms, PushCounters* hc) {
+    int rc;
+    if ((rc = m->push_cnt.ensure(sizeof(PushCounters))) || (rc = m->push_queue.ensure(64))) return rc;
+    HIP_TRY(hipMemsetAsync(m->push_cnt.p, 0, sizeof(PushCounters), st));
+    if (!m->ev0) HIP_TRY(hipEventCreate(&m->ev0));
+    if (!m->ev1) HIP_TRY(hipEventCreate(&m->ev1));
+    int todo[2] = {dirs[0], nd > 1 ? dirs[1] : -1};
+    int nt = nd;
+    while (nt > 0) {
+        PushSide S[2];
+        uint32_t nfib = 0;
+        for (int i = 0; i < nt; ++i) {
+            PushWork& W = m->pw[todo[i]];
+            if ((rc = W.e.ensure(sizeof(PEntry) * (size_t)W.cap * W.nf))) return rc;
+            HIP_TRY(hipMemsetAsync(W.ovf.p, 0, 4, st));
+            S[i] = push_side(m, todo[i], superset);
+            nfib += W.nf;
+        }
+        if (nt == 1) S[1] = S[0];
+        HIP_TRY(hipMemsetAsync(m->push_queue.p, 0, 4, st));
         HIP_TRY(hipEventRecord(m->ev0, st));
-        const uint32_t nb = (uint32_t)std::min<size_t>((nf + kPushWaves - 1) / kPushWaves, 2048);
-        const double* fd = m->push_fib.as<double>();
-        const uint32_t* pp = m->push_perm.as<uint32_t>();
+        const uint32_t nb = (uint32_t)std::min<size_t>((nfib + kPushWaves - 1) / kPushWaves, 2048);
+        uint32_t* q = m->push_queue.as<uint32_t>();
         PushCounters* pc = m->push_cnt.as<PushCounters>();
-        const double* ac = T.acc.as<double>();
+        const uint32_t ns = (uint32_t)nt;
         switch (cd.c.kind) {
-        case K_CYL: k_push_fibers<K_CYL><<<nb, 256, 0, st>>>(pm, ac, superset, cd, fd, pp, (uint32_t)nf, dir, PL, pc); break;
-        case K_BALL: k_push_fibers<K_BALL><<<nb, 256, 0, st>>>(pm, ac, superset, cd, fd, pp, (uint32_t)nf, dir, PL, pc); break;
-        case K_BULL: k_push_fibers<K_BULL><<<nb, 256, 0, st>>>(pm, ac, superset, cd, fd, pp, (uint32_t)nf, dir, PL, pc); break;
-        case K_CONE: k_push_fibers<K_CONE><<<nb, 256, 0, st>>>(pm, ac, superset, cd, fd, pp, (uint32_t)nf, dir, PL, pc); break;
-        default: k_push_fibers<K_COMPOSITE><<<nb, 256, 0, st>>>(pm, ac, superset, cd, fd, pp, (uint32_t)nf, dir, PL, pc); break;
+        case K_CYL: k_push_fibers<K_CYL><<<nb, 256, 0, st>>>(S[0], S[1], ns, q, superset, cd, pc); break;
+        case K_BALL: k_push_fibers<K_BALL><<<nb, 256, 0, st>>>(S[0], S[1], ns, q, superset, cd, pc); break;
+        case K_BULL: k_push_fibers<K_BULL><<<nb, 256, 0, st>>>(S[0], S[1], ns, q, superset, cd, pc); break;
+        case K_CONE: k_push_fibers<K_CONE><<<nb, 256, 0, st>>>(S[0], S[1], ns, q, superset, cd, pc); break;
+        default: k_push_fibers<K_COMPOSITE><<<nb, 256, 0, st>>>(S[0], S[1], ns, q, superset, cd, pc); break;
         }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(m->ev1, st));
-        uint32_t ovf = 0;
-        HIP_TRY(hipMemcpyAsync(&ovf, m->push_ovf.p, 4, hipMemcpyDeviceToHost, st));
+        uint32_t ovf[2] = {0, 0};
+        for (int i = 0; i < nt; ++i)
+            HIP_TRY(hipMemcpyAsync(&ovf[i], m->pw[todo[i]].ovf.p, 4, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
-        HIP_TRY(hipEventElapsedTime(&ms, m->ev0, m->ev1));
-        if (ovf == 0) break;
-        if (m->push_cap >= 4096) return fail(OCL_E_UNSUPPORTED, "push: more than 4096 intervals on one fiber");
-        m->push_cap *= 4;
+        float t = 0;
+        HIP_TRY(hipEventElapsedTime(&t, m->ev0, m->ev1));
+        *ms += t;
+        int again = 0;
+        for (int i = 0; i < nt; ++i) {
+            if (!ovf[i]) continue;
+            PushWork& W = m->pw[todo[i]];
+            if (W.cap >= 4096) return fail(OCL_E_UNSUPPORTED, "push: more than 4096 intervals on one fiber");
+            W.cap *= 4;
+            todo[again++] = todo[i];
+        }
+        nt = again;
     }
-    HIP_TRY(hipMemcpy(&hc, m->push_cnt.p, sizeof(hc), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hc, m->push_cnt.p, sizeof(*hc), hipMemcpyDeviceToHost));
+    return OCL_OK;
+}
+
+// Row offsets of one direction's last push (host and device), its total,
+// and the capacity the next push in this direction starts from.
+static int push_side_finish(ocl_mesh* m, int dir, uint64_t* offsets_out) {
+    PushWork& W = m->pw[dir];
+    const size_t nf = W.nf;
+    if (nf == 0 || m->ntri == 0) {
+        for (size_t i = 0; i <= nf; ++i) offsets_out[i] = 0;
+        W.total = 0;
+        return OCL_OK;
+    }
     std::vector<uint32_t> cnt(nf);
-    HIP_TRY(hipMemcpy(cnt.data(), m->push_n.p, 4ull * nf, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cnt.data(), W.n.p, 4ull * nf, hipMemcpyDeviceToHost));
     uint64_t run = 0;
+    uint32_t longest = 0;
     for (size_t i = 0; i < nf; ++i) {
         offsets_out[i] = run;
         run += cnt[i];
+        longest = std::max(longest, cnt[i]);
     }
     offsets_out[nf] = run;
-    m->push_total = run;
-    if (total_out) *total_out = run;
-    if ((rc = m->push_offs.ensure(8ull * (nf + 1)))) return rc;
-    HIP_TRY(hipMemcpy(m->push_offs.p, offsets_out, 8ull * (nf + 1), hipMemcpyHostToDevice));
-    if (stats) {
-        stats->points = nf;
-        stats->pairs = hc.pairs;
-        stats->intervals = run;
-        stats->brent_iters = hc.brent;
-        stats->kernel_ms = ms;
-        stats->select_ms = ms;
+    W.total = run;
+    W.cap_next = 16;
+    while (W.cap_next < longest) W.cap_next *= 4;
+    int rc;
+    if ((rc = W.offs.ensure(8ull * (nf + 1)))) return rc;
+    HIP_TRY(hipMemcpy(W.offs.p, offsets_out, 8ull * (nf + 1), hipMemcpyHostToDevice));
+    return OCL_OK;
+}
+
+static int push_precision(int& precision, int& superset) {
+    superset = (precision & OCL_PUSH_EXACT_FILTER) ? 0 : 1;
+    precision &= ~OCL_PUSH_EXACT_FILTER;
+    if (precision < OCL_PREC_EXACT64 || precision > OCL_PREC_MIXED) return fail(OCL_E_ARG, "unknown precision mode");
+    return OCL_OK;
+}
+
+static void push_stats(ocl_stats* stats, uint64_t fibers, uint64_t rows, const PushCounters& hc, float ms) {
+    if (!stats) return;
+    stats->points = fibers;
+    stats->pairs = hc.pairs;
+    stats->intervals = rows;
+    stats->brent_iters = hc.brent;
+    stats->kernel_ms = ms;
+    stats->select_ms = ms;
+}
+
+static int push_fetch_side(ocl_mesh* m, int dir, double* lohi, double* cc, hipStream_t st) {
+    PushWork& W = m->pw[dir];
+    const uint64_t total = W.total;
+    if (total == 0) return OCL_OK;
+    if (!lohi || !cc) return fail(OCL_E_ARG, "output pointers are NULL");
+    int rc;
+    if ((rc = W.lohi.ensure(16ull * total)) || (rc = W.cc.ensure(64ull * total))) return rc;
+    PushLists PL{W.e.as<PEntry>(), W.n.as<uint32_t>(), W.cap, W.ovf.as<uint32_t>()};
+    const uint32_t nf = W.nf;
+    k_push_gather<<<(nf + 255) / 256, 256, 0, st>>>(PL, W.offs.as<uint64_t>(), nf, W.lohi.as<double>(),
+                                                    W.cc.as<double>());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(lohi, W.lohi.p, 16ull * total, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(cc, W.cc.p, 64ull * total, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return OCL_OK;
+}
+
+}  // namespace ocl_amd
+
+extern "C" {
+
+int ocl_hip_push_batch(ocl_mesh* m, const ocl_cutter_desc* desc, int dir, const double* fib, size_t nf,
+                       int precision, uint64_t* offsets_out, uint64_t* total_out, ocl_stats* stats, void* stream_) {
+    using namespace ocl_amd;
+    if (!m) return fail(OCL_E_ARG, "mesh is NULL");
+    if (dir != 0 && dir != 1) return fail(OCL_E_ARG, "dir must be 0 (X fibers) or 1 (Y fibers)");
+    if (nf > 0 && !offsets_out) return fail(OCL_E_ARG, "offset pointer is NULL");
+    int superset, rc;
+    if ((rc = push_precision(precision, superset))) return rc;
+    CutterDev cd;
+    if ((rc = cutter_from_desc(desc, cd))) return rc;
+    hipStream_t st = (hipStream_t)stream_;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (total_out) *total_out = 0;
+    m->push_last = dir;
+    if ((rc = push_side_prepare(m, dir, fib, nf, st))) return rc;
+    if (nf == 0) {
+        if (offsets_out) offsets_out[0] = 0;
+        return OCL_OK;
     }
+    PushCounters hc{};
+    float ms = 0;
+    if (m->ntri > 0 && (rc = push_run(m, cd, superset, &dir, 1, st, &ms, &hc))) return rc;
+    if ((rc = push_side_finish(m, dir, offsets_out))) return rc;
+    if (total_out) *total_out = m->pw[dir].total;
+    push_stats(stats, nf, m->pw[dir].total, hc, ms);
+    return OCL_OK;
+}
+
+int ocl_hip_push_xy(ocl_mesh* m, const ocl_cutter_desc* desc, const double* xfib, size_t nx, const double* yfib,
+                    size_t ny, int precision, uint64_t* x_offsets_out, uint64_t* y_offsets_out, uint64_t* totals_out,
+                    ocl_stats* stats, void* stream_) {
+    using namespace ocl_amd;
+    if (!m) return fail(OCL_E_ARG, "mesh is NULL");
+    if ((nx > 0 && !x_offsets_out) || (ny > 0 && !y_offsets_out)) return fail(OCL_E_ARG, "offset pointers are NULL");
+    int superset, rc;
+    if ((rc = push_precision(precision, superset))) return rc;
+    CutterDev cd;
+    if ((rc = cutter_from_desc(desc, cd))) return rc;
+    hipStream_t st = (hipStream_t)stream_;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (totals_out) totals_out[0] = totals_out[1] = 0;
+    if ((rc = push_side_prepare(m, 0, xfib, nx, st)) || (rc = push_side_prepare(m, 1, yfib, ny, st))) return rc;
+    int dirs[2], nd = 0;
+    if (nx > 0) dirs[nd++] = 0;
+    if (ny > 0) dirs[nd++] = 1;
+    PushCounters hc{};
+    float ms = 0;
+    if (nd > 0 && m->ntri > 0 && (rc = push_run(m, cd, superset, dirs, nd, st, &ms, &hc))) return rc;
+    if (x_offsets_out && (rc = push_side_finish(m, 0, x_offsets_out))) return rc;
+    if (y_offsets_out && (rc = push_side_finish(m, 1, y_offsets_out))) return rc;
+    if (totals_out) {
+        totals_out[0] = m->pw[0].total;
+        totals_out[1] = m->pw[1].total;
+    }
+    push_stats(stats, nx + ny, m->pw[0].total + m->pw[1].total, hc, ms);
     return OCL_OK;
 }
 
 int ocl_hip_push_fetch(ocl_mesh* m, double* lohi, double* cc, void* stream_) {
-    if (!m) return fail(OCL_E_ARG, "mesh is NULL");
-    const uint64_t total = m->push_total;
-    if (total == 0) return OCL_OK;
-    if (!lohi || !cc) return fail(OCL_E_ARG, "output pointers are NULL");
-    hipStream_t st = (hipStream_t)stream_;
-    int rc;
-    if ((rc = m->push_lohi.ensure(16ull * total)) || (rc = m->push_cc.ensure(64ull * total))) return rc;
-    PushLists PL{m->push_e.as<PEntry>(), m->push_n.as<uint32_t>(), m->push_cap, m->push_ovf.as<uint32_t>()};
-    const uint32_t nf = m->push_nf;
-    k_push_gather<<<(nf + 255) / 256, 256, 0, st>>>(PL, m->push_offs.as<uint64_t>(), nf, m->push_lohi.as<double>(),
-                                                    m->push_cc.as<double>());
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(lohi, m->push_lohi.p, 16ull * total, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(cc, m->push_cc.p, 64ull * total, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    return OCL_OK;
+    if (!m) return ocl_amd::fail(OCL_E_ARG, "mesh is NULL");
+    return ocl_amd::push_fetch_side(m, m->push_last, lohi, cc, (hipStream_t)stream_);
+}
+
+int ocl_hip_push_fetch_dir(ocl_mesh* m, int dir, double* lohi, double* cc, void* stream_) {
+    if (!m) return ocl_amd::fail(OCL_E_ARG, "mesh is NULL");
+    if (dir != 0 && dir != 1) return ocl_amd::fail(OCL_E_ARG, "dir must be 0 (X fibers) or 1 (Y fibers)");
+    return ocl_amd::push_fetch_side(m, dir, lohi, cc, (hipStream_t)stream_);
 }
 
 int ocl_hip_drop_batch(ocl_mesh* m, const ocl_cutter_desc* desc, const double* cl, size_t n, int precision,
@@ -2797,12 +2966,13 @@ __global__ void k_zero_ranges(const ZeroRange* __restrict__ r) {
 
 struct BatchJob {
     ocl_mesh* m;
+    std::unique_ptr<DropWork> w;  // the job's own workspace and counters
     CutterDev cd;
     const double* cl;
     uint32_t n, base;
     DropOut o;
     bool fast;
-    FastPlan F;
+    FastPlan F;                   // holds the job's selection records
     MeshView mv;
 };
 
@@ -2822,6 +2992,7 @@ struct ocl_batch {
     hipGraphExec_t exec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr, s0 = nullptr, s1 = nullptr;
     ~ocl_batch() {
+        for (const ocl_amd::BatchJob& J : jobs) --J.m->batches;
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (cap) (void)hipStreamDestroy(cap);
@@ -2941,12 +3112,11 @@ int ocl_hip_batch_create(const ocl_drop_job* jobs, size_t njobs, int precision, 
     for (size_t j = 0; j < njobs; ++j) {
         const ocl_drop_job& q = jobs[j];
         if (!q.mesh) return fail(OCL_E_ARG, "job " + std::to_string(j) + ": mesh is NULL");
-        for (size_t k = 0; k < j; ++k)
-            if (jobs[k].mesh == q.mesh)
-                return fail(OCL_E_ARG, "job " + std::to_string(j) + ": a mesh can appear once per batch");
         if (q.n > 0 && (!q.d_cl_xyz || !q.d_z_out)) return fail(OCL_E_ARG, "job " + std::to_string(j) + ": cl/z are NULL");
         BatchJob J;
         J.m = q.mesh;
+        J.w.reset(new (std::nothrow) DropWork());
+        if (!J.w) return fail(OCL_E_NOMEM, "out of host memory");
         int rc = cutter_from_desc(&q.cutter, J.cd);
         if (rc) return rc;
         J.cl = q.d_cl_xyz;
@@ -2956,13 +3126,12 @@ int ocl_hip_batch_create(const ocl_drop_job* jobs, size_t njobs, int precision, 
         J.mv = q.mesh->view();
         J.fast = !(precision == OCL_PREC_EXACT64 || q.mesh->ntri == 0 || J.cd.c.kind == K_COMPOSITE);
         ocl_mesh* m = q.mesh;
-        if ((rc = m->counters.ensure(sizeof(Counters))) || (rc = m->cnt2.ensure(sizeof(Counters))) ||
-            (rc = m->cnt3.ensure(sizeof(Counters))) || (rc = m->qcount.ensure(16)))
-            return rc;
-        zr.push_back({m->counters.as<uint32_t>(), (uint32_t)(sizeof(Counters) / 4)});
-        zr.push_back({m->cnt2.as<uint32_t>(), (uint32_t)(sizeof(Counters) / 4)});
-        zr.push_back({m->cnt3.as<uint32_t>(), (uint32_t)(sizeof(Counters) / 4)});
-        zr.push_back({m->qcount.as<uint32_t>(), 4u});
+        DropWork& w = *J.w;
+        if ((rc = w.ensure_counters())) return rc;
+        zr.push_back({w.counters.as<uint32_t>(), (uint32_t)(sizeof(Counters) / 4)});
+        zr.push_back({w.cnt2.as<uint32_t>(), (uint32_t)(sizeof(Counters) / 4)});
+        zr.push_back({w.cnt3.as<uint32_t>(), (uint32_t)(sizeof(Counters) / 4)});
+        zr.push_back({w.qcount.as<uint32_t>(), 4u});
         const double R = J.cd.c.radius;
         KeyJob& k = kj[j];
         k.cl = q.d_cl_xyz;
@@ -2974,7 +3143,8 @@ int ocl_hip_batch_create(const ocl_drop_job* jobs, size_t njobs, int precision, 
         k.f.sx = side / std::max(m->bbox[1] + R - k.f.x0, 1e-9);
         k.f.sy = side / std::max(m->bbox[3] + R - k.f.y0, 1e-9);
         total += q.n;
-        b->jobs.push_back(J);
+        ++m->batches;  // released by ~ocl_batch
+        b->jobs.push_back(std::move(J));
     }
     if (total >= 0x7fffffffu) return fail(OCL_E_ARG, "too many CL points in one batch");
     b->total = (uint32_t)total;
@@ -2995,8 +3165,8 @@ int ocl_hip_batch_create(const ocl_drop_job* jobs, size_t njobs, int precision, 
     std::vector<uint32_t> sel;
     for (BatchJob& J : b->jobs) {
         if (!J.fast || J.n == 0) continue;
-        if ((rc = fast_prepare(J.m, J.mv, J.cd, J.cl, b->sorted_vals + J.base, J.n, precision, J.o,
-                               J.m->counters.as<Counters>(), st, J.F)))
+        if ((rc = fast_prepare(J.m, *J.w, J.mv, J.cd, J.cl, b->sorted_vals + J.base, J.n, precision, J.o,
+                               J.w->counters.as<Counters>(), st, J.F)))
             return rc;
         sel.push_back((uint32_t)(&J - b->jobs.data()));
     }
@@ -3029,10 +3199,10 @@ int ocl_hip_batch_create(const ocl_drop_job* jobs, size_t njobs, int precision, 
             const FastPlan& F = J.F;
             ocl_mesh* m = F.m;
             pj.push_back(PostJob{F.mv, F.cd, F.cl, F.perm, F.n, F.nts, F.etie, F.o, F.q1, F.qcount, F.zfloor,
-                                 m->rs64.cap.as<double>(), m->origins.as<BoxOrigin>(), m->cnt2.as<Counters>(),
-                                 m->cnt3.as<Counters>()});
+                                 F.r64->cap.as<double>(), m->origins.as<BoxOrigin>(), J.w->cnt2.as<Counters>(),
+                                 J.w->cnt3.as<Counters>()});
             poff.push_back(poff.back() + (F.n + 63) / 64);
-            m->exact_slot = 0;
+            J.w->exact_slot = 0;
         }
         b->npost = (uint32_t)pj.size();
         b->postchunks = poff.back();
@@ -3051,7 +3221,7 @@ int ocl_hip_batch_create(const ocl_drop_job* jobs, size_t njobs, int precision, 
         std::vector<uint32_t> eoff(1, 0);
         for (uint32_t j : ex) {
             const BatchJob& J = b->jobs[j];
-            ej.push_back(ExactJob{J.mv, J.cd, J.cl, b->sorted_vals + J.base, J.n, J.o, J.m->counters.as<Counters>()});
+            ej.push_back(ExactJob{J.mv, J.cd, J.cl, b->sorted_vals + J.base, J.n, J.o, J.w->counters.as<Counters>()});
             eoff.push_back(eoff.back() + (J.n + 63) / 64);
         }
         b->nexact = (uint32_t)ej.size();
@@ -3115,7 +3285,7 @@ int ocl_hip_batch_run(ocl_batch* b, ocl_stats* stats, void* stream_) {
         stats->points = b->total;
         for (const BatchJob& J : b->jobs) {
             Counters hc;
-            HIP_TRY(hipMemcpy(&hc, J.m->counters.p, sizeof(hc), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(&hc, J.w->counters.p, sizeof(hc), hipMemcpyDeviceToHost));
             stats->pairs += hc.pairs;
             stats->escalated += hc.escalated;
             stats->wave_evals += hc.evals;

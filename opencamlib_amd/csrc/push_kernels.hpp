@@ -26,7 +26,19 @@ struct PushLists {
     uint32_t* n;        // entries per fiber
     uint32_t cap;
     uint32_t* overflow; // fibers whose list overflowed
-    uint32_t* queue;    // next fiber (sorted position) to take; 0 at launch
+};
+
+// One fiber direction of a push launch: the direction's tree (rank order
+// and boxes), the KD acceptance bounds, its fibers in the order they are
+// taken (perm: sorted by z, then across) and its interval lists.
+struct PushSide {
+    MeshView pm;
+    const double* acc;
+    const double* fib;
+    const uint32_t* perm;
+    uint32_t nf;
+    int dir;  // 0: X fibers (across = y), 1: Y fibers (across = x)
+    PushLists PL;
 };
 
 OCL_HD PEntry pentry_of(const PIval& i) {
@@ -94,7 +106,9 @@ struct PushCounters {
 // the fiber's search box, gathers the candidate triangles in rank order in
 // batches of 64 (one per lane), computes the 64 pair intervals in parallel
 // (FP64) and lane 0 merges them into the fiber's list in rank order.
-// dir 0: X fibers (across = y), 1: Y fibers (across = x); K = cutter kind.
+// K = cutter kind.  One launch serves one or two sides (fiber directions,
+// Waterline's X and Y fibers): the sides' fibers form one queue, so the
+// second direction fills the GPU while the first one's last fibers finish.
 //
 // Candidates (superset = 1, the reference's behaviour): every triangle the
 // reference's KD search returns for the fiber's search box, reproduced
@@ -124,10 +138,9 @@ constexpr int kBullSolves = 6; // Brent solves per pair: 3 edges x 2 roots
 constexpr uint32_t kLdsList = 32;
 
 template <int K>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_push_fibers(MeshView pm, const double* __restrict__ acc, int superset,
-                                                     CutterDev cd, const double* __restrict__ fib,
-                                                     const uint32_t* __restrict__ perm, uint32_t nf, int dir,
-                                                     PushLists PL, PushCounters* cnt) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_push_fibers(PushSide s0, PushSide s1, uint32_t nsides, uint32_t* __restrict__ queue, int superset, CutterDev cd,
+              PushCounters* cnt) {
     constexpr int kS = (K == K_BULL) ? kBullSolves : 1;
     __shared__ uint32_t qmem[kPushWaves][128];
     __shared__ PEntry emem[kPushWaves][64];
@@ -141,16 +154,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     PEntry* ebuf = emem[wv];
     const double R = cd.c.radius, L = cd.c.length;
     unsigned long long pairs = 0, brent = 0, nint = 0;
+    const uint32_t total = s0.nf + (nsides > 1 ? s1.nf : 0u);
     // fibers cost very different amounts (a fiber through a sphere against
     // one that misses every triangle), so each wavefront takes the next
     // fiber of the sorted order from a device counter when it is done,
     // instead of a fixed share
     for (;;) {
         uint32_t w = 0;
-        if (lane == 0) w = atomicAdd(PL.queue, 1u);
+        if (lane == 0) w = atomicAdd(queue, 1u);
         w = uniform_u((uint32_t)__shfl((int)w, 0, 64));
-        if (w >= nf) break;
-        const uint32_t src = uniform_u(perm[w]);
+        if (w >= total) break;
+        const bool second = w >= s0.nf;
+        const PushSide& S = second ? s1 : s0;
+        const MeshView pm = S.pm;
+        const double* __restrict__ acc = S.acc;
+        const int dir = S.dir;
+        const PushLists PL = S.PL;
+        const uint32_t src = uniform_u(S.perm[second ? w - s0.nf : w]);
+        const double* __restrict__ fib = S.fib;
         const FibX f = make_fiber(fib + 6 * (size_t)src);
         const double across = (dir == 0) ? f.p1.y : f.p1.x;
         // search box (batchpushcutter.cpp:170-181), in (across, z)
@@ -286,12 +307,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             const bool valid = key < INFINITY;
             const double hi_s = __shfl(keep ? iv.hi : -INFINITY, (int)idx, 64);
             // upper ends before each position: an exclusive prefix max
-            double pm = valid ? hi_s : -INFINITY;
+            double pmax = valid ? hi_s : -INFINITY;
             for (int off = 1; off < 64; off <<= 1) {
-                const double o = __shfl_up(pm, off, 64);
-                if ((int)lane >= off) pm = fmax(pm, o);
+                const double o = __shfl_up(pmax, off, 64);
+                if ((int)lane >= off) pmax = fmax(pmax, o);
             }
-            double before = __shfl_up(pm, 1, 64);
+            double before = __shfl_up(pmax, 1, 64);
             if (lane == 0) before = -INFINITY;
             const bool start = valid && key > before;
             const unsigned long long sb = __ballot(start);

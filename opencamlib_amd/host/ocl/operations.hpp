@@ -198,12 +198,15 @@ class PathDropCutter : public Operation {
 //
 // setChordTolerance(tol > 0) replaces the flatness test by slope-driven
 // refinement: every sample carries the slope of z along the path (from the
-// device backward pass).  An interval is closed when the piecewise cubic
+// device backward pass).  An interval passes when the piecewise cubic
 // Hermite curve through its two ends and its middle (heights and slopes)
 // stays within tol/2 of the chord (the other half is the margin for what
-// three samples cannot see), or at the minimum step; otherwise it is
-// split at the middle.  The first samples are spaced at most `sampling`
-// apart; each level of intervals is one device batch.
+// the samples cannot see); it is then checked at its quarter points, and
+// closes when both lie within tol/2 of the chord.  Otherwise it is split
+// at the middle (after a failed check, the quarter points are the halves'
+// middles).  Intervals at the minimum step close.  The first samples are
+// spaced at most `sampling` apart; each level of intervals is one device
+// batch.
 class AdaptivePathDropCutter : public PathDropCutter {
   public:
     void setMinSampling(double s) { min_sampling = s; }
@@ -224,7 +227,7 @@ class AdaptivePathDropCutter : public PathDropCutter {
 // Waterline fibers (waterline.cpp:67-87, 143-178): X fibers at every y and
 // Y fibers at every x of a grid spanning the surface's bounding box grown by
 // twice the cutter radius, at height z, `sampling` apart, pushed on the
-// device (both directions on one mesh handle).  The weave that turns the
+// device, both directions in one launch (ocl_hip_push_xy).  The weave that turns the
 // fibers into loops is graph work the reference does with Boost Graph and is
 // not built here: getLoops() throws.
 class Waterline : public Operation {

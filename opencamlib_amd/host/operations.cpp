@@ -191,23 +191,24 @@ void PointDropCutter::run(CLPoint& cl) {
 
 // Pushes fibers (all along one axis: dir 0 = X, 1 = Y) on the device and
 // stores each fiber's intervals, merged into any it already held.
-static void push_fibers(ocl_mesh* m, const MillingCutter& c, int dir, std::vector<Fiber>& fibers, int precision,
-                        bool exact_filter, ocl_stats* st) {
-    const size_t nf = fibers.size();
-    std::vector<double> fib(6 * nf);
-    for (size_t i = 0; i < nf; ++i) {
+static std::vector<double> fiber_array(const std::vector<Fiber>& fibers) {
+    std::vector<double> fib(6 * fibers.size());
+    for (size_t i = 0; i < fibers.size(); ++i) {
         const Fiber& f = fibers[i];
         const double v[6] = {f.p1.x, f.p1.y, f.p1.z, f.p2.x, f.p2.y, f.p2.z};
         std::memcpy(&fib[6 * i], v, sizeof(v));
     }
-    const ocl_cutter_desc d = c.desc();
-    std::vector<uint64_t> offs(nf + 1, 0);
-    uint64_t total = 0;
-    const int prec = precision | (exact_filter ? OCL_PUSH_EXACT_FILTER : 0);
-    check(ocl_hip_push_batch(m, &d, dir, fib.data(), nf, prec, offs.data(), &total, st, nullptr));
+    return fib;
+}
+
+// Adds the device rows of the last push in direction dir to the fibers'
+// interval lists (as they are, on a fiber without intervals; merged by
+// Fiber::addInterval otherwise).
+static void attach_rows(ocl_mesh* m, int dir, std::vector<Fiber>& fibers, const std::vector<uint64_t>& offs,
+                        uint64_t total) {
     std::vector<double> lohi(2 * total), cc(8 * total);
-    if (total) check(ocl_hip_push_fetch(m, lohi.data(), cc.data(), nullptr));
-    for (size_t i = 0; i < nf; ++i) {
+    if (total) check(ocl_hip_push_fetch_dir(m, dir, lohi.data(), cc.data(), nullptr));
+    for (size_t i = 0; i < fibers.size(); ++i) {
         Fiber& f = fibers[i];
         const bool fresh = f.ints.empty();
         for (uint64_t k = offs[i]; k < offs[i + 1]; ++k) {
@@ -223,6 +224,18 @@ static void push_fibers(ocl_mesh* m, const MillingCutter& c, int dir, std::vecto
                 f.addInterval(iv);
         }
     }
+}
+
+static void push_fibers(ocl_mesh* m, const MillingCutter& c, int dir, std::vector<Fiber>& fibers, int precision,
+                        bool exact_filter, ocl_stats* st) {
+    const size_t nf = fibers.size();
+    const std::vector<double> fib = fiber_array(fibers);
+    const ocl_cutter_desc d = c.desc();
+    std::vector<uint64_t> offs(nf + 1, 0);
+    uint64_t total = 0;
+    const int prec = precision | (exact_filter ? OCL_PUSH_EXACT_FILTER : 0);
+    check(ocl_hip_push_batch(m, &d, dir, fib.data(), nf, prec, offs.data(), &total, st, nullptr));
+    attach_rows(m, dir, fibers, offs, total);
 }
 
 void BatchPushCutter::run() {
@@ -358,10 +371,14 @@ struct SPt {
     CLPoint p;
     double m;  // slope of z along the path direction
 };
+// An open interval of the slope-driven refinement: its ends, and, once
+// dropped, its middle (mid) and its quarter points (q1, q3).
 struct SSeg {
     size_t span;
     double t0, t1;
     SPt a, b;
+    int stage = 0;  // 0: its middle is dropped next; 1: its quarter points are
+    SPt m, q1, q3;
 };
 }  // namespace
 
@@ -447,35 +464,70 @@ void AdaptivePathDropCutter::slope_run() {
     for (size_t i = 0; i + 1 < first.size(); ++i)
         if (where[i].first == where[i + 1].first)
             level.push_back({where[i].first, where[i].second, where[i + 1].second, first[i], first[i + 1]});
-    // each level: drop the middles of all open intervals in one batch; an
-    // interval closes when the Hermite model through its ends and middle
-    // stays within the tolerance of its chord, or at the minimum step;
-    // otherwise its halves (whose ends are now known) form the next level
+    // Each level drops, in one batch, the middle of every interval that
+    // needs one and the quarter points of every interval under check.  An
+    // interval whose Hermite model (ends and middle, heights and slopes)
+    // stays within half the tolerance of its chord is checked: it closes if
+    // its quarter points also lie within half the tolerance of the chord,
+    // else it splits and the quarter points are its halves' middles.  An
+    // interval at the minimum step closes.
     std::vector<SSeg> done;
     levels = 1;
+    auto at = [&](const SSeg& g, double f) { return g.t0 + (g.t1 - g.t0) * f; };
+    std::vector<SSeg> next;
+    // an interval whose middle is known: close it at the minimum step, check
+    // its quarter points next if its Hermite model passes, else split it
+    auto decide = [&](SSeg g) {
+        const double D = spans[g.span]->length2d() * (g.t1 - g.t0);
+        if (D <= min_sampling) {
+            done.push_back(g);
+        } else if (hermite_chord_error(D, g.a.p.z, g.m.p.z, g.b.p.z, g.a.m, g.m.m, g.b.m) <= 0.5 * chord_tol) {
+            g.stage = 1;
+            next.push_back(g);
+        } else {
+            next.push_back({g.span, g.t0, at(g, 0.5), g.a, g.m});
+            next.push_back({g.span, at(g, 0.5), g.t1, g.m, g.b});
+        }
+    };
     while (!level.empty()) {
-        ++levels;
         pts.clear();
         dirs.clear();
+        auto sample = [&](const SSeg& g, double f) {
+            pts.emplace_back(spans[g.span]->getPoint(at(g, f)));
+            dirs.push_back(span_tangent(spans[g.span], at(g, f)));
+        };
         for (const SSeg& g : level) {
-            const double tm = g.t0 + (g.t1 - g.t0) / 2.0;
-            pts.emplace_back(spans[g.span]->getPoint(tm));
-            dirs.push_back(span_tangent(spans[g.span], tm));
+            if (g.stage == 0) {
+                sample(g, 0.5);
+            } else {
+                sample(g, 0.25);
+                sample(g, 0.75);
+            }
         }
-        std::vector<SPt> mids = drop_with_slopes(dev.handle(), *cutter, pts, dirs, precision, &stats);
+        ++levels;
+        std::vector<SPt> got = drop_with_slopes(dev.handle(), *cutter, pts, dirs, precision, &stats);
         calls += stats.pairs;
-        std::vector<SSeg> next;
-        for (size_t i = 0; i < level.size(); ++i) {
-            const SSeg& g = level[i];
-            const double D = spans[g.span]->length2d() * (g.t1 - g.t0);
-            const double e = hermite_chord_error(D, g.a.p.z, mids[i].p.z, g.b.p.z, g.a.m, mids[i].m, g.b.m);
-            // half the tolerance: margin for what three samples cannot see
-            if (e <= 0.5 * chord_tol || D <= min_sampling) {
+        size_t gi = 0;
+        next.clear();
+        for (SSeg g : level) {
+            if (g.stage == 0) {
+                g.m = got[gi++];
+                decide(g);
+                continue;
+            }
+            g.q1 = got[gi++];
+            g.q3 = got[gi++];
+            const double c1 = g.a.p.z + 0.25 * (g.b.p.z - g.a.p.z), c3 = g.a.p.z + 0.75 * (g.b.p.z - g.a.p.z);
+            if (std::fabs(g.q1.p.z - c1) <= 0.5 * chord_tol && std::fabs(g.q3.p.z - c3) <= 0.5 * chord_tol) {
                 done.push_back(g);
             } else {
-                const double tm = g.t0 + (g.t1 - g.t0) / 2.0;
-                next.push_back({g.span, g.t0, tm, g.a, mids[i]});
-                next.push_back({g.span, tm, g.t1, mids[i], g.b});
+                // the quarter points are the halves' middles
+                SSeg lo{g.span, g.t0, at(g, 0.5), g.a, g.m};
+                SSeg hi{g.span, at(g, 0.5), g.t1, g.m, g.b};
+                lo.m = g.q1;
+                hi.m = g.q3;
+                decide(lo);
+                decide(hi);
             }
         }
         level.swap(next);
@@ -517,15 +569,17 @@ void Waterline::run() {
     yfibers.clear();
     for (double y : generate_range(miny, maxy, ny)) xfibers.emplace_back(Point(minx, y, zh), Point(maxx, y, zh));
     for (double x : generate_range(minx, maxx, nx)) yfibers.emplace_back(Point(x, miny, zh), Point(x, maxy, zh));
-    ocl_stats sx{}, sy{};
-    push_fibers(dev.handle(), *cutter, 0, xfibers, precision, exact_filter, &sx);
-    push_fibers(dev.handle(), *cutter, 1, yfibers, precision, exact_filter, &sy);
-    stats = sx;
-    stats.points += sy.points;
-    stats.pairs += sy.pairs;
-    stats.intervals += sy.intervals;
-    stats.brent_iters += sy.brent_iters;
-    stats.kernel_ms += sy.kernel_ms;
+    // both directions in one device launch (the reference runs its two
+    // BatchPushCutters one after the other, waterline.cpp:80-81)
+    const std::vector<double> fx = fiber_array(xfibers), fy = fiber_array(yfibers);
+    std::vector<uint64_t> ox(xfibers.size() + 1, 0), oy(yfibers.size() + 1, 0);
+    uint64_t tot[2] = {0, 0};
+    const ocl_cutter_desc d = cutter->desc();
+    const int prec = precision | (exact_filter ? OCL_PUSH_EXACT_FILTER : 0);
+    check(ocl_hip_push_xy(dev.handle(), &d, fx.data(), xfibers.size(), fy.data(), yfibers.size(), prec, ox.data(),
+                          oy.data(), tot, &stats, nullptr));
+    attach_rows(dev.handle(), 0, xfibers, ox, tot[0]);
+    attach_rows(dev.handle(), 1, yfibers, oy, tot[1]);
     nCalls = stats.pairs;
 }
 

@@ -226,3 +226,14 @@ def config3_jobs(njobs: int = 64, seed: int = 1003, tri_range=(2_000, 500_000), 
         cl = cl_grid(side, side, step, z0=-10.0, origin=(ox, oy))[:n]
         jobs.append({"tris": tris, "spec": spec, "cl": np.ascontiguousarray(cl)})
     return jobs
+
+
+def random_polylines(n: int, extent: float, seed: int, nv=(3, 6), margin: float = 0.05):
+    """n polylines of nv[0]..nv[1] vertices uniform in [margin, 1 - margin]
+    x extent (the paths of the slope-driven refinement demo)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        k = int(rng.integers(nv[0], nv[1] + 1))
+        out.append(rng.uniform(margin * extent, (1.0 - margin) * extent, size=(k, 2)))
+    return out

@@ -157,10 +157,54 @@ def test_autotune_returns_a_working_tune(djobs):
 
 def test_batch_errors(djobs):
     jobs = djobs.ctypes_jobs()
-    dup = (capi.DropJob * 2)(jobs[0], jobs[0])
-    with pytest.raises(capi.OclError, match="once per batch"):
-        capi.Batch(dup, "mixed")
     with pytest.raises(capi.OclError, match="hilbert_bits"):
         capi.Batch(jobs, "mixed", {"hilbert_bits": 12})
     with pytest.raises(capi.OclError, match="at least one job"):
         capi.Batch((capi.DropJob * 0)(), "mixed")
+
+
+def test_batches_keep_their_state_when_the_mesh_is_reused(djobs):
+    """A batch keeps its own selection records and workspace.  Two batches
+    on the same meshes with different cutters, a second mesh appearing twice
+    in one batch with two cutters, and single calls with a larger point set
+    in between leave every replay bit-equal to the single calls; a mesh used
+    by a live batch cannot be destroyed."""
+    import torch
+    swap = {"cyl": "ball", "ball": "bull", "bull": "cone", "cone": "cyl", "composite": "ball"}
+    from cutters import ALL
+    other_specs = [ALL[swap[j["spec"]["kind"]]] for j in djobs.jobs]
+    b1 = capi.Batch(djobs.ctypes_jobs(), "mixed", {"use_graph": 1})
+    ref1 = djobs.single("mixed")
+    # batch 2: the same meshes with other cutters (rebuilds each mesh's records)
+    alt = DeviceJobs.__new__(DeviceJobs)
+    alt.jobs = [dict(j, spec=s) for j, s in zip(djobs.jobs, other_specs)]
+    alt.meshes, alt.cl = djobs.meshes, djobs.cl
+    alt.out = [DeviceJobs._alloc(len(j["cl"])) for j in alt.jobs]
+    b2 = capi.Batch(alt.ctypes_jobs(), "mixed", {"use_graph": 1})
+    ref2 = alt.single("mixed")
+    # a single call with 4x the points of every job (grows each mesh's workspace)
+    for m, j, c in zip(djobs.meshes, djobs.jobs, djobs.cl):
+        big = torch.cat([c] * 4)
+        o = DeviceJobs._alloc(len(big))
+        m.drop_device(j["spec"], big.data_ptr(), len(big), "mixed", *(o[k].data_ptr() for k in FIELDS))
+    torch.cuda.synchronize()
+    for r in range(2):
+        djobs.reset_outputs()
+        b1.run()
+        assert same_bits(djobs.snapshot(), ref1), ("batch 1", r)
+        alt.reset_outputs()
+        b2.run()
+        assert same_bits(alt.snapshot(), ref2), ("batch 2", r)
+    # one mesh twice in a batch, with two cutters
+    two = DeviceJobs.__new__(DeviceJobs)
+    two.jobs = [djobs.jobs[0], dict(djobs.jobs[0], spec=other_specs[0])]
+    two.meshes = [djobs.meshes[0], djobs.meshes[0]]
+    two.cl = [djobs.cl[0], djobs.cl[0]]
+    two.out = [DeviceJobs._alloc(len(j["cl"])) for j in two.jobs]
+    b3 = capi.Batch(two.ctypes_jobs(), "mixed", {"use_graph": 1})
+    b3.run()
+    assert same_bits(two.snapshot(), two.single("mixed"))
+    with pytest.raises(capi.OclError, match="live batch"):
+        djobs.meshes[0].close()
+    for b in (b1, b2, b3):
+        b.close()

@@ -260,3 +260,13 @@ def test_autograd_function_routes_gradients(gpu, grad_scene):
     assert torch.allclose(xy.grad.cpu(), torch.tensor(dz[:, :2] * w.cpu().numpy()[:, None]), atol=1e-12)
     assert np.allclose(verts.grad.cpu().numpy(), gtri, atol=1e-9)
     assert abs(rad.grad.item() - gr) < 1e-9 * (1 + abs(gr))
+    # radius as a plain float, vertices without grad: only the CL gradient
+    xy2 = xy.detach().clone().requires_grad_(True)
+    dev = capi.DeviceMesh(tris)
+    z2 = drop_z(xy2, verts.detach(), spec["d"] / 2, dev, spec, z0=-10.0, precision="fp64")
+    (z2 * w).sum().backward()
+    assert torch.equal(xy2.grad, xy.grad)
+    # a radius that is not the spec's is refused
+    with pytest.raises(ValueError, match="radius"):
+        drop_z(xy2, verts.detach(), 1.0 + spec["d"] / 2, dev, spec, z0=-10.0, precision="fp64")
+    dev.close()

@@ -373,13 +373,7 @@ def test_adaptive_path_drop_cutter_matches_recursive_oracle(gpu, scene):
     assert np.array_equal(got, want)
 
 
-def random_polylines(n, extent, seed, nv=(3, 6)):
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(n):
-        k = rng.integers(nv[0], nv[1] + 1)
-        out.append(rng.uniform(0.05 * extent, 0.95 * extent, size=(k, 2)))
-    return out
+random_polylines = W.random_polylines
 
 
 def refine_and_measure(tris, spec, polylines, tol, sampling, min_sampling, z0=-20.0):
@@ -444,6 +438,58 @@ def test_waterline_loops_are_not_built():
     wl = ocl.Waterline()
     with pytest.raises(ocl.Error, match="weave"):
         wl.getLoops()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["cone", "bull"])
+def test_waterline_fibers_match_oracle(gpu, name):
+    """Waterline::run (both directions in one launch) against the oracle's
+    batch push of the same fibers, per direction: counts, endpoints (Bull
+    within 1e-9) and CC types."""
+    tris = W.spheres_scene(3, 2, 50.0, seed=4)
+    spec = ALL[name]
+    s = surf_from(tris)
+    wl = ocl.Waterline()
+    wl.setSTL(s)
+    wl.setCutter(cutter_from(spec))
+    wl.setZ(3.5)
+    wl.setSampling(0.7)
+    wl.run()
+    om = O.Mesh(tris)
+    tol = 1e-9 if name == "bull" else 0.0
+    for fibers, d in ((wl.getXFibers(), 0), (wl.getYFibers(), 1)):
+        arr = np.array([[f.p1.x, f.p1.y, f.p1.z, f.p2.x, f.p2.y, f.p2.z] for f in fibers])
+        ref = om.batch_push(O.cutter_from_spec(spec), arr, d, search_mode=0)
+        offs = ref["offsets"]
+        nint = 0
+        for i, f in enumerate(fibers):
+            got = sorted((iv.lower, iv.upper, int(iv.lower_cc.type), int(iv.upper_cc.type)) for iv in f.getInts())
+            want = sorted((r[0], r[1], int(c[3]), int(c[7])) for r, c in
+                          zip(ref["lohi"][offs[i]:offs[i + 1]], ref["cc"][offs[i]:offs[i + 1]]))
+            assert len(got) == len(want), (d, i)
+            for g, w in zip(got, want):
+                assert abs(g[0] - w[0]) <= tol and abs(g[1] - w[1]) <= tol and g[2:] == w[2:], (d, i, g, w)
+            nint += len(got)
+        assert nint > 10
+
+
+@pytest.mark.gpu
+def test_push_xy_equals_two_single_direction_pushes(gpu):
+    """ocl_hip_push_xy's rows per direction equal ocl_hip_push_batch's for
+    that direction alone, bit for bit."""
+    tris = W.spheres_scene(4, 2, 60.0, seed=5)
+    spec = ALL["bull"]
+    xf, yf = W.waterline_fibers(tris, 5.0, 4.0, 80)
+    dev = capi.DeviceMesh(tris)
+    both = dev.push_xy(spec, xf, yf)
+    for d, fib in (("x", xf), ("y", yf)):
+        one = dev.push(spec, fib, d)
+        for k in ("offsets", "lohi", "cc"):
+            assert np.array_equal(both[d][k], one[k]), (d, k)
+    assert both["stats"]["points"] == len(xf) + len(yf)
+    empty = dev.push_xy(spec, xf, np.zeros((0, 6)))
+    assert np.array_equal(empty["x"]["lohi"], both["x"]["lohi"]) and len(empty["y"]["lohi"]) == 0
+    dev.close()
 
 
 @pytest.mark.gpu

@@ -37,11 +37,14 @@ def split(res, nf):
     return [(res["lohi"][offs[i]:offs[i + 1]], res["cc"][offs[i]:offs[i + 1]]) for i in range(nf)]
 
 
-def compare_push(tris, spec, fib, direction, label, exact_filter=False):
+def compare_push(tris, spec, fib, direction, label, exact_filter=False, dev=None):
     search_mode = 2 if exact_filter else 0
-    dev = capi.DeviceMesh(tris)
+    own = dev is None
+    if own:
+        dev = capi.DeviceMesh(tris)
     got = dev.push(spec, fib, direction, exact_filter=exact_filter)
-    dev.close()
+    if own:
+        dev.close()
     ref = O.Mesh(tris).batch_push(O.cutter_from_spec(spec), fib, 0 if direction == "x" else 1,
                                   search_mode=search_mode)
     nf = len(fib)
@@ -127,3 +130,43 @@ def test_push_errors_and_empty(gpu):
     res = empty.push(BULL10, np.array([[0, 1, 0, 10, 1, 0]], dtype=float), "x")
     assert list(res["offsets"]) == [0, 0]
     empty.close()
+
+
+def comb(n, pitch):
+    """n thin tent ridges (base 0.2 mm, height 5 mm) along y at x = pitch k."""
+    tris = []
+    for k in range(n):
+        x = pitch * k
+        a, b, c, d = (x - 0.1, -10, 0), (x + 0.1, -10, 0), (x + 0.1, 10, 0), (x - 0.1, 10, 0)
+        t1, t2 = (x, -10, 5), (x, 10, 5)
+        tris += [a + d + t2, a + t2 + t1, b + t1 + t2, b + t2 + c]
+    return np.array(tris, dtype=float)
+
+
+@pytest.mark.parametrize("direction", ["x", "y"])
+@pytest.mark.parametrize("kind", ["bull", "cone"])
+def test_push_comb_overflow_and_capacity(gpu, kind, direction):
+    """A fiber across 40 ridges holds 40 disjoint intervals: more than the
+    first capacity (16 per fiber), so the push retries with 64, past the
+    LDS list (32), and merges in global memory.  A sparse push on the same
+    mesh afterwards and a dense one again match the oracle too."""
+    spec = {"bull": {"kind": "bull", "d": 2.0, "r": 0.5, "L": 10.0},
+            "cone": {"kind": "cone", "d": 2.0, "angle": math.pi / 4, "L": 10.0}}[kind]
+    tris = comb(40, 3.0)
+    if direction == "y":
+        tris = tris.reshape(-1, 3, 3)[:, :, [1, 0, 2]].reshape(-1, 9)
+    def fibers(x1):
+        rows = []
+        for z in (1.0, 3.0):
+            for a in (-3.0, 0.0, 4.0):
+                rows.append([-5, a, z, x1, a, z] if direction == "x" else [a, -5, z, a, x1, z])
+        return np.array(rows, dtype=float)
+    dense, sparse = fibers(125.0), fibers(8.0)
+    dev = capi.DeviceMesh(tris)
+    for label, fib, nmin in (("comb-dense", dense, 40), ("comb-sparse", sparse, 2), ("comb-dense-again", dense, 40)):
+        got = dev.push(spec, fib, direction)
+        longest = int(np.max(np.diff(got["offsets"])))
+        assert longest >= nmin, (label, longest)
+        c, e, k = compare_push(tris, spec, fib, direction, label, dev=dev)
+        assert c == 0 and e == 0 and k == 0, label
+    dev.close()
