@@ -128,6 +128,26 @@ int ocl_hip_drop_device(ocl_mesh* mesh, const ocl_cutter_desc* cutter, const dou
                         uint8_t* d_cc_type_out, uint32_t* d_tri_out, uint8_t* d_feat_out,
                         ocl_stats* stats, void* stream);
 
+/* Allocation-free variant of ocl_hip_drop_device (same arguments without
+ * stats, same results bit for bit).  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory, such as code under HIP
+ * graph capture, tight serving loops and harnesses that forbid hipMalloc.
+ * The per-call workspace (keys, sort scratch, per-point selection records,
+ * queues, counters) is one fixed array per GPU in the library, about 52 MB,
+ * held until the process ends; the call makes no hipMalloc, hipFree or
+ * other allocating call and clears what must start at zero with a kernel.
+ * Limits: n <= 524288 CL points per call (OCL_E_ARG beyond; split larger
+ * sets into several calls on one stream).  The selection records of the
+ * mesh for this cutter and precision are mesh state: an earlier
+ * ocl_hip_drop_device call on the mesh with this cutter and precision
+ * builds them (OCL_E_ARG otherwise; EXACT64 and composite cutters need
+ * none).  Every call on one GPU shares the array: calls that overlap on one
+ * GPU (other streams, other threads) race; calls in order on one stream do
+ * not. */
+int ocl_hip_drop_device_noalloc(ocl_mesh* mesh, const ocl_cutter_desc* cutter, const double* d_cl_xyz, size_t n,
+                                int precision, double* d_z_out, double* d_cc_xyz_out, uint8_t* d_cc_type_out,
+                                uint32_t* d_tri_out, uint8_t* d_feat_out, void* stream);
+
 /* Backward of the drop-cutter.  No counterpart in the reference: it serves the
  * differentiable path (the torch.autograd.Function of the Python binding and
  * slope-driven path refinement).  From the active contact a forward call on

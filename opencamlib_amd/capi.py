@@ -96,6 +96,9 @@ def lib():
         L.ocl_hip_push_xy.restype = C.c_int
         L.ocl_hip_push_fetch_dir.argtypes = [vp, C.c_int, dp, dp, vp]
         L.ocl_hip_push_fetch_dir.restype = C.c_int
+        L.ocl_hip_drop_device_noalloc.argtypes = [vp, C.POINTER(CutterDesc), vp, sz, C.c_int, vp, vp, vp, vp, vp,
+                                                  vp]
+        L.ocl_hip_drop_device_noalloc.restype = C.c_int
         L.ocl_hip_drop_backward.argtypes = [vp, C.POINTER(CutterDesc), vp, sz, vp, vp, vp, vp, vp, C.c_int, vp, vp,
                                             vp, vp]
         L.ocl_hip_drop_backward.restype = C.c_int
@@ -277,6 +280,15 @@ class DeviceMesh:
                                         d_cc or None, d_type or None, d_tri or None, d_feat or None,
                                         C.byref(st) if stats else None, stream))
         return st.as_dict() if stats else None
+
+    def drop_device_noalloc(self, spec: dict, d_cl: int, n: int, precision, d_z: int, d_cc=0, d_type=0, d_tri=0,
+                            d_feat=0, stream=None):
+        """ocl_hip_drop_device_noalloc: drop_device without device allocation
+        (at most 524288 points; the mesh must already hold this cutter's
+        selection records)."""
+        desc = cutter_desc(spec)
+        check(lib().ocl_hip_drop_device_noalloc(self.h, C.byref(desc), d_cl, n, PREC[precision], d_z, d_cc or None,
+                                                d_type or None, d_tri or None, d_feat or None, stream))
 
     def drop_backward(self, spec: dict, d_cl: int, n: int, d_cc: int, d_type: int, d_tri: int, d_feat: int,
                       d_grad_z=0, precision="mixed", d_dz=0, d_grad_tri=0, d_grad_r=0, stream=None):

@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1951,9 +1952,24 @@ static double env_scale(const char* name, double def) {
 // (zfloor) that lets the later passes skip triangles that cannot matter.
 // One job of the fast pipeline: workspace, cached selection records and the
 // arguments of its passes.
+// Device pointers of one job's per-call workspace (a DropWork's buffers,
+// or a carved-up scratch area).
+struct WorkPtrs {
+    uint32_t *esc_list, *esc_list2, *esc_list3;
+    double* zfloor;
+    uint32_t* ntset;
+    uint8_t* ntcnt;
+    double* zband;
+    double* selrec;  // 20 bytes per point
+    uint32_t* qcount;
+    Counters *counters, *cnt2, *cnt3;
+    int* exact_slot;  // host: the queue-count slot the exact pass reads
+};
+
 struct FastPlan {
     ocl_mesh* m;
-    DropWork* w;                      // the job's workspace and counters
+    Counters *cnt2, *cnt3;            // counters of the passes after the selection
+    int* exact_slot;
     std::shared_ptr<RecSet> r32, r64; // the job's selection records
     MeshView mv;
     CutterDev cd;
@@ -1973,6 +1989,74 @@ struct FastPlan {
 
 // Workspace and records of a job (ensure_records may synchronise the stream
 // once per mesh and cutter: call it before any graph capture).
+static WorkPtrs work_ptrs(DropWork& w) {
+    WorkPtrs p;
+    p.esc_list = w.esc_list.as<uint32_t>();
+    p.esc_list2 = w.esc_list2.as<uint32_t>();
+    p.esc_list3 = w.esc_list3.as<uint32_t>();
+    p.zfloor = w.zfloor.as<double>();
+    p.ntset = w.ntset.as<uint32_t>();
+    p.ntcnt = w.ntcnt.as<uint8_t>();
+    p.zband = w.zband.as<double>();
+    p.selrec = w.selrec.as<double>();
+    p.qcount = w.qcount.as<uint32_t>();
+    p.counters = w.counters.as<Counters>();
+    p.cnt2 = w.cnt2.as<Counters>();
+    p.cnt3 = w.cnt3.as<Counters>();
+    p.exact_slot = &w.exact_slot;
+    return p;
+}
+
+// The plan's arguments from the workspace pointers and the records already
+// in F.r32 / F.r64 (no allocation, no device work).
+static void fast_bind(FastPlan& F, ocl_mesh* m, const WorkPtrs& w, const MeshView& mv, const CutterDev& cd,
+                      const double* d_cl, const uint32_t* perm, uint32_t n, int precision, const DropOut& o,
+                      Counters* cnt) {
+    F.m = m;
+    F.cnt2 = w.cnt2;
+    F.cnt3 = w.cnt3;
+    F.exact_slot = w.exact_slot;
+    F.mv = mv;
+    F.cd = cd;
+    F.cl = d_cl;
+    F.perm = perm;
+    F.n = n;
+    F.precision = precision;
+    F.o = o;
+    F.cnt = cnt;
+    F.zfloor = w.zfloor;
+    F.nts.set = w.ntset;
+    F.nts.cnt = w.ntcnt;
+    F.nts.bf = F.nts.cnt + n;
+    F.nts.zband = w.zband;
+    F.nts.zsel = w.selrec;
+    F.nts.bt = reinterpret_cast<uint32_t*>(F.nts.zsel + n);
+    F.nts.esel = reinterpret_cast<float*>(F.nts.bt + n);
+    F.q1 = w.esc_list;
+    F.q2 = w.esc_list2;
+    F.qnt = w.esc_list3;
+    F.qcount = w.qcount;
+    F.nts.q = F.qnt;
+    F.nts.qn = F.qcount + 2;
+    const double se = env_scale("OCL_SEL_E_SCALE", 1.0);
+    const SelErr er64 = make_err<double>(cd, se);
+    F.P64 = make_params<double>(cd, er64);
+    set_cut(F.P64, *F.r64);
+    F.etie = (double)F.P64.E;
+    if (F.r32) {
+        const SelErr er32 = make_err<float>(cd, se);
+        F.P32 = make_params<float>(cd, er32);
+        set_cut(F.P32, *F.r32);
+        F.etie = (double)F.P32.E;
+    }
+}
+
+static bool uses_fp32_records(int precision) {
+    return precision == OCL_PREC_FP32 || precision == OCL_PREC_MIXED;
+}
+
+// Workspace and records of a job (ensure_records may synchronise the stream
+// once per mesh and cutter: call it before any graph capture).
 static int fast_prepare(ocl_mesh* m, DropWork& w, const MeshView& mv, const CutterDev& cd, const double* d_cl,
                         const uint32_t* perm, uint32_t n, int precision, const DropOut& o, Counters* cnt,
                         hipStream_t st, FastPlan& F) {
@@ -1983,46 +2067,15 @@ static int fast_prepare(ocl_mesh* m, DropWork& w, const MeshView& mv, const Cutt
         (rc = w.ntcnt.ensure(2ull * n + 64)) || (rc = w.zband.ensure(8ull * n + 64)) ||
         (rc = w.selrec.ensure(20ull * n + 64)) || (rc = w.ensure_counters()))
         return rc;
-    F.m = m;
-    F.w = &w;
-    F.mv = mv;
-    F.cd = cd;
-    F.cl = d_cl;
-    F.perm = perm;
-    F.n = n;
-    F.precision = precision;
-    F.o = o;
-    F.cnt = cnt;
-    F.zfloor = w.zfloor.as<double>();
-    F.nts.set = w.ntset.as<uint32_t>();
-    F.nts.cnt = w.ntcnt.as<uint8_t>();
-    F.nts.bf = F.nts.cnt + n;
-    F.nts.zband = w.zband.as<double>();
-    F.nts.zsel = w.selrec.as<double>();
-    F.nts.bt = reinterpret_cast<uint32_t*>(F.nts.zsel + n);
-    F.nts.esel = reinterpret_cast<float*>(F.nts.bt + n);
-    F.q1 = w.esc_list.as<uint32_t>();
-    F.q2 = w.esc_list2.as<uint32_t>();
-    F.qnt = w.esc_list3.as<uint32_t>();
-    F.qcount = w.qcount.as<uint32_t>();
-    F.nts.q = F.qnt;
-    F.nts.qn = F.qcount + 2;
     const double se = env_scale("OCL_SEL_E_SCALE", 1.0);
-    const SelErr er64 = make_err<double>(cd, se);
-    F.P64 = make_params<double>(cd, er64);
-    if ((rc = ensure_records<double>(m, cd, m->rs64, er64, st))) return rc;
+    if ((rc = ensure_records<double>(m, cd, m->rs64, make_err<double>(cd, se), st))) return rc;
     F.r64 = m->rs64;
-    set_cut(F.P64, *F.r64);
-    F.etie = (double)F.P64.E;
     F.r32.reset();
-    if (precision == OCL_PREC_FP32 || precision == OCL_PREC_MIXED) {
-        const SelErr er32 = make_err<float>(cd, se);
-        F.P32 = make_params<float>(cd, er32);
-        if ((rc = ensure_records<float>(m, cd, m->rs32, er32, st))) return rc;
+    if (uses_fp32_records(precision)) {
+        if ((rc = ensure_records<float>(m, cd, m->rs32, make_err<float>(cd, se), st))) return rc;
         F.r32 = m->rs32;
-        set_cut(F.P32, *F.r32);
-        F.etie = (double)F.P32.E;
     }
+    fast_bind(F, m, work_ptrs(w), mv, cd, d_cl, perm, n, precision, o, cnt);
     return OCL_OK;
 }
 
@@ -2043,23 +2096,22 @@ static SelArgs<double> fast_args64(const FastPlan& F) {
 static int fast_post(const FastPlan& F, hipStream_t st) {
     if (F.precision == OCL_PREC_FP32) return OCL_OK;
     ocl_mesh* m = F.m;
-    DropWork& w = *F.w;
     const MeshView& mv = F.mv;
     const CutterDev& cd = F.cd;
     const uint32_t n = F.n;
     const bool mixed = F.precision == OCL_PREC_MIXED;
     k_drop_resolve<<<(n + 255) / 256, 256, 0, st>>>(mv, cd, F.cl, F.perm, n, F.nts, F.etie, F.o, F.q1, F.qcount,
-                                                     w.cnt2.as<Counters>());
+                                                     F.cnt2);
     HIP_TRY(hipGetLastError());
     k_drop_neartie<<<kNearTieBlocks, 256, 0, st>>>(
-        mv, cd, F.cl, F.qnt, F.qcount + 2, F.nts, F.o, F.q1, F.qcount, w.cnt2.as<Counters>());
+        mv, cd, F.cl, F.qnt, F.qcount + 2, F.nts, F.o, F.q1, F.qcount, F.cnt2);
     HIP_TRY(hipGetLastError());
     const uint32_t* exact_list = F.q1;
     const uint32_t* exact_n = F.qcount;
     const BoxOrigin* org = m->origins.as<BoxOrigin>();
     const double* cap64 = F.r64->cap.as<double>();
     if (mixed && cd.c.kind != K_CONE && env_scale("OCL_COOP_SELECT", 0.0) != 0.0) {
-        Counters* c2 = w.cnt2.as<Counters>();
+        Counters* c2 = F.cnt2;
         const double* r64 = F.r64->rec.as<double>();
         const SelParams<double>& P64 = F.P64;
         double* zfloor = F.zfloor;
@@ -2076,18 +2128,19 @@ static int fast_post(const FastPlan& F, hipStream_t st) {
         exact_n = F.qcount + 1;
     }
     k_drop_coop_exact<<<kCoopBlocks, 256, 0, st>>>(mv, cd, F.cl, exact_list, exact_n, F.zfloor, cap64, org, F.o,
-                                                   w.cnt3.as<Counters>());
+                                                   F.cnt3);
     HIP_TRY(hipGetLastError());
-    w.exact_slot = (int)(exact_n - F.qcount);
+    *F.exact_slot = (int)(exact_n - F.qcount);
     return OCL_OK;
 }
 
-static int launch_drop_fast(ocl_mesh* m, const MeshView& mv, const CutterDev& cd, const double* d_cl,
-                            const uint32_t* perm, uint32_t n, int precision, const DropOut& o, Counters* cnt,
-                            hipStream_t st, bool timed) {
-    FastPlan F;
-    int rc = fast_prepare(m, m->work, mv, cd, d_cl, perm, n, precision, o, cnt, st, F);
-    if (rc) return rc;
+// The selection and the passes after it for a prepared plan.
+static int launch_drop_fast(const FastPlan& F, hipStream_t st, bool timed) {
+    ocl_mesh* m = F.m;
+    const MeshView& mv = F.mv;
+    const CutterDev& cd = F.cd;
+    const uint32_t n = F.n;
+    const int precision = F.precision;
     if (timed) HIP_TRY(hipEventRecord(m->ev_s0, st));
     const uint32_t nb = select_grid(n, mv.xcd_group);
     const bool f32 = precision == OCL_PREC_FP32 || precision == OCL_PREC_MIXED;
@@ -2166,6 +2219,10 @@ static std::vector<double> tri_bboxes(const double* tri_xyz, size_t ntri) {
     return bb;
 }
 
+namespace ocl_amd {
+static int scratch_address(void** out);  // the allocation-free entry's workspace (below)
+}
+
 static const int kDimsXY[4] = {0, 1, 2, 3};
 static const int kDimsYZ[4] = {2, 3, 4, 5};
 static const int kDimsXZ[4] = {0, 1, 4, 5};
@@ -2194,6 +2251,10 @@ int ocl_hip_mesh_create(const double* tri_xyz, size_t ntri, unsigned bucket, ocl
     *out = nullptr;
     if (ntri > 0 && !tri_xyz) return fail(OCL_E_ARG, "tri_xyz is NULL");
     if (ntri >= 0x7fffffffu) return fail(OCL_E_ARG, "too many triangles");
+    // the allocation-free entry's scratch address on this device (looked
+    // up once per device, here outside any graph capture)
+    void* scratch_at = nullptr;
+    if (int rc0 = scratch_address(&scratch_at)) return rc0;
     ocl_mesh* m = new (std::nothrow) ocl_mesh();
     if (!m) return fail(OCL_E_NOMEM, "out of host memory");
     HIP_TRY(hipGetDevice(&m->device));
@@ -2371,7 +2432,9 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
     } else {
         // the selection's pair/escalation counters feed only the statistics:
         // without them its wavefronts skip three same-address atomics each
-        rc = launch_drop_fast(m, mv, cd, d_cl, perm, n32, precision, o, stats ? cnt : nullptr, st, stats != nullptr);
+        FastPlan F;
+        if ((rc = fast_prepare(m, w, mv, cd, d_cl, perm, n32, precision, o, stats ? cnt : nullptr, st, F))) return rc;
+        rc = launch_drop_fast(F, st, stats != nullptr);
         fast = true;
         if (rc) return rc;
     }
@@ -2404,6 +2467,157 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
 }
 
 }  // extern "C"
+
+// ------------------------------------------------- allocation-free drop
+namespace ocl_amd {
+
+// Per-call workspace of ocl_hip_drop_device_noalloc, for up to
+// kNoAllocPoints CL points: the Hilbert keys and the sort's scratch, the
+// per-point selection records, near-tie sets, floors and escalation queues,
+// and the call's counters.  One fixed array per GPU, held until the process
+// ends; every call on a GPU shares it, so calls that overlap on one GPU
+// race.  Its address on each device comes from hipGetSymbolAddress, never
+// from the array's name in host code.
+constexpr uint32_t kNoAllocPoints = 1u << 19;
+
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+constexpr size_t noalloc_bytes(uint32_t n) {
+    return align256(8ull * n)                           // keys + values
+           + align256(4ull * rs_scratch_words(n))       // radix sort scratch
+           + 3 * align256(4ull * n + 64)                // escalation queues
+           + align256(8ull * n + 64)                    // floors
+           + align256(4ull * kNearTie * n + 64)         // near-tie sets
+           + align256(2ull * n + 64)                    // set sizes, winner features
+           + align256(8ull * n + 64)                    // band bottoms
+           + align256(20ull * n + 64)                   // selection records
+           + 3 * align256(sizeof(Counters)) + align256(16);  // counters, queue counts
+}
+
+static_assert(noalloc_bytes(kNoAllocPoints) <= (size_t)52 << 20, "the stated scratch size (52 MiB) is exceeded");
+static __device__ __align__(256) unsigned char scratch[noalloc_bytes(kNoAllocPoints)];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_at[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (mesh creation does it first, outside any graph capture).
+static int scratch_address(void** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(OCL_E_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_at[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(OCL_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_at[dev] = p;
+    }
+    *out = g_scratch_at[dev];
+    return OCL_OK;
+}
+
+// Carves the workspace of an n-point call out of the scratch area.
+struct NoAllocWork {
+    uint32_t* keys;   // 2n words: keys, then values
+    uint32_t* hist;   // rs_scratch_words(n)
+    WorkPtrs w;
+};
+
+static NoAllocWork carve_noalloc(unsigned char* base, uint32_t n, int* exact_slot) {
+    NoAllocWork a;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char* p = base + off;
+        off += align256(bytes);
+        return p;
+    };
+    a.keys = reinterpret_cast<uint32_t*>(take(8ull * n));
+    a.hist = reinterpret_cast<uint32_t*>(take(4ull * rs_scratch_words(n)));
+    a.w.esc_list = reinterpret_cast<uint32_t*>(take(4ull * n + 64));
+    a.w.esc_list2 = reinterpret_cast<uint32_t*>(take(4ull * n + 64));
+    a.w.esc_list3 = reinterpret_cast<uint32_t*>(take(4ull * n + 64));
+    a.w.zfloor = reinterpret_cast<double*>(take(8ull * n + 64));
+    a.w.ntset = reinterpret_cast<uint32_t*>(take(4ull * kNearTie * n + 64));
+    a.w.ntcnt = take(2ull * n + 64);
+    a.w.zband = reinterpret_cast<double*>(take(8ull * n + 64));
+    a.w.selrec = reinterpret_cast<double*>(take(20ull * n + 64));
+    a.w.counters = reinterpret_cast<Counters*>(take(sizeof(Counters)));
+    a.w.cnt2 = reinterpret_cast<Counters*>(take(sizeof(Counters)));
+    a.w.cnt3 = reinterpret_cast<Counters*>(take(sizeof(Counters)));
+    a.w.qcount = reinterpret_cast<uint32_t*>(take(16));
+    a.w.exact_slot = exact_slot;
+    return a;
+}
+
+static bool records_hold(const std::shared_ptr<RecSet>& rs, const CutterDev& cd) {
+    return rs && rs->ok && std::memcmp(&rs->key, &cd, sizeof(cd)) == 0;
+}
+
+}  // namespace ocl_amd
+
+extern "C" int ocl_hip_drop_device_noalloc(ocl_mesh* m, const ocl_cutter_desc* desc, const double* d_cl, size_t n,
+                                           int precision, double* d_z, double* d_cc, uint8_t* d_type,
+                                           uint32_t* d_tri, uint8_t* d_feat, void* stream_) {
+    using namespace ocl_amd;
+    if (!m) return fail(OCL_E_ARG, "mesh is NULL");
+    if (n > 0 && (!d_cl || !d_z)) return fail(OCL_E_ARG, "cl/z pointers are NULL");
+    if (n > kNoAllocPoints)
+        return fail(OCL_E_ARG, "at most " + std::to_string(kNoAllocPoints) +
+                                   " CL points per allocation-free call (split the points into several calls)");
+    if (precision < OCL_PREC_EXACT64 || precision > OCL_PREC_MIXED) return fail(OCL_E_ARG, "unknown precision mode");
+    CutterDev cd;
+    int rc = cutter_from_desc(desc, cd);
+    if (rc) return rc;
+    const bool exact = precision == OCL_PREC_EXACT64 || m->ntri == 0 || cd.c.kind == K_COMPOSITE;
+    // the selection records are mesh state built by an allocating call
+    if (!exact && (!records_hold(m->rs64, cd) || (uses_fp32_records(precision) && !records_hold(m->rs32, cd))))
+        return fail(OCL_E_ARG, "the mesh holds no selection records for this cutter and precision: call "
+                               "ocl_hip_drop_device once with them first (that call builds them)");
+    if (n == 0) return OCL_OK;
+    void* base = nullptr;
+    if ((rc = scratch_address(&base))) return rc;
+    hipStream_t st = (hipStream_t)stream_;
+    const uint32_t n32 = (uint32_t)n;
+    int exact_slot = 0;
+    const NoAllocWork a = carve_noalloc(static_cast<unsigned char*>(base), n32, &exact_slot);
+    const DropOut o{d_z, d_cc, d_type, d_tri, d_feat};
+    // the counters and queue counts are cleared by k_cl_keys (no memset)
+    const ZeroSet zs{{reinterpret_cast<uint32_t*>(a.w.counters), reinterpret_cast<uint32_t*>(a.w.cnt2),
+                      reinterpret_cast<uint32_t*>(a.w.cnt3), a.w.qcount},
+                     {(uint32_t)(sizeof(Counters) / 4), (uint32_t)(sizeof(Counters) / 4),
+                      (uint32_t)(sizeof(Counters) / 4), 4u}};
+    const double R = cd.c.radius;
+    BinFrame f;
+    f.x0 = m->bbox[0] - R;
+    f.y0 = m->bbox[2] - R;
+    const double ex = std::max(m->bbox[1] + R - f.x0, 1e-9), ey = std::max(m->bbox[3] + R - f.y0, 1e-9);
+    const uint32_t hb = m->hilbert_bits;
+    f.sx = (double)(1u << hb) / ex;
+    f.sy = (double)(1u << hb) / ey;
+    const uint32_t nb = (n32 + 255) / 256;
+    k_cl_keys<<<nb, 256, 0, st>>>(d_cl, n32, f, hb, a.keys, a.keys + n32, zs);
+    uint32_t *ko = nullptr, *perm = nullptr;
+    rs_sort(a.keys, a.keys + n32, n32, 2 * hb, a.hist, &ko, &perm, st);
+    HIP_TRY(hipGetLastError());
+    const MeshView mv = m->view();
+    if (exact) {
+        if (m->ntri > 0)
+            k_drop_exact<<<xcd_grid((n32 + 63) / 64, mv.xcd_group * 4u), 64, 0, st>>>(mv, cd, d_cl, perm, nullptr,
+                                                                                    n32, o, a.w.counters);
+        else
+            k_drop_none<<<nb, 256, 0, st>>>(d_cl, n32, o);
+        HIP_TRY(hipGetLastError());
+        return OCL_OK;
+    }
+    FastPlan F;
+    F.r64 = m->rs64;
+    if (uses_fp32_records(precision)) F.r32 = m->rs32;
+    fast_bind(F, m, a.w, mv, cd, d_cl, perm, n32, precision, o, nullptr);
+    return launch_drop_fast(F, st, false);
+}
 
 // Builds a rank-ordered triangle copy with its box hierarchy over
 // (coordinate cu, z), for the push trees (dims = the reference KD-tree's
@@ -3199,8 +3413,7 @@ int ocl_hip_batch_create(const ocl_drop_job* jobs, size_t njobs, int precision, 
             const FastPlan& F = J.F;
             ocl_mesh* m = F.m;
             pj.push_back(PostJob{F.mv, F.cd, F.cl, F.perm, F.n, F.nts, F.etie, F.o, F.q1, F.qcount, F.zfloor,
-                                 F.r64->cap.as<double>(), m->origins.as<BoxOrigin>(), J.w->cnt2.as<Counters>(),
-                                 J.w->cnt3.as<Counters>()});
+                                 F.r64->cap.as<double>(), m->origins.as<BoxOrigin>(), F.cnt2, F.cnt3});
             poff.push_back(poff.back() + (F.n + 63) / 64);
             J.w->exact_slot = 0;
         }

@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(256) k_rs_scatter(const uint32_t* __restrict__
 
 // Scratch words needed by rs_sort for n pairs: hist (256 x ntiles), two
 // ping-pong buffers of n keys and n values, row totals and row offsets.
-inline size_t rs_scratch_words(uint32_t n) {
+constexpr size_t rs_scratch_words(uint32_t n) {
     const uint32_t ntiles = (n + kRsTile - 1) / kRsTile;
     return (size_t)256 * ntiles + 2ull * n + 512 + 64;
 }
