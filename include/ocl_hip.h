@@ -98,6 +98,17 @@ typedef struct {
  * BatchPushCutter::setSTL (batchpushcutter.cpp:55-70). */
 int ocl_hip_mesh_create(const double* tri_xyz, size_t ntri, unsigned bucket, ocl_mesh** out);
 int ocl_hip_mesh_destroy(ocl_mesh* mesh);
+
+/* Host only (no device work): STLReader::read_from_file
+ * (stlreader.cpp:57-150), binary or text STL, float32 coordinates promoted
+ * to double, 9 doubles per triangle in file order.  Sets *ntri_out to the
+ * triangle count and fills tri_xyz_out when cap >= that count (call with
+ * NULL first to size the array).  OCL_E_ARG if the file cannot be read. */
+int ocl_hip_stl_read(const char* path, double* tri_xyz_out, size_t cap, size_t* ntri_out);
+
+/* An STL file straight to a device mesh (STLReader + setSTL without the
+ * STLSurf triangle list): ocl_hip_stl_read then ocl_hip_mesh_create. */
+int ocl_hip_mesh_create_stl(const char* path, unsigned bucket, ocl_mesh** out);
 size_t ocl_hip_mesh_size(const ocl_mesh* mesh);
 
 /* Host only (no device work): the reference KD-tree's leaf order for a

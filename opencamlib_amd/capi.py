@@ -96,6 +96,10 @@ def lib():
         L.ocl_hip_push_xy.restype = C.c_int
         L.ocl_hip_push_fetch_dir.argtypes = [vp, C.c_int, dp, dp, vp]
         L.ocl_hip_push_fetch_dir.restype = C.c_int
+        L.ocl_hip_stl_read.argtypes = [C.c_char_p, dp, sz, C.POINTER(sz)]
+        L.ocl_hip_stl_read.restype = C.c_int
+        L.ocl_hip_mesh_create_stl.argtypes = [C.c_char_p, C.c_uint, C.POINTER(vp)]
+        L.ocl_hip_mesh_create_stl.restype = C.c_int
         L.ocl_hip_drop_device_noalloc.argtypes = [vp, C.POINTER(CutterDesc), vp, sz, C.c_int, vp, vp, vp, vp, vp,
                                                   vp]
         L.ocl_hip_drop_device_noalloc.restype = C.c_int
@@ -179,6 +183,18 @@ def kd_order(tris, dimset="xy", bucket: int = 1, with_accept: bool = False):
     return (out, acc) if with_accept else out
 
 
+def stl_read(path) -> np.ndarray:
+    """Triangles (n, 9) of a binary or text STL file, float32 coordinates
+    promoted to double (ocl_hip_stl_read; host only)."""
+    p = os.fsencode(path)
+    n = C.c_size_t(0)
+    check(lib().ocl_hip_stl_read(p, None, 0, C.byref(n)))
+    out = np.empty((n.value, 9))
+    if n.value:
+        check(lib().ocl_hip_stl_read(p, _ptr(out, C.c_double), n.value, C.byref(n)))
+    return out
+
+
 class DeviceMesh:
     """Owns an ocl_mesh handle (device copy + box hierarchy in KD order)."""
 
@@ -188,6 +204,16 @@ class DeviceMesh:
         check(lib().ocl_hip_mesh_create(_ptr(t, C.c_double), t.shape[0], bucket, C.byref(h)))
         self.h = h
         self.ntri = t.shape[0]
+
+    @classmethod
+    def from_stl(cls, path, bucket: int = 1):
+        """A device mesh straight from an STL file (ocl_hip_mesh_create_stl)."""
+        self = cls.__new__(cls)
+        h = C.c_void_p()
+        check(lib().ocl_hip_mesh_create_stl(os.fsencode(path), bucket, C.byref(h)))
+        self.h = h
+        self.ntri = int(lib().ocl_hip_mesh_size(h))
+        return self
 
     def close(self):
         """Frees the handle; raises OclError (and keeps it) while a batch

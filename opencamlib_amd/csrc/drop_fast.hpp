@@ -30,6 +30,11 @@
 // rank order; a lifting facet skips the triangle's vertices and edges).  To
 // decide when T cannot reproduce the FP64 outcome, each lane also tracks
 //   - the highest contact of each CC type (near-ties between types),
+//   - in FP32, the highest upper value (z + error bound) of every valid
+//     contact other than the winner, of any type and triangle: a contact
+//     within the winner's band may be the FP64 winner (an exact tie of two
+//     contacts on a flat face, which the reference breaks by rank, or an
+//     ill-conditioned edge whose error bound is large),
 //   - the highest contact whose validity test lies within the error margin of
 //     its threshold or whose height is ill-conditioned ("ambiguous"),
 //   - whether the winner itself is ambiguous.
@@ -42,6 +47,7 @@
 #include "cutter.hpp"
 #include "drop_exact.hpp"
 #include "geom64.hpp"
+#include "feature_ids.hpp"
 
 namespace ocl_amd {
 
@@ -59,14 +65,36 @@ enum FastRec : int {
     R_ZCAP,                    // cull cap: max z + largest error bound / slack of the contacts
     R_EMAX,                    // largest error bound of the triangle's contacts
     R_PMAR,                    // plane cap margin (see tri_cap), +inf: no plane cap
-    kFastRec = 72              // 288 B in FP32: 16-byte aligned records
+    kFastRec = 72,             // FP64 record: 576 B
+    R_FID0 = 72,               // FP32 records only: feature IDs (bit patterns) of
+                               // vertices 0..2 and edges 0..2 (feature_ids.hpp)
+    kFastRec32 = 80            // FP32 record: 320 B (16-byte aligned)
 };
+// record length in T values
+template <class T>
+OCL_HD constexpr int rec_len() {
+    return sizeof(T) == 4 ? (int)kFastRec32 : (int)kFastRec;
+}
+// a feature ID stored in, and read from, an FP32 record slot (bit pattern)
+OCL_HD float id_bits(uint32_t v) {
+    float f;
+    __builtin_memcpy(&f, &v, 4);
+    return f;
+}
+OCL_HD uint32_t id_of(float f) {
+    uint32_t v;
+    __builtin_memcpy(&v, &f, 4);
+    return v;
+}
+// ID of a facet contact (tip/plane facet, or the cone's rim facet) of the
+// triangle of rank k: unique to the triangle
+OCL_HD uint32_t facet_id(uint32_t k, bool rim) { return 0x80000000u | (k << 1) | (rim ? 1u : 0u); }
 // per-edge values: direction (EX,EY), line constants C and A, 1/length, slope
 // M, start height Z1, three cutter constants K1..K3, the s^2 band below which
 // the contact height is ill-conditioned (S2E), the height slack of an
 // ambiguous contact (EAE) and the height error bound of the contact (EZE).
 enum EdgeVal : int { E_EX = 0, E_EY, E_C, E_A, E_INVL, E_M, E_Z1, E_K1, E_K2, E_K3, E_S2E, E_EAE, E_EZE };
-static_assert(R_PMAR < kFastRec, "record overflow");
+static_assert(R_PMAR < kFastRec && R_FID0 + 6 <= kFastRec32, "record overflow");
 
 enum : int { FL_VERT = 1, FL_HORIZ = 2, FL_EDEG0 = 8, FL_EHOR0 = 64 };
 
@@ -163,10 +191,11 @@ template <> struct FM<double> {
 // ------------------------------------------------------ record builder
 // Builds the T record of one triangle for one cutter, in FP64, from the
 // exact record (vertices + reference normal), relative to the box origin
-// (ox, oy, oz), with the error bounds of the facet and edge contacts.
+// (ox, oy, oz), with the error bounds of the facet and edge contacts.  fid:
+// the triangle's 6 feature IDs (FP32 records; null: distinct IDs 0..5).
 template <class T>
 OCL_HD void build_record(const TriX& t, const CutterDev& cd, double ox, double oy, double zmid, const SelErr& er,
-                         T* out) {
+                         T* out, const uint32_t* fid = nullptr) {
     const double u = FM<T>::eps();
     const double Eabs = er.E();
     const CutterP& c = cd.c;
@@ -341,7 +370,19 @@ OCL_HD void build_record(const TriX& t, const CutterDev& cd, double ox, double o
     const bool pcap = !(flags & FL_VERT) && (c.kind == K_CYL || c.kind == K_BALL || c.kind == K_BULL);
     const double pextra = (c.kind == K_BULL) ? 4e-9 : 0.0;
     out[R_PMAR] = pcap ? (T)((2.0 * ezf + Eabs + slk + pextra) * (1.0 + 4.0 * u) + 4.0 * u) : FM<T>::inf();
-    for (int q = R_PMAR + 1; q < kFastRec; ++q) out[q] = (T)0;
+    for (int q = R_PMAR + 1; q < rec_len<T>(); ++q) out[q] = (T)0;
+    if (sizeof(T) == 4)
+        for (int q = 0; q < 6; ++q) {
+            // The cylinder's contact on a horizontal edge is one end of the
+            // chord under the disk, the end towards the edge's first vertex
+            // (cylcutter.cpp:62-68 with equal heights), so it depends on the
+            // edge's direction; every other contact does not.
+            const int j = q - 3;
+            const bool directed = j >= 0 && c.kind == K_CYL && t.p[j].z == t.p[(j + 1) % 3].z;
+            uint32_t id = fid ? fid[q] : (uint32_t)q;
+            if (!directed) id &= ~kEdgeReversed;
+            out[R_FID0 + q] = (T)id_bits(id);
+        }
 }
 
 // Selection constants for one cutter (host side, FP64 -> T).
@@ -493,6 +534,9 @@ struct Sel {
     bool bamb;     // the best contact's validity test was within its margin
     T zt[4];       // per-type max of z + e: vertex, edge, facet(+tip), facet-rim
     T za;          // max of z + slack over ambiguous contacts
+    T zr;          // FP32: max of z + e over valid contacts other than the winner
+                   // and other copies of the winner's feature
+    uint32_t wid;  // FP32: feature ID of the winner
     bool oamb;     // cone: an order-dependent skip decision was within error
     uint32_t pairs;
 };
@@ -506,6 +550,8 @@ OCL_HD void sel_init(Sel<T>& S, T z0) {
     S.bamb = false;
     for (int i = 0; i < 4; ++i) S.zt[i] = -FM<T>::inf();
     S.za = -FM<T>::inf();
+    S.zr = -FM<T>::inf();
+    S.wid = 0xffffffffu;
     S.oamb = false;
     S.pairs = 0;
 }
@@ -548,7 +594,7 @@ OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>&
     T th = (fval | famb) ? zf + ezf : NEG;
     const bool flift = fval & (zf > S.zb);
     bool rlift = false, ramb = false;
-    T zr = NEG;
+    T zr = NEG, urp = NEG;  // rim contact height and upper value (cone)
     if (K == K_CONE) {
         const T ur = F::fma_(rec[R_U2], y, F::fma_(rec[R_U1], x, rec[R_U0B]));
         const T vr = F::fma_(rec[R_V2], y, F::fma_(rec[R_V1], x, rec[R_V0B]));
@@ -556,7 +602,8 @@ OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>&
         zr = F::fma_(rec[R_GY], y, F::fma_(rec[R_GX], x, rec[R_F0B]));
         const bool rval = live & (mr > (T)0);
         ramb = live & (fabs(mr) <= ebf);
-        S.zt[3] = F::max_(S.zt[3], rval ? zr + ezf : NEG);
+        urp = rval ? zr + ezf : NEG;
+        S.zt[3] = F::max_(S.zt[3], urp);
         S.za = F::max_(S.za, ramb ? zr + ezf : NEG);
         th = F::max_(th, (rval | ramb) ? zr + ezf : NEG);
         rlift = !flift & rval & (zr > S.zb);
@@ -568,7 +615,10 @@ OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>&
     // vertices and edges only count when no facet contact lifted
     const bool ve = live & !flift & !rlift;
     // ---------------- vertices
-    T zv = NEG;
+    constexpr bool kRun = sizeof(T) == 4;  // track the runner-up upper value (FP32)
+    T zv = NEG, uvb = NEG, uvo = NEG;  // best vertex; upper values of the best and of the others
+    T uvn = NEG;                       // upper values of vertices that are not the winner's feature
+    uint32_t idv = 0;                  // feature ID of the best vertex
     int fv = FT_NONE;
     bool vamb = false;
 #pragma unroll
@@ -577,16 +627,25 @@ OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>&
         const T z = vertex_height<T, K>(rec, j, x, y, P, m);
         const bool val = ve & (m >= (T)0);
         const bool amb = ve & (m <= P.S2V) & (m >= -P.DM);
-        S.zt[0] = F::max_(S.zt[0], val ? z + P.E : NEG);
+        const T up = val ? z + P.E : NEG;
+        S.zt[0] = F::max_(S.zt[0], up);
         S.za = F::max_(S.za, amb ? z + P.EAV : NEG);
-        th = F::max_(th, F::max_(val ? z + P.E : NEG, amb ? z + P.EAV : NEG));
+        th = F::max_(th, F::max_(up, amb ? z + P.EAV : NEG));
         const bool take = val & (z > zv);
+        if constexpr (kRun) {
+            const uint32_t id = id_of((float)rec[R_FID0 + j]);
+            uvo = F::max_(uvo, take ? uvb : up);
+            uvb = take ? up : uvb;
+            uvn = F::max_(uvn, id != S.wid ? up : NEG);
+            idv = take ? id : idv;
+        }
         zv = take ? z : zv;
         fv = take ? (FT_VTX0 + j) : fv;
         vamb = take ? amb : vamb;
     }
     // ---------------- edges
-    T ze = NEG, ee = (T)0;
+    T ze = NEG, ee = (T)0, ueb = NEG, ueo = NEG, uen = NEG;
+    uint32_t ide = 0;
     int fe = FT_NONE;
     bool eamb = false;
 #pragma unroll
@@ -624,9 +683,12 @@ OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>&
             }
             const T c = F::sqrt_(F::max_(F::fma_(-tau, tau, (T)1), (T)0));
             const T is = F::rsqrt_(F::fma_(tau, tau, M2));
-            const T X = c * F::fma_(P.r1 * AM, is, P.r2 * Q * IAM);
-            const T zg = F::fma_(AM, X, zl - P.r2);
-            const T cg = e[E_K2] * F::fma_(-P.r2 * c, IAM * F::rcp_(Q), X);
+            // With Q^2 = 1 + M^2 the contact's offset along the edge and its
+            // height are c |M| (r1 is + r2 / Q) and c (r1 M^2 is + r2 Q):
+            // no 1/|M| term, so a nearly horizontal edge (|M| just above the
+            // reference's 1e-7 threshold) loses nothing to cancellation.
+            const T zg = F::fma_(c, F::fma_(P.r1 * M2, is, P.r2 * Q), zl - P.r2);
+            const T cg = e[E_K2] * c * AM * F::fma_(P.r1, is, P.r2 * F::rcp_(Q));
             const bool hz = (flags & (FL_EHOR0 << j)) != 0;
             z = hz ? zh : zg;
             ccu = hz ? (T)0 : cg;
@@ -653,10 +715,18 @@ OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>&
         // contact that may still lie on the segment
         const bool amb = ve & (m >= -P.DM) &
                          ((fabs(mt) <= dt) | ((m <= s2e) & (mt >= -dt - P.SQDM * invl)));
-        S.zt[1] = F::max_(S.zt[1], val ? z + eze : NEG);
+        const T up = val ? z + eze : NEG;
+        S.zt[1] = F::max_(S.zt[1], up);
         S.za = F::max_(S.za, amb ? z + eae : NEG);
-        th = F::max_(th, F::max_(val ? z + eze : NEG, amb ? z + eae : NEG));
+        th = F::max_(th, F::max_(up, amb ? z + eae : NEG));
         const bool take = val & (z > ze);
+        if constexpr (kRun) {
+            const uint32_t id = id_of((float)rec[R_FID0 + 3 + j]);
+            ueo = F::max_(ueo, take ? ueb : up);
+            ueb = take ? up : ueb;
+            uen = F::max_(uen, id != S.wid ? up : NEG);
+            ide = take ? id : ide;
+        }
         ze = take ? z : ze;
         ee = take ? eze : ee;
         fe = take ? (FT_EDGE0 + j) : fe;
@@ -669,8 +739,23 @@ OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>&
     const int fw = eb_ ? fe : fv;
     const bool aw = eb_ ? eamb : vamb;
     const bool wlift = (fw != FT_NONE) & (zw > S.zb);
-    // new running best
+    // runner-up: when the triangle lifts, every other valid contact of it and
+    // the previous winner (unless it is a copy of the same feature); when it
+    // does not, its valid contacts that are not copies of the winner's feature
     const bool any = flift | rlift | wlift;
+    if constexpr (kRun) {
+        const T uf = fval ? zf + ezf : NEG;
+        // a facet or rim lift skips the vertices and edges (their values are
+        // -inf then)
+        const T uwin_v = F::max_(uvo, F::max_(ueb, ueo));   // a vertex lifts
+        const T uwin_e = F::max_(F::max_(uvb, uvo), ueo);   // an edge lifts
+        const T uexc = flift ? urp : (rlift ? uf : F::max_(F::max_(uf, urp), eb_ ? uwin_e : uwin_v));
+        const T unew = F::max_(F::max_(uf, urp), F::max_(uvn, uen));
+        const uint32_t wnew = flift ? facet_id(k, false) : (rlift ? facet_id(k, true) : (eb_ ? ide : idv));
+        const T uprev = (S.bt != 0xffffffffu && S.wid != wnew) ? S.zb + S.eb : NEG;
+        S.zr = F::max_(S.zr, any ? F::max_(uprev, uexc) : unew);
+        S.wid = any ? wnew : S.wid;
+    }
     const T zn = flift ? zf : (rlift ? zr : zw);
     const T en = (flift | rlift) ? ezf : ew;
     const int fn = flift ? (int)FT_FACET : (rlift ? (int)FT_FACET_RIM : fw);
@@ -696,6 +781,7 @@ OCL_HD bool sel_escalate(const Sel<T>& S, T z0, const SelParams<T>& P) {
     const int s = feat_slot(S.bf);
     for (int i = 0; i < 4; ++i)
         if (i != s) other = FM<T>::max_(other, S.zt[i]);
+    if (sizeof(T) == 4) other = FM<T>::max_(other, S.zr);  // same-type contacts too
     other = FM<T>::max_(other, z0 + P.E);
     const T lo = S.zb - S.eb;
     return other >= lo || S.za >= lo;

@@ -18,9 +18,11 @@
 #include "drop_exact.hpp"
 #include "drop_fast.hpp"
 #include "drop_grad.hpp"
+#include "feature_ids.hpp"
 #include "kd_rank.hpp"
 #include "mesh.hpp"
 #include "radix.hpp"
+#include "stl_read.hpp"
 #include "walk.hpp"
 
 namespace ocl_amd {
@@ -135,6 +137,7 @@ struct ocl_mesh {
     ocl_amd::DevBuf keys, hist, cl, out_z, out_cc, out_ty, out_tri, out_ft, out_dz;
     ocl_amd::DropWork work;      // selection workspace and counters of the single calls
     ocl_amd::DevBuf origins;     // frame (BoxOrigin) per level-0 box
+    ocl_amd::DevBuf fid;         // 6 feature IDs per triangle (rank order; feature_ids.hpp)
     ocl_amd::DevBuf boxzmax;     // highest vertex per level-0 box
     ocl_amd::DevBuf gr_part;     // backward: per-block partial sums of dz/dr
     double zmid = 0, zhalf = 0, xyb = 0;
@@ -567,12 +570,12 @@ __global__ void __launch_bounds__(64) k_drop_exact_jobs(const ExactJob* __restri
 // Per-triangle selection records for one cutter, in T, box-local frames.
 template <class T>
 __global__ void k_sel_prep(const double* __restrict__ tri, uint32_t ntri, const BoxOrigin* __restrict__ org,
-                           CutterDev cd, SelErr er, T* __restrict__ rec) {
+                           const uint32_t* __restrict__ fid, CutterDev cd, SelErr er, T* __restrict__ rec) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntri) return;
     const TriX t = load_tri(tri, i);
     const BoxOrigin o = org[i / kLeaf];
-    build_record<T>(t, cd, o.ox, o.oy, o.oz, er, rec + (size_t)kFastRec * i);
+    build_record<T>(t, cd, o.ox, o.oy, o.oz, er, rec + (size_t)rec_len<T>() * i, fid + 6 * (size_t)i);
 }
 
 // Per level-0 box: the largest cull cap of its records (box frame), and the
@@ -587,7 +590,7 @@ __global__ void k_box_cap(const T* __restrict__ rec, uint32_t ntri, uint32_t nbo
     float em = 0.0f;
     const uint32_t e = min(ntri, (b + 1) * kLeaf);
     for (uint32_t i = b * kLeaf; i < e; ++i) {
-        const T* r = rec + (size_t)kFastRec * i;
+        const T* r = rec + (size_t)rec_len<T>() * i;
         c = FM<T>::max_(c, r[R_ZCAP]);
         em = fmaxf(em, __double2float_ru((double)r[R_EMAX]));
     }
@@ -632,6 +635,8 @@ struct NearTieSets {
     uint8_t* bf;
     double* zsel;
     float* esel;
+    float* zgap;     // FP32: the highest upper value of the other contacts, less zsel
+                     // (rounded up; -inf: none, or an FP64 selection)
     uint32_t* q;     // queue of the points with a near-tie set (count *qn),
     uint32_t* qn;    // filled by the selection pass
 };
@@ -640,7 +645,7 @@ constexpr uint8_t kSelNone = 0xff;     // undecided, no usable near-tie set
 
 template <class T>
 struct WaveState {
-    T zb, zt[4], za;
+    T zb, zt[4], za, zr;
 };
 
 // Arguments of one selection pass (one job of a batch).
@@ -735,6 +740,7 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
     W.zb = (T)0;
     for (int i = 0; i < 4; ++i) W.zt[i] = -F::inf();
     W.za = -F::inf();
+    W.zr = -F::inf();
     // XY hull of the wave's points
     const double hull_x0 = wave_min(act ? x : INFINITY), hull_x1 = wave_max(act ? x : -INFINITY);
     const double hull_y0 = wave_min(act ? y : INFINITY), hull_y1 = wave_max(act ? y : -INFINITY);
@@ -762,7 +768,7 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
             const uint32_t nv = 9u * nt;
             for (uint32_t i = lane_id(); i < 9u * kLeaf; i += 64) {
                 const uint32_t tr = i / 9u, c = i - 9u * tr;
-                if (i < nv) my[i] = rec[(size_t)kFastRec * (t0 + tr) + R_P0X + c];
+                if (i < nv) my[i] = rec[(size_t)rec_len<T>() * (t0 + tr) + R_P0X + c];
             }
             wave_lds_sync();
             for (uint32_t k = 0; k < nt; ++k) zc = F::max_(zc, vertex_floor_at<T, K>(my + 9u * k, xl, yl, P, act) + dz);
@@ -814,10 +820,10 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
         const uint32_t t0 = b * kLeaf;
         const uint32_t nt = min(m.ntri - t0, (uint32_t)kLeaf);
         {
-            constexpr uint32_t kVec = kLeaf * kFastRec * sizeof(T) / 16;
+            constexpr uint32_t kVec = kLeaf * rec_len<T>() * sizeof(T) / 16;
             constexpr uint32_t kIt = (kVec + 63) / 64;
-            const uint32_t nv = nt * (uint32_t)(kFastRec * sizeof(T) / 16);
-            const float4* src = reinterpret_cast<const float4*>(rec + (size_t)kFastRec * t0);
+            const uint32_t nv = nt * (uint32_t)(rec_len<T>() * sizeof(T) / 16);
+            const float4* src = reinterpret_cast<const float4*>(rec + (size_t)rec_len<T>() * t0);
             float4* dst = reinterpret_cast<float4*>(my);
             // clamped (always in-bounds) loads first, then the stores, so the
             // loads of one box are all in flight together
@@ -832,6 +838,7 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
         // enter the box frame
         const uint32_t bt_in = S.bt;
         S.zb = W.zb - dz;
+        S.zr = W.zr - dz;
         for (int i = 0; i < 4; ++i) S.zt[i] = -F::inf();
         S.za = -F::inf();
         // wave-level pre-test, one lane per triangle of the box: its XY box
@@ -845,7 +852,7 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
             T thr_min = F::inf();
             if (P.cull) thr_min = wave_min_t<T>(act ? F::max_(S.zb, zc - dz) : F::inf()) - P.CUT;
             if (jl < nt) {
-                const T* r = my + kFastRec * jl;
+                const T* r = my + rec_len<T>() * jl;
                 const T bx0 = r[R_BX0], bx1 = r[R_BX1], by0 = r[R_BY0], by1 = r[R_BY1];
                 keep = !((bx1 < lx0) | (bx0 > lx1) | (by1 < ly0) | (by0 > ly1));
                 if (P.cull) {
@@ -894,7 +901,7 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
             for (; tmask; tmask &= tmask - 1) {
                 const uint32_t j = (uint32_t)__builtin_ctz(tmask);
                 const uint32_t ku = uniform_u(t0 + j);
-                const T* r = my + kFastRec * j;
+                const T* r = my + rec_len<T>() * j;
                 const T bx0 = r[R_BX0], bx1 = r[R_BX1], by0 = r[R_BY0], by1 = r[R_BY1];
                 bool on = act & !((bx1 < xl - P.R) | (bx0 > xl + P.R) | (by1 < yl - P.R) | (by0 > yl + P.R));
                 if (P.cull) {
@@ -915,7 +922,7 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
             uint32_t lmask = 0;
             for (; tmask; tmask &= tmask - 1) {
                 const uint32_t j = (uint32_t)__builtin_ctz(tmask);
-                const T* r = my + kFastRec * j;
+                const T* r = my + rec_len<T>() * j;
                 const T bx0 = r[R_BX0], bx1 = r[R_BX1], by0 = r[R_BY0], by1 = r[R_BY1];
                 bool on = act & !((bx1 < xl - P.R) | (bx0 > xl + P.R) | (by1 < yl - P.R) | (by0 > yl + P.R));
                 if (P.cull) {
@@ -960,7 +967,7 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
                 Sel<T> Q;
                 sel_init(Q, pz);
                 T th;
-                sel_triangle<T, K>(my + kFastRec * j, t0 + j, px, py, P, Q, has, &th);
+                sel_triangle<T, K>(my + rec_len<T>() * j, t0 + j, px, py, P, Q, has, &th);
                 ++evals;
                 const uint32_t meta = (uint32_t)(Q.bf & 0xff) | (Q.bamb ? 0x100u : 0u) | (Q.pairs ? 0x200u : 0u) |
                                       (Q.bt != 0xffffffffu ? 0x400u : 0u) | (j << 16);
@@ -998,6 +1005,7 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
         }
         // leave the box frame
         if (S.bt != bt_in) W.zb = S.zb + dz;
+        W.zr = S.zr + dz;
         for (int i = 0; i < 4; ++i) W.zt[i] = F::max_(W.zt[i], S.zt[i] + dz);
         W.za = F::max_(W.za, S.za + dz);
     };
@@ -1089,6 +1097,7 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
     D.zb = W.zb;
     for (int i = 0; i < 4; ++i) D.zt[i] = W.zt[i];
     D.za = W.za;
+    D.zr = W.zr;
     const T z0w = (T)(z0 - ozw);
     const bool up = refine && framed && sel_escalate(D, z0w, P);
     if (refine) {
@@ -1102,6 +1111,7 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
             nts.bf[src] = (uint8_t)D.bf;
             nts.zsel[src] = framed ? ozw + (double)D.zb : z0;
             nts.esel[src] = (float)D.eb;
+            nts.zgap[src] = (sizeof(T) == 4 && framed) ? __double2float_ru((double)D.zr - (double)D.zb) : -INFINITY;
             if (up && usable) {
                 for (uint32_t i = 0; i < ntn; ++i) nts.set[(size_t)src * kNearTie + i] = nt_k[i][ln];
                 const double zs0 = ozw + (double)(D.zb - P.CUT);
@@ -1151,7 +1161,7 @@ constexpr uint32_t kSelRun = 4;
 template <class T, int K>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3)))
 k_drop_select(SelArgs<T> A) {
-    __shared__ __align__(16) T srec[kLeaf * kFastRec];
+    __shared__ __align__(16) T srec[kLeaf * rec_len<T>()];
     __shared__ uint32_t nt_k[kNearTie][64];
     __shared__ float nt_h[kNearTie][64];
     __shared__ uint32_t bl_b[sizeof(T) == 4 ? kBoxList : 1];
@@ -1175,7 +1185,7 @@ template <class T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3)))
 k_drop_select_jobs(const SelArgs<T>* __restrict__ jobs, const uint32_t* __restrict__ chunk_off, uint32_t njobs,
                    uint32_t* __restrict__ queue) {
-    __shared__ __align__(16) T srec[4][kLeaf * kFastRec];
+    __shared__ __align__(16) T srec[4][kLeaf * rec_len<T>()];
     __shared__ uint32_t nt_k[4][kNearTie][64];
     __shared__ float nt_h[4][kNearTie][64];
     __shared__ uint32_t bl_b[4][sizeof(T) == 4 ? kBoxList : 1];
@@ -1244,8 +1254,11 @@ __device__ __attribute__((always_inline)) inline void resolve_points(
                 int it = 0;
                 exact_feature(cd.c, X, t, bf, it);
                 brent += it;
+                // the winner's FP64 height must stay consistent with its
+                // selection value and above every other contact's upper value
                 if (X.type != CC_NONE && feat_slot(X.feat) == feat_slot(bf) && X.z > z0 &&
-                    fabs(X.z - nts.zsel[i]) <= 4.0 * ((double)nts.esel[i] + etie)) {
+                    fabs(X.z - nts.zsel[i]) <= 4.0 * ((double)nts.esel[i] + etie) &&
+                    X.z > nts.zsel[i] + (double)nts.zgap[i]) {
                     X.tri = bt;
                     write_drop(o, m.order, i, X);
                     done = true;
@@ -1888,11 +1901,11 @@ static int ensure_records(ocl_mesh* m, const CutterDev& cd, std::shared_ptr<RecS
     rs.ok = false;
     const uint32_t n = (uint32_t)m->ntri, nbox = m->cnt[0];
     int rc;
-    if ((rc = rs.rec.ensure(sizeof(T) * kFastRec * m->ntri)) || (rc = rs.cap.ensure(sizeof(T) * nbox + 16)) ||
+    if ((rc = rs.rec.ensure(sizeof(T) * rec_len<T>() * m->ntri)) || (rc = rs.cap.ensure(sizeof(T) * nbox + 16)) ||
         (rc = rs.emax.ensure(16)))
         return rc;
-    k_sel_prep<T><<<(n + 255) / 256, 256, 0, st>>>(m->tri.as<double>(), n, m->origins.as<BoxOrigin>(), cd, er,
-                                                   rs.rec.as<T>());
+    k_sel_prep<T><<<(n + 255) / 256, 256, 0, st>>>(m->tri.as<double>(), n, m->origins.as<BoxOrigin>(),
+                                                   m->fid.as<uint32_t>(), cd, er, rs.rec.as<T>());
     HIP_TRY(hipMemsetAsync(rs.emax.p, 0, 4, st));
     k_box_cap<T><<<(nbox + 255) / 256, 256, 0, st>>>(rs.rec.as<T>(), n, nbox, rs.cap.as<T>(),
                                                      rs.emax.as<unsigned int>());
@@ -2032,6 +2045,7 @@ static void fast_bind(FastPlan& F, ocl_mesh* m, const WorkPtrs& w, const MeshVie
     F.nts.zsel = w.selrec;
     F.nts.bt = reinterpret_cast<uint32_t*>(F.nts.zsel + n);
     F.nts.esel = reinterpret_cast<float*>(F.nts.bt + n);
+    F.nts.zgap = F.nts.esel + n;  // the record's 20 bytes: zsel 8, bt 4, esel 4, zgap 4
     F.q1 = w.esc_list;
     F.q2 = w.esc_list2;
     F.qnt = w.esc_list3;
@@ -2330,6 +2344,14 @@ int ocl_hip_mesh_create(const double* tri_xyz, size_t ntri, unsigned bucket, ocl
     hipError_t e = hipMemcpy(staging.p, sorted.data(), sizeof(double) * 9 * ntri, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(m->order.p, order.data(), sizeof(uint32_t) * ntri, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(m->rank_of.p, rank_of.data(), sizeof(uint32_t) * ntri, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        const std::vector<uint32_t> fid = feature_ids(tri_xyz, ntri, order.data());
+        if ((rc = m->fid.ensure(sizeof(uint32_t) * fid.size()))) {
+            delete m;
+            return rc;
+        }
+        e = hipMemcpy(m->fid.p, fid.data(), sizeof(uint32_t) * fid.size(), hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
         delete m;
         return fail(OCL_E_HIP, std::string("mesh upload: ") + hipGetErrorString(e));
@@ -2358,6 +2380,25 @@ int ocl_hip_mesh_create(const double* tri_xyz, size_t ntri, unsigned bucket, ocl
     }
     *out = m;
     return OCL_OK;
+}
+
+int ocl_hip_stl_read(const char* path, double* tri_xyz_out, size_t cap, size_t* ntri_out) {
+    if (!path || !ntri_out) return fail(OCL_E_ARG, "path / ntri_out is NULL");
+    std::vector<double> v;
+    if (!ocl_amd::stl_read(path, v)) return fail(OCL_E_ARG, std::string("cannot read STL file ") + path);
+    const size_t n = v.size() / 9;
+    *ntri_out = n;
+    if (tri_xyz_out && cap >= n && n) std::memcpy(tri_xyz_out, v.data(), sizeof(double) * v.size());
+    return OCL_OK;
+}
+
+int ocl_hip_mesh_create_stl(const char* path, unsigned bucket, ocl_mesh** out) {
+    if (!out) return fail(OCL_E_ARG, "out is NULL");
+    *out = nullptr;
+    if (!path) return fail(OCL_E_ARG, "path is NULL");
+    std::vector<double> v;
+    if (!ocl_amd::stl_read(path, v)) return fail(OCL_E_ARG, std::string("cannot read STL file ") + path);
+    return ocl_hip_mesh_create(v.data(), v.size() / 9, bucket, out);
 }
 
 int ocl_hip_mesh_destroy(ocl_mesh* mesh) {

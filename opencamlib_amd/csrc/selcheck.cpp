@@ -13,6 +13,7 @@
 #include "cutter.hpp"
 #include "drop_exact.hpp"
 #include "drop_fast.hpp"
+#include "feature_ids.hpp"
 #include "kd_rank.hpp"
 #include "push_exact.hpp"
 
@@ -54,7 +55,7 @@ void run_points(const std::vector<TriX>& tx, const std::vector<BoxOrigin>& org, 
         const double x = cl[3 * i], y = cl[3 * i + 1], z0 = cl[3 * i + 2];
         Sel<T> S;
         sel_init(S, (T)0);
-        T wzb = 0, wzt[4], wza = -F::inf();
+        T wzb = 0, wzt[4], wza = -F::inf(), wzr = -F::inf();
         for (int q = 0; q < 4; ++q) wzt[q] = -F::inf();
         bool framed = false;
         double ozw = 0;
@@ -75,7 +76,7 @@ void run_points(const std::vector<TriX>& tx, const std::vector<BoxOrigin>& org, 
                 const T xl = (T)(x - bo.ox), yl = (T)(y - bo.oy);
                 const size_t e = std::min(ntri, (b + 1) * kLeafH);
                 for (size_t k = b * kLeafH; k < e; ++k)
-                    zc = F::max_(zc, vertex_floor<T, K>(&rec[(size_t)kFastRec * k], xl, yl, P, true) + dz);
+                    zc = F::max_(zc, vertex_floor<T, K>(&rec[(size_t)rec_len<T>() * k], xl, yl, P, true) + dz);
             }
         for (size_t b = 0; b < nbox; ++b) {
             const double* bb = &bbx[4 * b];
@@ -94,11 +95,12 @@ void run_points(const std::vector<TriX>& tx, const std::vector<BoxOrigin>& org, 
             }
             const uint32_t bt_in = S.bt;
             S.zb = wzb - dz;
+            S.zr = wzr - dz;
             for (int q = 0; q < 4; ++q) S.zt[q] = -F::inf();
             S.za = -F::inf();
             const size_t e = std::min(ntri, (b + 1) * kLeafH);
             for (size_t k = b * kLeafH; k < e; ++k) {
-                const T* r = &rec[(size_t)kFastRec * k];
+                const T* r = &rec[(size_t)rec_len<T>() * k];
                 if (r[R_BX1] < xl - P.R || r[R_BX0] > xl + P.R || r[R_BY1] < yl - P.R || r[R_BY0] > yl + P.R)
                     continue;
                 if (P.cull) {
@@ -108,6 +110,7 @@ void run_points(const std::vector<TriX>& tx, const std::vector<BoxOrigin>& org, 
                 sel_triangle<T, K>(r, (uint32_t)k, xl, yl, P, S, true);
             }
             if (S.bt != bt_in) wzb = S.zb + dz;
+            wzr = S.zr + dz;
             for (int q = 0; q < 4; ++q) wzt[q] = F::max_(wzt[q], S.zt[q] + dz);
             wza = F::max_(wza, S.za + dz);
         }
@@ -115,6 +118,7 @@ void run_points(const std::vector<TriX>& tx, const std::vector<BoxOrigin>& org, 
         D.zb = wzb;
         for (int q = 0; q < 4; ++q) D.zt[q] = wzt[q];
         D.za = wza;
+        D.zr = wzr;
         bool up = framed && sel_escalate(D, (T)(z0 - ozw), P);
         int why = 0;
         if (framed) {
@@ -141,8 +145,9 @@ void run_points(const std::vector<TriX>& tx, const std::vector<BoxOrigin>& org, 
             ty = X.type;
             ft = D.bf;
             tr = (int)order[D.bt];
+            const double zgap = sizeof(T) == 4 ? (double)D.zr - (double)D.zb : -INFINITY;
             if (!up && (X.type == CC_NONE || feat_slot(X.feat) != feat_slot(D.bf) || !(X.z > z0) ||
-                        fabs(X.z - zr) > 4.0 * ((double)D.eb + (double)P.E))) {
+                        fabs(X.z - zr) > 4.0 * ((double)D.eb + (double)P.E) || !(X.z > zr + zgap))) {
                 up = true;
                 why |= 16;
             }
@@ -186,6 +191,7 @@ extern "C" int ocl_selcheck_drop(const double* tris, size_t ntri, unsigned bucke
         t.n = vunit(vcross(vsub(t.p[0], t.p[1]), vsub(t.p[0], t.p[2])));
         t.maxz = std::fmax(std::fmax(q[2], q[5]), q[8]);
     }
+    const std::vector<uint32_t> fid = feature_ids(tris, ntri, order.data());
     const size_t nbox = (ntri + kLeafH - 1) / kLeafH;
     std::vector<BoxOrigin> org(nbox);
     std::vector<double> bbx(4 * nbox);
@@ -214,13 +220,13 @@ extern "C" int ocl_selcheck_drop(const double* tris, size_t ntri, unsigned bucke
         er.R = cd.c.radius;
         er.zmag = 4.0 * cd.c.radius + 10.0;
         SelParams<T> P = params<T>(cd, er);
-        std::vector<T> rec((size_t)kFastRec * ntri);
+        std::vector<T> rec((size_t)rec_len<T>() * ntri);
         std::vector<T> boxcap(nbox, -FM<T>::inf());
         double emax = 0;
         for (size_t k = 0; k < ntri; ++k) {
             const BoxOrigin& o = org[k / kLeafH];
-            T* r = &rec[(size_t)kFastRec * k];
-            build_record<T>(tx[k], cd, o.ox, o.oy, o.oz, er, r);
+            T* r = &rec[(size_t)rec_len<T>() * k];
+            build_record<T>(tx[k], cd, o.ox, o.oy, o.oz, er, r, &fid[6 * k]);
             boxcap[k / kLeafH] = std::max(boxcap[k / kLeafH], r[R_ZCAP]);
             emax = std::max(emax, (double)r[R_EMAX]);
         }
@@ -269,8 +275,8 @@ extern "C" int ocl_selcheck_calibrate(const double* tris, size_t ntri, const ocl
             if (bx1 < x - R || bx0 > x + R || by1 < y - R || by0 > y + R) continue;
             // box origin: snap to a 2R grid like a level-0 box centre would be
             const double ox = std::floor(bx0 / 2.0) * 2.0 + 1.0, oy = std::floor(by0 / 2.0) * 2.0 + 1.0, oz = t.maxz;
-            float rf[kFastRec];
-            double rd[kFastRec];
+            float rf[kFastRec32];
+            double rd[kFastRec32];
             build_record<float>(t, cd, ox, oy, oz, e32, rf);
             build_record<double>(t, cd, ox, oy, oz, e64, rd);
             const float xf = (float)(x - ox), yf = (float)(y - oy);
@@ -376,7 +382,7 @@ extern "C" int ocl_selcheck_tricap(const double* tris, size_t ntri, const ocl_cu
         er.R = cd.c.radius;
         er.zmag = 4.0 * cd.c.radius + 10.0;
         const SelParams<T> P = params<T>(cd, er);
-        T r[kFastRec];
+        T r[kFastRec32];
         for (size_t k = 0; k < ntri; ++k) {
             const double* q = tris + 9 * k;
             TriX t;
@@ -422,5 +428,63 @@ extern "C" int ocl_selcheck_tricap(const double* tris, size_t ntri, const ocl_cu
     };
     if (use_float) go(0.0f);
     else go(0.0);
+    return OCL_OK;
+}
+
+// One (point, triangle) pair through sel_triangle from an empty lane, the
+// record built in the frame of the triangle's own box centre.  out (12):
+// zb, eb, feature, zt[0..3], za, zr, live pairs, facet (u,v) margin, unused.
+extern "C" int ocl_selcheck_pair(const double* tri9, const ocl_cutter_desc* desc, double x, double y, int use_float,
+                                 double* out) {
+    CutterDev cd;
+    if (!cutter_of(desc, cd)) return OCL_E_UNSUPPORTED;
+    TriX t;
+    t.p[0] = v3(tri9[0], tri9[1], tri9[2]);
+    t.p[1] = v3(tri9[3], tri9[4], tri9[5]);
+    t.p[2] = v3(tri9[6], tri9[7], tri9[8]);
+    t.n = vunit(vcross(vsub(t.p[0], t.p[1]), vsub(t.p[0], t.p[2])));
+    t.maxz = std::fmax(std::fmax(tri9[2], tri9[5]), tri9[8]);
+    const double ox = 0.5 * (std::fmin(std::fmin(tri9[0], tri9[3]), tri9[6]) + std::fmax(std::fmax(tri9[0], tri9[3]), tri9[6]));
+    const double oy = 0.5 * (std::fmin(std::fmin(tri9[1], tri9[4]), tri9[7]) + std::fmax(std::fmax(tri9[1], tri9[4]), tri9[7]));
+    const double oz = t.maxz;
+    auto go = [&](auto tag) {
+        typedef decltype(tag) T;
+        SelErr er;
+        er.u = FM<T>::eps();
+        er.R = cd.c.radius;
+        er.zmag = 4.0 * cd.c.radius + 10.0;
+        SelParams<T> P = params<T>(cd, er);
+        T rec[kFastRec32];
+        build_record<T>(t, cd, ox, oy, oz, er, rec);
+        P.CUT = (T)(2.0 * (double)rec[R_EMAX] + (double)P.HPAD);
+        Sel<T> S;
+        sel_init(S, -FM<T>::inf());
+        const T xl = (T)(x - ox), yl = (T)(y - oy);
+        switch (cd.c.kind) {
+        case K_CYL: sel_triangle<T, K_CYL>(rec, 0, xl, yl, P, S, true); break;
+        case K_BALL: sel_triangle<T, K_BALL>(rec, 0, xl, yl, P, S, true); break;
+        case K_BULL: sel_triangle<T, K_BULL>(rec, 0, xl, yl, P, S, true); break;
+        default: sel_triangle<T, K_CONE>(rec, 0, xl, yl, P, S, true); break;
+        }
+        out[0] = oz + (double)S.zb;
+        out[1] = S.eb;
+        out[2] = S.bf;
+        for (int q = 0; q < 4; ++q) out[3 + q] = oz + (double)S.zt[q];
+        out[7] = oz + (double)S.za;
+        out[8] = oz + (double)S.zr;
+        out[9] = S.pairs;
+        out[10] = (double)rec[R_EMAX];
+        out[11] = (double)rec[R_FLAGS];
+    };
+    if (use_float) go(0.0f);
+    else go(0.0);
+    return OCL_OK;
+}
+
+// Feature IDs of a triangle soup in input order (6 per triangle: vertices
+// p0..p2, edges (p0,p1), (p1,p2), (p2,p0)), as the mesh upload computes them.
+extern "C" int ocl_selcheck_feature_ids(const double* tris, size_t ntri, uint32_t* out) {
+    const std::vector<uint32_t> ids = feature_ids(tris, ntri, nullptr);
+    std::memcpy(out, ids.data(), sizeof(uint32_t) * ids.size());
     return OCL_OK;
 }

@@ -6,6 +6,7 @@
 #include <sstream>
 
 #include "ocl/geometry.hpp"
+#include "ocl_hip.h"
 
 namespace ocl {
 
@@ -163,70 +164,17 @@ STLReader::STLReader(const std::wstring& path, STLSurf& surface) {
 
 STLReader::STLReader(const std::string& path, STLSurf& surface) { read_from_file(path, surface); }
 
-static Triangle tri_from(const float v[3][3]) {
-    return Triangle(Point(v[0][0], v[0][1], v[0][2]), Point(v[1][0], v[1][1], v[1][2]), Point(v[2][0], v[2][1], v[2][2]));
-}
-
-// Text STL: "vertex" lines fill up to three vertices (a fourth overwrites the
-// third), "facet" starts a facet, "endfacet" adds it once the third vertex
-// slot was reached; keywords are matched on their first five non-blank
-// characters (stlreader.cpp:92-146).
-static void read_ascii(std::ifstream& in, STLSurf& s) {
-    std::string line;
-    std::getline(in, line);  // rest of the "solid" line
-    float v[3][3] = {};
-    int k = 0;
-    while (std::getline(in, line)) {
-        char key[6] = {0, 0, 0, 0, 0, 0};
-        size_t j = 0;
-        int i = 0;
-        for (; i < 5; ++i, ++j) {
-            if (j >= line.size()) break;
-            while (j < line.size() && (line[j] == ' ' || line[j] == '\t')) ++j;
-            if (j >= line.size()) break;
-            key[i] = line[j];
-        }
-        if (i < 5) continue;
-        if (!std::strcmp(key, "verte")) {
-            const size_t at = line.find("vertex");
-            if (at != std::string::npos)
-                std::sscanf(line.c_str() + at + 6, "%f %f %f", &v[k][0], &v[k][1], &v[k][2]);
-            k = k + 1 > 2 ? 2 : k + 1;
-        } else if (!std::strcmp(key, "facet")) {
-            k = 0;
-        } else if (!std::strcmp(key, "endfa")) {
-            if (k == 2) s.addTriangle(tri_from(v));
-        }
-    }
-}
-
-// Binary STL: 80-byte header, uint32 count, then per facet 12 float32 (normal,
-// three vertices) and a 2-byte attribute (stlreader.cpp:70-89).
-static void read_binary(std::ifstream& in, STLSurf& s) {
-    char header[75];
-    in.read(header, 75);
-    uint32_t count = 0;
-    in.read(reinterpret_cast<char*>(&count), 4);
-    for (uint32_t i = 0; i < count && in; ++i) {
-        unsigned char rec[50];
-        in.read(reinterpret_cast<char*>(rec), 50);
-        if (!in) break;
-        float v[3][3];
-        std::memcpy(v, rec + 12, 36);
-        s.addTriangle(tri_from(v));
-    }
-}
-
+// The C ABI's reader (ocl_hip_stl_read) fills a flat array; the reference
+// returns silently with no triangles when the file cannot be read.
 void STLReader::read_from_file(const std::string& path, STLSurf& surface) {
-    std::ifstream in(path, std::ios::binary);
-    if (!in) return;
-    char head[5];
-    in.read(head, 5);
-    if (in.gcount() < 5) return;
-    if (std::memcmp(head, "solid", 5) == 0)
-        read_ascii(in, surface);
-    else
-        read_binary(in, surface);
+    size_t n = 0;
+    if (ocl_hip_stl_read(path.c_str(), nullptr, 0, &n) != OCL_OK || n == 0) return;
+    std::vector<double> v(9 * n);
+    if (ocl_hip_stl_read(path.c_str(), v.data(), n, &n) != OCL_OK) return;
+    for (size_t i = 0; i < n; ++i) {
+        const double* q = &v[9 * i];
+        surface.addTriangle(Triangle(Point(q[0], q[1], q[2]), Point(q[3], q[4], q[5]), Point(q[6], q[7], q[8])));
+    }
 }
 
 // ------------------------------------------------------------- paths

@@ -84,10 +84,22 @@ class DeviceSurface {
     ocl_mesh* handle() const { return mesh; }
     // triangle index (input order) -> triangle, for the debug queries
     const std::vector<Triangle>& triangles() const { return tris; }
+    // Triangles the reference's KD search returns for a query box (a0..a1 on
+    // the first search axis, b0..b1 on the second; dimset 0 XY, 1 YZ, 2 XZ),
+    // in its order.  The KD order and acceptance bounds of a dimension set
+    // are built on the first query and kept until the next build.
+    std::vector<Triangle> kd_query(int dimset, unsigned bucket, double a0, double a1, double b0, double b1);
 
   private:
+    struct KdOrder {
+        bool built = false;
+        unsigned bucket = 0;
+        std::vector<uint32_t> order;  // rank -> triangle index
+        std::vector<double> acc;      // per rank: the search's acceptance bounds
+    };
     ocl_mesh* mesh = nullptr;
     std::vector<Triangle> tris;
+    KdOrder kd[3];
 };
 
 class Operation {

@@ -109,6 +109,7 @@ void DeviceSurface::build(const STLSurf& s, unsigned bucket) {
         mesh = nullptr;
     }
     tris.assign(s.tris.begin(), s.tris.end());
+    for (KdOrder& k : kd) k.built = false;
     const std::vector<double> v = s.flat();
     check(ocl_hip_mesh_create(v.data(), tris.size(), bucket, &mesh));
 }
@@ -151,26 +152,30 @@ void BatchDropCutter::run() {
     nCalls = stats.pairs;
 }
 
-// Triangles returned by the reference's KD search for a query box, in its
-// order: the search reaches rank k iff the box reaches the rank's acceptance
-// bounds on both search axes (include/ocl_hip.h, ocl_hip_kd_order).
-static std::vector<Triangle> kd_query(const std::vector<Triangle>& tris, unsigned bucket, int dimset, double a0,
-                                      double a1, double b0, double b1) {
-    std::vector<double> v;
-    v.reserve(9 * tris.size());
-    for (const Triangle& t : tris)
-        for (const Point& q : t.p) {
-            v.push_back(q.x);
-            v.push_back(q.y);
-            v.push_back(q.z);
-        }
-    std::vector<uint32_t> order(tris.size());
-    std::vector<double> acc(4 * tris.size());
-    check(ocl_hip_kd_order(v.data(), tris.size(), dimset, bucket, order.data(), acc.data()));
+// The search reaches rank k iff the box reaches the rank's acceptance bounds
+// on both search axes (include/ocl_hip.h, ocl_hip_kd_order).
+std::vector<Triangle> DeviceSurface::kd_query(int dimset, unsigned bucket, double a0, double a1, double b0,
+                                              double b1) {
+    KdOrder& k = kd[dimset];
+    if (!k.built || k.bucket != bucket) {
+        std::vector<double> v;
+        v.reserve(9 * tris.size());
+        for (const Triangle& t : tris)
+            for (const Point& q : t.p) {
+                v.push_back(q.x);
+                v.push_back(q.y);
+                v.push_back(q.z);
+            }
+        k.order.resize(tris.size());
+        k.acc.resize(4 * tris.size());
+        check(ocl_hip_kd_order(v.data(), tris.size(), dimset, bucket, k.order.data(), k.acc.data()));
+        k.bucket = bucket;
+        k.built = true;
+    }
     std::vector<Triangle> out;
     for (size_t r = 0; r < tris.size(); ++r) {
-        const double* q = &acc[4 * r];
-        if (a1 >= q[0] && a0 <= q[1] && b1 >= q[2] && b0 <= q[3]) out.push_back(tris[order[r]]);
+        const double* q = &k.acc[4 * r];
+        if (a1 >= q[0] && a0 <= q[1] && b1 >= q[2] && b0 <= q[3]) out.push_back(tris[k.order[r]]);
     }
     return out;
 }
@@ -178,7 +183,7 @@ static std::vector<Triangle> kd_query(const std::vector<Triangle>& tris, unsigne
 std::vector<Triangle> BatchDropCutter::getTrianglesUnderCutter(const CLPoint& cl, const MillingCutter& c) {
     if (!dev.handle()) throw Error("setSTL() must be called first");
     const double r = c.getRadius();
-    return kd_query(dev.triangles(), bucketSize, 0, cl.x - r, cl.x + r, cl.y - r, cl.y + r);
+    return dev.kd_query(0, bucketSize, cl.x - r, cl.x + r, cl.y - r, cl.y + r);
 }
 
 void PointDropCutter::run(CLPoint& cl) {
@@ -264,7 +269,7 @@ std::vector<Triangle> BatchPushCutter::getOverlapTriangles(const Fiber& f) {
     const double r = cutter->getRadius(), L = cutter->getLength();
     // X fibers: the YZ tree at (y, z); Y fibers: the XZ tree at (x, z)
     const double across = dir == 0 ? f.p1.y : f.p1.x;
-    return kd_query(dev.triangles(), bucketSize, dir == 0 ? 1 : 2, across - r, across + r, f.p1.z, f.p1.z + L);
+    return dev.kd_query(dir == 0 ? 1 : 2, bucketSize, across - r, across + r, f.p1.z, f.p1.z + L);
 }
 
 // ------------------------------------------------------------ paths
