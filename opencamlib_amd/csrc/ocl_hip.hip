@@ -77,12 +77,17 @@ struct RecSet {
 // the interval lists (cap entries per fiber), the row offsets and the
 // gathered rows of the last push.
 struct PushWork {
-    DevBuf fib, perm, e, n, ovf, offs, lohi, cc;
-    uint32_t cap = 16;       // interval capacity per fiber of the last push (stride of e)
-    uint32_t cap_next = 16;  // the next push starts here: what the last one needed
+    DevBuf fib, perm, pool, off, n, retry, cnt, big, offs, lohi, cc;
+    uint64_t pool_cap = 0;   // entries of pool for the current push
+    uint32_t rcap = 0;       // list capacity of the last retry pass (stride of big), 0: none
+    uint32_t nretry = 0;     // fibers of the last push that needed the retry pass
     uint32_t nf = 0;
     uint64_t total = 0;
 };
+// cnt of a PushWork: retry count of the first pass, overflow count of a
+// retry pass, and the pool entries taken (64-bit)
+constexpr size_t kPushCntBytes = 16;
+constexpr uint32_t kMaxFiberList = 16384;  // longest interval list a fiber may build
 
 // Workspace of one drop job: the per-point selection records, near-tie
 // sets, escalation queues and floors, and the job's counters.  A mesh owns
@@ -121,7 +126,7 @@ struct ocl_mesh {
     ocl_tree push_tree[2];
     ocl_amd::PushWork pw[2];       // per fiber direction: the last push of that direction
     int push_last = 0;             // direction of the last push (ocl_hip_push_fetch)
-    ocl_amd::DevBuf push_cnt, push_queue;
+    ocl_amd::DevBuf push_cnt, push_queue, push_sides;
     size_t ntri = 0;
     double bbox[6] = {0, 0, 0, 0, 0, 0};
     unsigned bucket = 1;
@@ -2745,8 +2750,13 @@ static int push_side_prepare(ocl_mesh* m, int dir, const double* fib, size_t nf,
             return fail(OCL_E_ARG, "fiber " + std::to_string(i) + " does not run along the requested axis");
     }
     PushWork& W = m->pw[dir];
+    // pool: 4 entries per fiber, or twice what the previous push in this
+    // direction kept; fibers that do not fit go to the retry pass
+    W.pool_cap = std::max<uint64_t>(4ull * nf, 2ull * W.total) + 1024;
     W.nf = (uint32_t)nf;
     W.total = 0;
+    W.rcap = 0;
+    W.nretry = 0;
     if (nf == 0 || m->ntri == 0) return OCL_OK;
     ocl_tree& T = m->push_tree[dir];
     if (!T.built) {
@@ -2764,16 +2774,23 @@ static int push_side_prepare(ocl_mesh* m, int dir, const double* fib, size_t nf,
     });
     int rc;
     if ((rc = W.fib.ensure(48ull * nf)) || (rc = W.perm.ensure(4ull * nf)) || (rc = W.n.ensure(4ull * nf + 64)) ||
-        (rc = W.ovf.ensure(64)))
+        (rc = W.off.ensure(8ull * nf + 64)) || (rc = W.retry.ensure(4ull * nf + 64)) ||
+        (rc = W.cnt.ensure(kPushCntBytes)) || (rc = W.pool.ensure(sizeof(PEntry) * W.pool_cap)))
         return rc;
     HIP_TRY(hipMemcpyAsync(W.fib.p, fib, 48ull * nf, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(W.perm.p, perm.data(), 4ull * nf, hipMemcpyHostToDevice, st));
-    // capacity per fiber: what the previous push in this direction needed
-    // (16 unless it had a longer list), times 4 on each overflow
-    W.cap = W.cap_next;
     return OCL_OK;
 }
 
+static PushLists push_lists(PushWork& W) {
+    unsigned char* c = W.cnt.as<unsigned char>();
+    return PushLists{W.pool.as<PEntry>(), reinterpret_cast<unsigned long long*>(c + 8), W.pool_cap,
+                     W.off.as<uint64_t>(), W.n.as<uint32_t>(), W.retry.as<uint32_t>(),
+                     reinterpret_cast<uint32_t*>(c + (W.rcap ? 4 : 0)), W.big.as<PEntry>(), W.rcap};
+}
+
+// One side of a launch: the first pass over all of the direction's fibers,
+// or (W.rcap > 0) the retry pass over the fibers listed in W.retry.
 static PushSide push_side(ocl_mesh* m, int dir, int superset) {
     PushWork& W = m->pw[dir];
     const ocl_tree& T = m->push_tree[dir];
@@ -2782,16 +2799,18 @@ static PushSide push_side(ocl_mesh* m, int dir, int superset) {
     if (superset) S.pm.tree.box = T.kdboxes.as<const float4>();  // walk the KD acceptance boxes
     S.acc = T.acc.as<const double>();
     S.fib = W.fib.as<const double>();
-    S.perm = W.perm.as<const uint32_t>();
-    S.nf = W.nf;
+    S.perm = W.rcap ? W.retry.as<const uint32_t>() : W.perm.as<const uint32_t>();
+    S.nf = W.rcap ? W.nretry : W.nf;
     S.dir = dir;
-    S.PL = PushLists{W.e.as<PEntry>(), W.n.as<uint32_t>(), W.cap, W.ovf.as<uint32_t>()};
+    S.PL = push_lists(W);
     return S;
 }
 
-// Pushes the directions dirs[0..nd) in one launch per attempt; a direction
-// whose lists overflowed runs again with 4x the capacity.  Adds the device
-// time of every attempt to *ms and the counters to *hc.
+// Pushes the directions dirs[0..nd) in one launch (first pass), then the
+// fibers whose lists did not fit in LDS or in the pool in retry passes over
+// those fibers only, with list capacity 256, then 4x as long as some list
+// outgrows it.  Adds the device time of every launch to *ms and the counters
+// to *hc.
 static int push_run(ocl_mesh* m, const CutterDev& cd, int superset, const int* dirs, int nd, hipStream_t st,
                     float* ms, PushCounters* hc) {
     int rc;
@@ -2799,6 +2818,11 @@ static int push_run(ocl_mesh* m, const CutterDev& cd, int superset, const int* d
     HIP_TRY(hipMemsetAsync(m->push_cnt.p, 0, sizeof(PushCounters), st));
     if (!m->ev0) HIP_TRY(hipEventCreate(&m->ev0));
     if (!m->ev1) HIP_TRY(hipEventCreate(&m->ev1));
+    for (int i = 0; i < nd; ++i) {
+        PushWork& W = m->pw[dirs[i]];
+        W.rcap = 0;
+        HIP_TRY(hipMemsetAsync(W.cnt.p, 0, kPushCntBytes, st));
+    }
     int todo[2] = {dirs[0], nd > 1 ? dirs[1] : -1};
     int nt = nd;
     while (nt > 0) {
@@ -2806,40 +2830,52 @@ static int push_run(ocl_mesh* m, const CutterDev& cd, int superset, const int* d
         uint32_t nfib = 0;
         for (int i = 0; i < nt; ++i) {
             PushWork& W = m->pw[todo[i]];
-            if ((rc = W.e.ensure(sizeof(PEntry) * (size_t)W.cap * W.nf))) return rc;
-            HIP_TRY(hipMemsetAsync(W.ovf.p, 0, 4, st));
+            if (W.rcap) {
+                if ((rc = W.big.ensure(sizeof(PEntry) * (size_t)W.rcap * W.nretry))) return rc;
+                HIP_TRY(hipMemsetAsync(W.cnt.as<unsigned char>() + 4, 0, 4, st));
+            }
             S[i] = push_side(m, todo[i], superset);
-            nfib += W.nf;
+            nfib += S[i].nf;
         }
-        if (nt == 1) S[1] = S[0];
+        if ((rc = m->push_sides.ensure(sizeof(S)))) return rc;
+        HIP_TRY(hipMemcpyAsync(m->push_sides.p, S, sizeof(PushSide) * nt, hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemsetAsync(m->push_queue.p, 0, 4, st));
         HIP_TRY(hipEventRecord(m->ev0, st));
         const uint32_t nb = (uint32_t)std::min<size_t>((nfib + kPushWaves - 1) / kPushWaves, 2048);
         uint32_t* q = m->push_queue.as<uint32_t>();
         PushCounters* pc = m->push_cnt.as<PushCounters>();
         const uint32_t ns = (uint32_t)nt;
+        const PushSide* sd = m->push_sides.as<const PushSide>();
         switch (cd.c.kind) {
-        case K_CYL: k_push_fibers<K_CYL><<<nb, 256, 0, st>>>(S[0], S[1], ns, q, superset, cd, pc); break;
-        case K_BALL: k_push_fibers<K_BALL><<<nb, 256, 0, st>>>(S[0], S[1], ns, q, superset, cd, pc); break;
-        case K_BULL: k_push_fibers<K_BULL><<<nb, 256, 0, st>>>(S[0], S[1], ns, q, superset, cd, pc); break;
-        case K_CONE: k_push_fibers<K_CONE><<<nb, 256, 0, st>>>(S[0], S[1], ns, q, superset, cd, pc); break;
-        default: k_push_fibers<K_COMPOSITE><<<nb, 256, 0, st>>>(S[0], S[1], ns, q, superset, cd, pc); break;
+        case K_CYL: k_push_fibers<K_CYL><<<nb, 256, 0, st>>>(sd, ns, q, superset, cd, pc); break;
+        case K_BALL: k_push_fibers<K_BALL><<<nb, 256, 0, st>>>(sd, ns, q, superset, cd, pc); break;
+        case K_BULL: k_push_fibers<K_BULL><<<nb, 256, 0, st>>>(sd, ns, q, superset, cd, pc); break;
+        case K_CONE: k_push_fibers<K_CONE><<<nb, 256, 0, st>>>(sd, ns, q, superset, cd, pc); break;
+        default: k_push_fibers<K_COMPOSITE><<<nb, 256, 0, st>>>(sd, ns, q, superset, cd, pc); break;
         }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(m->ev1, st));
-        uint32_t ovf[2] = {0, 0};
+        uint32_t cnt[2][2] = {{0, 0}, {0, 0}};
         for (int i = 0; i < nt; ++i)
-            HIP_TRY(hipMemcpyAsync(&ovf[i], m->pw[todo[i]].ovf.p, 4, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(cnt[i], m->pw[todo[i]].cnt.p, 8, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         float t = 0;
         HIP_TRY(hipEventElapsedTime(&t, m->ev0, m->ev1));
         *ms += t;
         int again = 0;
         for (int i = 0; i < nt; ++i) {
-            if (!ovf[i]) continue;
             PushWork& W = m->pw[todo[i]];
-            if (W.cap >= 4096) return fail(OCL_E_UNSUPPORTED, "push: more than 4096 intervals on one fiber");
-            W.cap *= 4;
+            if (W.rcap == 0) {
+                W.nretry = cnt[i][0];
+                if (W.nretry == 0) continue;
+                W.rcap = 256;
+            } else {
+                if (cnt[i][1] == 0) continue;
+                if (W.rcap >= kMaxFiberList)
+                    return fail(OCL_E_UNSUPPORTED,
+                                "push: more than " + std::to_string(kMaxFiberList) + " intervals on one fiber");
+                W.rcap *= 4;
+            }
             todo[again++] = todo[i];
         }
         nt = again;
@@ -2848,8 +2884,8 @@ static int push_run(ocl_mesh* m, const CutterDev& cd, int superset, const int* d
     return OCL_OK;
 }
 
-// Row offsets of one direction's last push (host and device), its total,
-// and the capacity the next push in this direction starts from.
+// Row offsets of one direction's last push (host and device) and its total
+// (the next push in this direction sizes its pool from it).
 static int push_side_finish(ocl_mesh* m, int dir, uint64_t* offsets_out) {
     PushWork& W = m->pw[dir];
     const size_t nf = W.nf;
@@ -2861,16 +2897,12 @@ static int push_side_finish(ocl_mesh* m, int dir, uint64_t* offsets_out) {
     std::vector<uint32_t> cnt(nf);
     HIP_TRY(hipMemcpy(cnt.data(), W.n.p, 4ull * nf, hipMemcpyDeviceToHost));
     uint64_t run = 0;
-    uint32_t longest = 0;
     for (size_t i = 0; i < nf; ++i) {
         offsets_out[i] = run;
         run += cnt[i];
-        longest = std::max(longest, cnt[i]);
     }
     offsets_out[nf] = run;
     W.total = run;
-    W.cap_next = 16;
-    while (W.cap_next < longest) W.cap_next *= 4;
     int rc;
     if ((rc = W.offs.ensure(8ull * (nf + 1)))) return rc;
     HIP_TRY(hipMemcpy(W.offs.p, offsets_out, 8ull * (nf + 1), hipMemcpyHostToDevice));
@@ -2901,7 +2933,7 @@ static int push_fetch_side(ocl_mesh* m, int dir, double* lohi, double* cc, hipSt
     if (!lohi || !cc) return fail(OCL_E_ARG, "output pointers are NULL");
     int rc;
     if ((rc = W.lohi.ensure(16ull * total)) || (rc = W.cc.ensure(64ull * total))) return rc;
-    PushLists PL{W.e.as<PEntry>(), W.n.as<uint32_t>(), W.cap, W.ovf.as<uint32_t>()};
+    const PushLists PL = push_lists(W);
     const uint32_t nf = W.nf;
     k_push_gather<<<(nf + 255) / 256, 256, 0, st>>>(PL, W.offs.as<uint64_t>(), nf, W.lohi.as<double>(),
                                                     W.cc.as<double>());

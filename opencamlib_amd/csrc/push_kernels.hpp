@@ -4,8 +4,7 @@
 // The mesh copy used here is ordered by the reference's YZ (X fibers) or XZ
 // (Y fibers) KD-tree rank, with box hierarchies over (across, z) and over the
 // KD search's acceptance bounds.  Each fiber's interval list
-// (Fiber::addInterval, fiber.cpp:58-93) is kept sorted and disjoint in
-// global memory.
+// (Fiber::addInterval, fiber.cpp:58-93) is kept sorted and disjoint.
 #pragma once
 
 #include "mesh.hpp"
@@ -21,11 +20,26 @@ struct PEntry {
     int lty, uty;
 };
 
+// Where the fibers' lists go.  First pass (rcap == 0): a list is built in
+// LDS (kLdsList entries) and copied into a compact pool at an offset taken
+// from a counter; a fiber whose list outgrows LDS, or does not fit in the
+// pool, is appended to `retry` (and written with 0 entries).  Retry pass
+// (rcap > 0, over the `retry` fibers only): slot s of the pass builds its list
+// in global memory at big + s * rcap; a list that outgrows rcap counts in
+// *retry_n and the pass runs again with a larger rcap.  Intermediate lists
+// can be longer than the final one (intervals met early merge later), so a
+// list's final length does not bound what it needs while it is built.
+constexpr uint64_t kBigList = 1ull << 63;  // off[]: the list is in `big`
 struct PushLists {
-    PEntry* e;          // cap entries per fiber (input fiber index)
-    uint32_t* n;        // entries per fiber
-    uint32_t cap;
-    uint32_t* overflow; // fibers whose list overflowed
+    PEntry* pool;
+    unsigned long long* used;  // pool entries taken
+    unsigned long long pool_cap;
+    uint64_t* off;             // per fiber (input index): offset in pool, or kBigList | offset in big
+    uint32_t* n;               // entries per fiber
+    uint32_t* retry;           // first pass: fibers to retry (output); retry pass: the fibers (input)
+    uint32_t* retry_n;         // first pass: their count; retry pass: lists that outgrew rcap
+    PEntry* big;
+    uint32_t rcap;
 };
 
 // One fiber direction of a push launch: the direction's tree (rank order
@@ -130,17 +144,17 @@ struct PushCounters {
 // in parallel (sort by lower end, components by a prefix max), so lane 0
 // inserts a few components rather than every interval; one by one, the
 // merges took 65% of the ConeCutter push.  The fiber's list lives in LDS
-// while it is built and is copied out at the end, when the list capacity
-// fits (kLdsList entries); larger capacities, after an overflow retry,
-// merge in global memory.
+// while it is built (first pass) and is copied out at the end; the few
+// fibers whose lists outgrow it stop there and run again, alone, in a retry
+// pass that merges in global memory (PushLists).
 constexpr int kPushWaves = 4;  // wavefronts per 256-thread block
 constexpr int kBullSolves = 6; // Brent solves per pair: 3 edges x 2 roots
-constexpr uint32_t kLdsList = 32;
+constexpr uint32_t kLdsList = 64;
 
 template <int K>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-k_push_fibers(PushSide s0, PushSide s1, uint32_t nsides, uint32_t* __restrict__ queue, int superset, CutterDev cd,
-              PushCounters* cnt) {
+k_push_fibers(const PushSide* __restrict__ sides, uint32_t nsides, uint32_t* __restrict__ queue, int superset,
+              CutterDev cd, PushCounters* cnt) {
     constexpr int kS = (K == K_BULL) ? kBullSolves : 1;
     __shared__ uint32_t qmem[kPushWaves][128];
     __shared__ PEntry emem[kPushWaves][64];
@@ -154,7 +168,11 @@ k_push_fibers(PushSide s0, PushSide s1, uint32_t nsides, uint32_t* __restrict__ 
     PEntry* ebuf = emem[wv];
     const double R = cd.c.radius, L = cd.c.length;
     unsigned long long pairs = 0, brent = 0, nint = 0;
-    const uint32_t total = s0.nf + (nsides > 1 ? s1.nf : 0u);
+    // the sides' descriptors stay in global memory (read with uniform
+    // addresses): two copies as kernel arguments, picked per fiber, cost
+    // registers and spills
+    const uint32_t nf0 = sides[0].nf;
+    const uint32_t total = nf0 + (nsides > 1 ? sides[1].nf : 0u);
     // fibers cost very different amounts (a fiber through a sphere against
     // one that misses every triangle), so each wavefront takes the next
     // fiber of the sorted order from a device counter when it is done,
@@ -164,13 +182,14 @@ k_push_fibers(PushSide s0, PushSide s1, uint32_t nsides, uint32_t* __restrict__ 
         if (lane == 0) w = atomicAdd(queue, 1u);
         w = uniform_u((uint32_t)__shfl((int)w, 0, 64));
         if (w >= total) break;
-        const bool second = w >= s0.nf;
-        const PushSide& S = second ? s1 : s0;
+        const bool second = w >= nf0;
+        const PushSide& S = sides[second ? 1 : 0];
         const MeshView pm = S.pm;
         const double* __restrict__ acc = S.acc;
         const int dir = S.dir;
-        const PushLists PL = S.PL;
-        const uint32_t src = uniform_u(S.perm[second ? w - s0.nf : w]);
+        const PushLists& PL = S.PL;  // read where used (global memory, uniform)
+        const uint32_t slot = second ? w - nf0 : w;  // position in the side's fiber order
+        const uint32_t src = uniform_u(S.perm[slot]);
         const double* __restrict__ fib = S.fib;
         const FibX f = make_fiber(fib + 6 * (size_t)src);
         const double across = (dir == 0) ? f.p1.y : f.p1.x;
@@ -178,9 +197,9 @@ k_push_fibers(PushSide s0, PushSide s1, uint32_t nsides, uint32_t* __restrict__ 
         const double a0 = across - R, a1 = across + R, v0 = f.p1.z, v1 = f.p1.z + L;
         const float qa0 = __double2float_rd(a0), qa1 = __double2float_ru(a1);
         const float qv0 = __double2float_rd(v0), qv1 = __double2float_ru(v1);
-        PEntry* E = PL.e + (size_t)src * PL.cap;
-        const bool in_lds = PL.cap <= kLdsList;
-        PEntry* EL = in_lds ? lmem[wv] : E;  // where the list is built
+        const bool in_lds = PL.rcap == 0;
+        const uint32_t cap = in_lds ? kLdsList : PL.rcap;
+        PEntry* EL = in_lds ? lmem[wv] : PL.big + (size_t)slot * PL.rcap;  // where the list is built
         uint32_t n = 0;  // list size (kept by lane 0, broadcast when needed)
         bool ovf = false;
         uint32_t qn = 0;
@@ -332,7 +351,7 @@ k_push_fibers(PushSide s0, PushSide s1, uint32_t nsides, uint32_t* __restrict__ 
                     m.hi = u.hi;
                     m.ucc[0] = u.ucc[0]; m.ucc[1] = u.ucc[1]; m.ucc[2] = u.ucc[2];
                     m.uty = u.uty;
-                    if (!plist_insert(EL, n, PL.cap, m)) ovf = true;
+                    if (!plist_insert(EL, n, cap, m)) ovf = true;
                 }
             }
             wave_lds_sync();
@@ -370,6 +389,8 @@ k_push_fibers(PushSide s0, PushSide s1, uint32_t nsides, uint32_t* __restrict__ 
             if (qn >= 64 || (walk.done && qn > 0)) {
                 const uint32_t nb = qn < 64 ? qn : 64u;
                 batch(nb);
+                // a list that outgrew its capacity is redone in a retry pass
+                if (__shfl((int)ovf, 0, 64) != 0) break;
                 const uint32_t rest = qn - nb;
                 const uint32_t moved = (lane < rest) ? qbuf[64 + lane] : 0u;
                 wave_lds_sync();
@@ -379,17 +400,29 @@ k_push_fibers(PushSide s0, PushSide s1, uint32_t nsides, uint32_t* __restrict__ 
             }
             if (walk.done && qn == 0) break;
         }
+        const bool of = __shfl((int)ovf, 0, 64) != 0;
+        const uint32_t nn = uniform_u(__shfl(n, 0, 64));
         if (in_lds) {
-            // copy the list out (n is lane 0's)
-            const uint32_t nn = uniform_u(__shfl(n, 0, 64));
+            // copy the list into the pool (n and the offset are lane 0's)
+            unsigned long long o = 0;
+            if (lane == 0 && !of) o = atomicAdd(PL.used, (unsigned long long)nn);
+            o = __shfl(o, 0, 64);
+            const bool fits = !of && o + nn <= PL.pool_cap;
             wave_lds_sync();
-            for (uint32_t j = lane; j < nn; j += 64) E[j] = lmem[wv][j];
+            if (fits)
+                for (uint32_t j = lane; j < nn; j += 64) PL.pool[o + j] = lmem[wv][j];
             wave_lds_sync();
-        }
-        if (lane == 0) {
-            PL.n[src] = n;
-            if (ovf) atomicAdd(PL.overflow, 1u);
-            nint += n;
+            if (lane == 0) {
+                PL.n[src] = fits ? nn : 0u;
+                PL.off[src] = fits ? (uint64_t)o : 0ull;
+                if (!fits) PL.retry[atomicAdd(PL.retry_n, 1u)] = src;
+                nint += fits ? nn : 0u;
+            }
+        } else if (lane == 0) {
+            PL.n[src] = of ? 0u : nn;
+            PL.off[src] = kBigList | ((uint64_t)slot * PL.rcap);
+            if (of) atomicAdd(PL.retry_n, 1u);
+            nint += of ? 0u : nn;
         }
     }
     if (cnt) {
@@ -405,7 +438,8 @@ __global__ void k_push_gather(PushLists PL, const uint64_t* __restrict__ offs, u
                               double* __restrict__ cc) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nf) return;
-    const PEntry* E = PL.e + (size_t)f * PL.cap;
+    const uint64_t at = PL.off[f];
+    const PEntry* E = (at & kBigList) ? PL.big + (at & ~kBigList) : PL.pool + at;
     const uint64_t o = offs[f];
     const uint32_t n = PL.n[f];
     for (uint32_t j = 0; j < n; ++j) {

@@ -146,15 +146,19 @@ def comb(n, pitch):
 @pytest.mark.parametrize("direction", ["x", "y"])
 @pytest.mark.parametrize("kind", ["bull", "cone"])
 def test_push_comb_overflow_and_capacity(gpu, kind, direction):
-    """A fiber across 40 ridges holds 40 disjoint intervals: more than the
-    first capacity (16 per fiber), so the push retries with 64, past the
-    LDS list (32), and merges in global memory.  A sparse push on the same
-    mesh afterwards and a dense one again match the oracle too."""
+    """Fibers across 40 ridges hold 40 disjoint intervals (the sparse ones
+    2): lists that fit the 64-entry LDS list and go to the compact pool.  A
+    comb of 100 ridges gives 100-interval lists that outgrow LDS and run
+    again in the retry pass, which merges in global memory.  Pushes in a row
+    on one mesh (dense, sparse, dense again, longer) all match the oracle."""
     spec = {"bull": {"kind": "bull", "d": 2.0, "r": 0.5, "L": 10.0},
             "cone": {"kind": "cone", "d": 2.0, "angle": math.pi / 4, "L": 10.0}}[kind]
-    tris = comb(40, 3.0)
-    if direction == "y":
-        tris = tris.reshape(-1, 3, 3)[:, :, [1, 0, 2]].reshape(-1, 9)
+    def mesh(n):
+        t = comb(n, 3.0)
+        if direction == "y":
+            t = t.reshape(-1, 3, 3)[:, :, [1, 0, 2]].reshape(-1, 9)
+        return t
+    tris = mesh(40)
     def fibers(x1):
         rows = []
         for z in (1.0, 3.0):
@@ -167,6 +171,29 @@ def test_push_comb_overflow_and_capacity(gpu, kind, direction):
         got = dev.push(spec, fib, direction)
         longest = int(np.max(np.diff(got["offsets"])))
         assert longest >= nmin, (label, longest)
+        c, e, k = compare_push(tris, spec, fib, direction, label, dev=dev)
+        assert c == 0 and e == 0 and k == 0, label
+    dev.close()
+    tris = mesh(100)
+    dev = capi.DeviceMesh(tris)
+    fib = fibers(305.0)
+    got = dev.push(spec, fib, direction)
+    assert int(np.max(np.diff(got["offsets"]))) >= 100
+    c, e, k = compare_push(tris, spec, fib, direction, "comb-100", dev=dev)
+    assert c == 0 and e == 0 and k == 0
+    dev.close()
+    # 2,000 fibers of 40 intervals: 80,000 entries, more than the first
+    # pool (4 per fiber), so most lists go to the retry pass; the next push
+    # sizes its pool from this one's total
+    tris = mesh(40)
+    dev = capi.DeviceMesh(tris)
+    a = np.linspace(-9.0, 9.0, 2000)
+    z = np.full_like(a, 1.0)
+    if direction == "x":
+        fib = np.stack([np.full_like(a, -5.0), a, z, np.full_like(a, 125.0), a, z], axis=1)
+    else:
+        fib = np.stack([a, np.full_like(a, -5.0), z, a, np.full_like(a, 125.0), z], axis=1)
+    for label in ("pool-full", "pool-sized"):
         c, e, k = compare_push(tris, spec, fib, direction, label, dev=dev)
         assert c == 0 and e == 0 and k == 0, label
     dev.close()
