@@ -15,14 +15,18 @@ oracle on a seeded sample, and the oracle itself timed on the host cores on a
 bounded sample (the ``cpu_baseline``).  Unless ``--no-extra``: the same
 mesh and grid with the BullCutter (the configuration's second run), the backward
 pass on 1M random points of the same mesh, the push-cutter on the 80,000
-waterline fibers of the sphere scene (Bull and Cone), and 64 uneven batched
-jobs (persistent graph launch vs per-job launches).  Prints one JSON line.
+waterline fibers of the sphere scene (Bull and Cone, X and Y fibers in one
+launch), 64 uneven batched jobs (persistent graph launch vs per-job
+launches) and the slope-driven refinement of 500 polylines on the same mesh.
+Every section with a hot kernel carries a roofline object.  Prints one JSON
+line.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -37,9 +41,17 @@ METRIC = ("drop-cutter CL pts/s @524k tris: FP32+FP64-fix vs FP64 GPU, |Δz|≤1
 PEAK_FP32 = 157.3
 PEAK_FP32_UNPACKED = 78.6
 PEAK_FP64 = 78.6
-# Algorithmic FP32 work per evaluated (CL point, triangle) pair: facet +
-# 3 vertices + 3 edges, add/mul = 1, fma = 2, sqrt/rcp/div = 1.
-F_PAIR = {"cyl": 91, "ball": 103, "bull": 79, "cone": 149}
+# Algorithmic FP32 work per evaluated (CL point, triangle) pair of the
+# selection's contact function (facet + 3 vertices + 3 edges with their error
+# bounds; add/sub/mul = 1, fma = 2, sqrt/rsqrt/rcp/div = 1), counted by
+# running that code on the host with a counting float type
+# (tests/flops/flop_drop.cpp; tests/test_flop_counts.py keeps these equal).
+F_PAIR = {"cyl": 101, "ball": 109, "bull": 479, "cone": 164}
+# FP64 work per (fiber, candidate triangle) pair of the push contact code,
+# the same rule, averaged over the KD-superset pairs of 40 seeded X fibers of
+# the push workload (tests/flops/flop_push.cpp); BullCutter includes its
+# edges' Brent solves (15.2 iterations per pair on that sample).
+F_PUSH = {"bull": 2430, "cone": 1244}
 
 SPECS = {
     "ball": {"kind": "ball", "d": 10.0, "L": 20.0},
@@ -51,6 +63,33 @@ SPECS = {
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_cpu():
+    """The host CPU as lscpu reports it, and the CPUs this process may run on."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "logical_cpus"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+        if "sockets" in info and "cores_per_socket" in info:
+            info["physical_cores"] = int(info["sockets"]) * int(info["cores_per_socket"])
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    info["cpus_available_to_this_process"] = len(os.sched_getaffinity(0))
+    return info
+
+
+# The reference itself, timed by the survey in the build container (it is not
+# built here: its build needs Boost); beside the port's rate for scale.
+REFERENCE_MEASURED = {
+    "value": 136.0, "unit": "CL pts/s", "cores": 8, "host": "8-core Intel Xeon @ 2.70 GHz, 1 thread per core",
+    "workload": "2,500 CL points spread over a 524,288-triangle heightfield (8 bumps, no steps), BallCutter d=10",
+    "source": "survey measurement of the reference's own build (not on the GPU box)"}
 
 
 def time_mode(mesh, spec, d_cl, n, mode, outs, steps, warmup, torch):
@@ -210,12 +249,14 @@ PUSH_SPECS = {"bull": {"kind": "bull", "d": 10.0, "r": 2.0, "L": 20.0},
 def bench_push(capi, W, cpu_fibers, do_cpu):
     """Waterline fibers on the 204,802-triangle sphere scene: 2,000 X and
     2,000 Y fibers at each of 20 levels (80,000 fibers) per cutter, the
-    reference's KD-search candidate set, FP64.  Kernel time from HIP events
-    on the launch stream (ocl_stats.kernel_ms), summed over the 40 calls."""
+    reference's KD-search candidate set, FP64, X and Y in one launch
+    (ocl_hip_push_xy).  A first call builds both directions' trees and sizes
+    the interval pools; the second is timed.  Kernel time from HIP events on
+    the launch stream (ocl_stats.kernel_ms)."""
     tris = W.config2_mesh()
     levels = W.config2_levels()
     dev = capi.DeviceMesh(tris)
-    out = {"workload": f"{len(tris)}-triangle sphere scene, 20 levels x (2000 X + 2000 Y fibers)"}
+    out = {"workload": f"{len(tris)}-triangle sphere scene, 20 levels x (2000 X + 2000 Y fibers), one launch"}
     om = None
     for name, spec in PUSH_SPECS.items():
         fibs = {"x": [], "y": []}
@@ -224,22 +265,22 @@ def bench_push(capi, W, cpu_fibers, do_cpu):
             fibs["x"].append(xf)
             fibs["y"].append(yf)
         fibs = {d: np.concatenate(v) for d, v in fibs.items()}
-        for d in ("x", "y"):  # warm-up: builds the direction's tree
-            dev.push(spec, fibs[d][:64], d)
-        kms = 0.0
-        nint = pairs = brent = 0
+        dev.push_xy(spec, fibs["x"], fibs["y"])  # warm-up: trees and pool sizes
         t0 = time.perf_counter()
-        for d in ("x", "y"):
-            r = dev.push(spec, fibs[d], d)
-            kms += r["stats"]["kernel_ms"]
-            nint += len(r["lohi"])
-            pairs += r["stats"]["pairs"]
-            brent += r["stats"]["brent_iters"]
+        r = dev.push_xy(spec, fibs["x"], fibs["y"])
         wall = time.perf_counter() - t0
+        st = r["stats"]
+        kms = st["kernel_ms"]
+        nint = len(r["x"]["lohi"]) + len(r["y"]["lohi"])
         nf = len(fibs["x"]) + len(fibs["y"])
-        line = {"fibers": nf, "intervals": nint, "pairs": int(pairs), "brent_iters": int(brent),
+        ach = st["pairs"] * F_PUSH[name] / (kms * 1e-3) / 1e12
+        line = {"fibers": nf, "intervals": nint, "pairs": int(st["pairs"]), "brent_iters": int(st["brent_iters"]),
                 "kernel_ms": kms, "fibers_per_s": nf / (kms / 1e3), "intervals_per_s": nint / (kms / 1e3),
-                "wall_s_incl_host": wall}
+                "wall_s_incl_host": wall,
+                "roofline": {"bound": "valu", "kernel": f"k_push_fibers<{name}>", "achieved": ach,
+                             "peak": PEAK_FP64, "unit": "TFLOP/s (FP64)", "frac": ach / PEAK_FP64,
+                             "flop_per_pair": F_PUSH[name], "pairs_per_launch": int(st["pairs"]),
+                             "kernel_ms": kms, "traffic": None}}
         if do_cpu:
             from oracle import oracle as O
             if om is None:
@@ -256,6 +297,44 @@ def bench_push(capi, W, cpu_fibers, do_cpu):
         log(f"push {name}: {nf} fibers, {nint} intervals, kernel {kms:.1f} ms")
     dev.close()
     return out
+
+
+def bench_refine(W, tris, tol=0.01, sampling=1.0, min_step=0.01, z0=-20.0):
+    """Slope-driven refinement demo: 500 random polylines (3-6 vertices,
+    seed 44, over 200 x 200 mm) on the 524,288-triangle heightfield,
+    BallCutter d=10 L=20, ocl::AdaptivePathDropCutter to a 0.01 mm chord
+    tolerance (first samples 1 mm apart, minimum step 0.01 mm): the C++ host
+    loop refines level by level, each level one device batch (mixed
+    precision) of the midpoints and quarter points still open.  Times run()
+    (setSTL, the device mesh build, is outside)."""
+    from opencamlib_amd import ocl
+    polys = W.random_polylines(500, 200.0, seed=44)
+    surf = ocl.STLSurf()
+    for t in np.asarray(tris).reshape(-1, 9):
+        surf.addTriangle(ocl.Triangle(ocl.Point(*t[0:3]), ocl.Point(*t[3:6]), ocl.Point(*t[6:9])))
+    path = ocl.Path()
+    for poly in polys:
+        for a, b in zip(poly[:-1], poly[1:]):
+            path.append(ocl.Line(ocl.Point(a[0], a[1], z0), ocl.Point(b[0], b[1], z0)))
+    apdc = ocl.AdaptivePathDropCutter()
+    apdc.setSTL(surf)
+    apdc.setCutter(ocl.BallCutter(10.0, 20.0))
+    apdc.setPath(path)
+    apdc.setSampling(sampling)
+    apdc.setMinSampling(min_step)
+    apdc.setChordTolerance(tol)
+    t0 = time.perf_counter()
+    apdc.run()
+    secs = time.perf_counter() - t0
+    npts = len(apdc.getCLPoints())
+    length = float(sum(np.sum(np.hypot(*np.diff(p, axis=0).T)) for p in polys))
+    return {"workload": "500 random polylines (164 m of path) on the 524,288-triangle heightfield, BallCutter "
+                        "d=10, AdaptivePathDropCutter to 0.01 mm chord tolerance (1 mm first samples, 0.01 mm "
+                        "minimum step)",
+            "path_mm": length, "points": npts, "levels": apdc.getLevels(), "seconds": secs,
+            "value": npts / secs, "unit": "refined CL pts/s (host loop + device batches)",
+            "uniform_points_at_min_step": int(length / min_step),
+            "points_saved_vs_uniform": length / min_step / npts}
 
 
 def bench_batch(torch, capi, W, steps):
@@ -300,6 +379,20 @@ def bench_batch(torch, capi, W, steps):
     b = capi.Batch(cj, "mixed", {**best, "use_graph": 1})
     ms_graph = timed(b)
     b.close()
+    # the persistent selection launch alone (eager run, HIP events around it)
+    # and each job's evaluated pairs, for the selection kernel's roofline
+    b = capi.Batch(cj, "mixed", {**best, "use_graph": 0})
+    b.run(stream=stream, want_stats=False)
+    st = b.run(stream=stream, want_stats=True)
+    sel_flops = 0.0
+    sel_pairs = 0
+    for jn, j in enumerate(jobs):
+        if j["spec"]["kind"] in F_PAIR:
+            p = b.job_stats(jn)["pairs"]
+            sel_pairs += p
+            sel_flops += p * F_PAIR[j["spec"]["kind"]]
+    b.close()
+    sel_ach = sel_flops / (st["select_ms"] * 1e-3) / 1e12
     b = capi.Batch(cj, "mixed", {"per_job_launch": 1})
     ms_perjob = timed(b)
     b.close()
@@ -327,7 +420,12 @@ def bench_batch(torch, capi, W, steps):
             "graph_persistent": {"ms_per_step": ms_graph, "value": total / (ms_graph / 1e3)},
             "per_job_launch": {"ms_per_step": ms_perjob, "value": total / (ms_perjob / 1e3)},
             "call_per_job": {"ms_per_step": ms_single, "value": total / (ms_single / 1e3)},
-            "unit": "CL pts/s", "speedup_vs_call_per_job": ms_single / ms_graph}
+            "unit": "CL pts/s", "speedup_vs_call_per_job": ms_single / ms_graph,
+            "roofline": {"bound": "valu", "kernel": "k_drop_select_jobs<float>", "achieved": sel_ach,
+                         "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": sel_ach / PEAK_FP32,
+                         "frac_unpacked": sel_ach / PEAK_FP32_UNPACKED, "pairs_per_launch": int(sel_pairs),
+                         "flop_per_pair": "per job: " + json.dumps(F_PAIR), "kernel_ms": st["select_ms"],
+                         "eager_ms_per_step": st["kernel_ms"], "traffic": None}}
 
 
 def main():
@@ -448,7 +546,12 @@ def main():
         out["cpu_baseline"] = {
             "value": args.cpu_kd_points / cb["t_kd"], "unit": "CL pts/s", "cores": cb["threads"], "kind": "port",
             "sample": f"{args.cpu_kd_points} seeded CL points of the same 1M grid, FP64 oracle with the "
-                      f"reference's KD-tree search (bucket 1), OpenMP dynamic over points"}
+                      f"reference's KD-tree search (bucket 1), OpenMP dynamic over points, {cb['threads']} threads",
+            "host": host_cpu(),
+            "note": "a C restatement of the reference's algorithm (same KD-tree split and search, same contact "
+                    "math), not the reference: it skips the reference's per-query std::list copy of every "
+                    "candidate triangle (kdtree.hpp:276-277), which dominates the reference's time",
+            "reference_measured": REFERENCE_MEASURED}
         out["cpu_scan_oracle"] = {"value": args.cpu_scan_points / cb["t_scan"], "unit": "CL pts/s",
                                   "cores": cb["threads"],
                                   "sample": f"{args.cpu_scan_points} seeded CL points, exhaustive overlap scan "
@@ -464,10 +567,15 @@ def main():
         b_ms, b_sel, b_st = time_mode(mesh, bspec, d_cl, n, "mixed", outs, max(3, args.steps // 2), 1, torch)
         b_mixed = fetch()
         f_ms, f_sel, _ = time_mode(mesh, bspec, d_cl, n, "fp64", outs, 3, 1, torch)
+        b_ach = b_st["pairs"] * F_PAIR["bull"] / (b_sel * 1e-3) / 1e12
         bull = {"mixed": {"value": n / (b_ms / 1e3), "ms_per_step": b_ms, "select_ms": b_sel,
                           "escalated": b_st["escalated"], "brent_iters": b_st["brent_iters"]},
                 "fp64_gpu": {"value": n / (f_ms / 1e3), "ms_per_step": f_ms, "select_ms": f_sel},
-                "mixed_over_fp64": f_ms / b_ms}
+                "mixed_over_fp64": f_ms / b_ms,
+                "roofline": {"bound": "valu", "kernel": "k_drop_select<float,bull>", "achieved": b_ach,
+                             "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": b_ach / PEAK_FP32,
+                             "frac_unpacked": b_ach / PEAK_FP32_UNPACKED, "pairs_per_launch": int(b_st["pairs"]),
+                             "flop_per_pair": F_PAIR["bull"], "kernel_ms": b_sel, "traffic": None}}
         if not args.no_cpu:
             from oracle import oracle as O
             idx = np.sort(np.random.default_rng(100).choice(n, 20_000, replace=False))
@@ -482,6 +590,9 @@ def main():
     if not args.no_extra:
         out["push"] = bench_push(capi, W, args.cpu_push_fibers, not args.no_cpu)
         out["batch"] = bench_batch(torch, capi, W, max(3, args.steps // 2))
+        out["refine"] = bench_refine(W, tris)
+        log(f"refine: {out['refine']['points']} points in {out['refine']['seconds']:.2f} s, "
+            f"{out['refine']['levels']} levels")
         log(f"batch: graph {out['batch']['graph_persistent']['ms_per_step']:.2f} ms, per-job launch "
             f"{out['batch']['per_job_launch']['ms_per_step']:.2f} ms, call per job "
             f"{out['batch']['call_per_job']['ms_per_step']:.2f} ms")

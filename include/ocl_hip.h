@@ -244,6 +244,10 @@ int ocl_hip_batch_create(const ocl_drop_job* jobs, size_t njobs, int precision, 
  * points, pairs, escalated, wave_evals and kernel_ms (select_ms: the
  * selection launch, eager runs only). */
 int ocl_hip_batch_run(ocl_batch* batch, ocl_stats* stats, void* stream);
+/* Counters of one job of the last ocl_hip_batch_run (synchronous): points,
+ * pairs evaluated by the selection, escalated, wave_evals.  No counterpart
+ * in the reference (a diagnostic, as BatchDropCutter::getCalls). */
+int ocl_hip_batch_job_stats(ocl_batch* batch, size_t job, ocl_stats* stats);
 int ocl_hip_batch_destroy(ocl_batch* batch);
 /* Autotuner: times every combination of hilbert_bits {7, 8, 9} and
  * blocks_per_cu {2, 4, 8} on the jobs themselves (graph replays, best of

@@ -113,6 +113,8 @@ def lib():
         L.ocl_hip_batch_create.restype = C.c_int
         L.ocl_hip_batch_run.argtypes = [vp, C.POINTER(Stats), vp]
         L.ocl_hip_batch_run.restype = C.c_int
+        L.ocl_hip_batch_job_stats.argtypes = [vp, C.c_size_t, C.POINTER(Stats)]
+        L.ocl_hip_batch_job_stats.restype = C.c_int
         L.ocl_hip_batch_destroy.argtypes = [vp]
         L.ocl_hip_batch_destroy.restype = C.c_int
         L.ocl_hip_batch_autotune.argtypes = [C.POINTER(DropJob), sz, C.c_int, C.c_int, C.POINTER(Tune),
@@ -366,6 +368,13 @@ class Batch:
         st = Stats()
         check(lib().ocl_hip_batch_run(self.h, C.byref(st) if want_stats else None, stream))
         return st.as_dict() if want_stats else None
+
+    def job_stats(self, j):
+        """Counters of job j in the last run (points, pairs, escalated,
+        wave_evals)."""
+        st = Stats()
+        check(lib().ocl_hip_batch_job_stats(self.h, j, C.byref(st)))
+        return st.as_dict()
 
     def close(self):
         if getattr(self, "h", None):

@@ -1143,11 +1143,9 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
         }
         write_drop(o, m.order, src, L);
     }
-    const unsigned long long brent = 0;
-    if (cnt) {
+    if (cnt) {  // the FP32/FP64 selection runs no Brent solves
         wave_add(&cnt->pairs, act ? S.pairs : 0);
         wave_add(&cnt->escalated, (act && up) ? 1 : 0);
-        wave_add(&cnt->brent, brent);
         wave_add(&cnt->evals, lane_id() == 0 ? evals : 0);
     }
 }
@@ -3577,6 +3575,23 @@ int ocl_hip_batch_run(ocl_batch* b, ocl_stats* stats, void* stream_) {
             stats->wave_evals += hc.evals;
         }
     }
+    return OCL_OK;
+}
+
+int ocl_hip_batch_job_stats(ocl_batch* b, size_t job, ocl_stats* stats) {
+    using namespace ocl_amd;
+    if (!b || !stats) return fail(OCL_E_ARG, "batch / stats is NULL");
+    if (job >= b->jobs.size()) return fail(OCL_E_ARG, "job index out of range");
+    std::memset(stats, 0, sizeof(*stats));
+    const BatchJob& J = b->jobs[job];
+    stats->points = J.n;
+    if (J.n == 0) return OCL_OK;
+    Counters hc;
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(&hc, J.w->counters.p, sizeof(hc), hipMemcpyDeviceToHost));
+    stats->pairs = hc.pairs;
+    stats->escalated = hc.escalated;
+    stats->wave_evals = hc.evals;
     return OCL_OK;
 }
 

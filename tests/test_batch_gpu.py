@@ -98,8 +98,13 @@ def test_batch_equals_single_jobs(djobs, precision):
         djobs.reset_outputs()
         b = capi.Batch(djobs.ctypes_jobs(), precision, tune)
         st = b.run()
+        per_job = [b.job_stats(j) for j in range(len(djobs.jobs))]
         got = djobs.snapshot()
         b.close()
+        # the per-job counters add up to the batch's
+        assert [p["points"] for p in per_job] == [len(j["cl"]) for j in djobs.jobs]
+        assert sum(p["pairs"] for p in per_job) == st["pairs"]
+        assert sum(p["escalated"] for p in per_job) == st["escalated"]
         print(f"[batch] {precision} tune={tune} jobs={len(got)} kinds={kinds} stats={st}", flush=True)
         fields = FIELDS if precision != "fp32" else ("z", "type", "tri", "feat")
         assert same_bits(got, ref, fields)
