@@ -556,6 +556,15 @@ OCL_HD void sel_init(Sel<T>& S, T z0) {
     S.pairs = 0;
 }
 
+// true if b holds for any lane of the wavefront (for one lane on the host)
+OCL_HD bool sel_any(bool b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __any(b) != 0;
+#else
+    return b;
+#endif
+}
+
 // slot of the CC type of a feature: 0 vertex, 1 edge, 2 facet/tip, 3 rim
 OCL_HD int feat_slot(int f) {
     const int k = f & 15;
@@ -616,9 +625,8 @@ OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>&
     const bool ve = live & !flift & !rlift;
     // ---------------- vertices
     constexpr bool kRun = sizeof(T) == 4;  // track the runner-up upper value (FP32)
-    T zv = NEG, uvb = NEG, uvo = NEG;  // best vertex; upper values of the best and of the others
-    T uvn = NEG;                       // upper values of vertices that are not the winner's feature
-    uint32_t idv = 0;                  // feature ID of the best vertex
+    T upv[3], upe[3];                      // upper value of each valid contact (runner-up, FP32)
+    T zv = NEG;
     int fv = FT_NONE;
     bool vamb = false;
 #pragma unroll
@@ -632,20 +640,13 @@ OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>&
         S.za = F::max_(S.za, amb ? z + P.EAV : NEG);
         th = F::max_(th, F::max_(up, amb ? z + P.EAV : NEG));
         const bool take = val & (z > zv);
-        if constexpr (kRun) {
-            const uint32_t id = id_of((float)rec[R_FID0 + j]);
-            uvo = F::max_(uvo, take ? uvb : up);
-            uvb = take ? up : uvb;
-            uvn = F::max_(uvn, id != S.wid ? up : NEG);
-            idv = take ? id : idv;
-        }
+        if constexpr (kRun) upv[j] = up;
         zv = take ? z : zv;
         fv = take ? (FT_VTX0 + j) : fv;
         vamb = take ? amb : vamb;
     }
     // ---------------- edges
-    T ze = NEG, ee = (T)0, ueb = NEG, ueo = NEG, uen = NEG;
-    uint32_t ide = 0;
+    T ze = NEG, ee = (T)0;
     int fe = FT_NONE;
     bool eamb = false;
 #pragma unroll
@@ -720,13 +721,7 @@ OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>&
         S.za = F::max_(S.za, amb ? z + eae : NEG);
         th = F::max_(th, F::max_(up, amb ? z + eae : NEG));
         const bool take = val & (z > ze);
-        if constexpr (kRun) {
-            const uint32_t id = id_of((float)rec[R_FID0 + 3 + j]);
-            ueo = F::max_(ueo, take ? ueb : up);
-            ueb = take ? up : ueb;
-            uen = F::max_(uen, id != S.wid ? up : NEG);
-            ide = take ? id : ide;
-        }
+        if constexpr (kRun) upe[j] = up;
         ze = take ? z : ze;
         ee = take ? eze : ee;
         fe = take ? (FT_EDGE0 + j) : fe;
@@ -741,9 +736,29 @@ OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>&
     const bool wlift = (fw != FT_NONE) & (zw > S.zb);
     // runner-up: when the triangle lifts, every other valid contact of it and
     // the previous winner (unless it is a copy of the same feature); when it
-    // does not, its valid contacts that are not copies of the winner's feature
+    // does not, its valid contacts that are not copies of the winner's feature.
+    // Skipped when every lane's contacts lie more than CUT below its running
+    // best: they cannot reach the final winner's band (its lower end is at
+    // least the running best less the winner's error bound, below CUT), and
+    // no lane lifts.
     const bool any = flift | rlift | wlift;
     if constexpr (kRun) {
+      if (sel_any(live & (th >= S.zb - P.CUT))) {
+        T uvb = NEG, uvo = NEG, uvn = NEG, ueb = NEG, ueo = NEG, uen = NEG;
+        uint32_t idv = 0, ide = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const uint32_t iv = id_of((float)rec[R_FID0 + j]), ie = id_of((float)rec[R_FID0 + 3 + j]);
+            const bool bv = fv == FT_VTX0 + j, be = fe == FT_EDGE0 + j;
+            uvb = bv ? upv[j] : uvb;
+            uvo = F::max_(uvo, bv ? NEG : upv[j]);
+            uvn = F::max_(uvn, iv != S.wid ? upv[j] : NEG);
+            idv = bv ? iv : idv;
+            ueb = be ? upe[j] : ueb;
+            ueo = F::max_(ueo, be ? NEG : upe[j]);
+            uen = F::max_(uen, ie != S.wid ? upe[j] : NEG);
+            ide = be ? ie : ide;
+        }
         const T uf = fval ? zf + ezf : NEG;
         // a facet or rim lift skips the vertices and edges (their values are
         // -inf then)
@@ -755,6 +770,7 @@ OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>&
         const T uprev = (S.bt != 0xffffffffu && S.wid != wnew) ? S.zb + S.eb : NEG;
         S.zr = F::max_(S.zr, any ? F::max_(uprev, uexc) : unew);
         S.wid = any ? wnew : S.wid;
+      }
     }
     const T zn = flift ? zf : (rlift ? zr : zw);
     const T en = (flift | rlift) ? ezf : ew;
