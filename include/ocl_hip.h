@@ -175,11 +175,12 @@ int ocl_hip_drop_device_noalloc(ocl_mesh* mesh, const ocl_cutter_desc* cutter, c
  * precision: OCL_PREC_FP32 / OCL_PREC_MIXED compute facet and vertex contacts
  * in FP32 (FP64 where the profile is ill-conditioned) and edge contacts in
  * FP64; OCL_PREC_FP64 / OCL_PREC_EXACT64 compute everything in FP64.
- * d_dz and d_grad_r are deterministic (d_grad_r is summed per block, then by
- * one fixed-order reduction); d_grad_tri is accumulated with FP64 atomics,
- * so its last bits vary run to run.  The per-block partial sums live in the
- * mesh handle: device calls on one mesh must run on one stream, or be
- * ordered with events, like the reference's non-re-entrant Operation. */
+ * Every output is deterministic, bit for bit: d_grad_r is summed per block,
+ * then by one fixed-order reduction; d_grad_tri sums each triangle's terms in
+ * point order (a stable sort of the points by contact triangle, then
+ * segmented sums), with no atomics.  The partial sums and the sort's buffers
+ * live in the mesh handle: device calls on one mesh must run on one stream,
+ * or be ordered with events, like the reference's non-re-entrant Operation. */
 int ocl_hip_drop_backward(ocl_mesh* mesh, const ocl_cutter_desc* cutter, const double* d_cl_xyz, size_t n,
                           const double* d_cc_xyz, const uint8_t* d_cc_type, const uint32_t* d_tri,
                           const uint8_t* d_feat, const double* d_grad_z, int precision, double* d_dz,

@@ -241,6 +241,32 @@ def test_device_backward_config4_full(gpu):
 
 
 @pytest.mark.gpu
+def test_backward_vertex_sums_are_deterministic_over_long_segments(gpu):
+    """Two large triangles under 200,000 random points: each triangle's terms
+    span hundreds of 256-point blocks of the sorted order, so its sum is
+    carried across blocks.  The sums match the oracle's per-point gradients
+    summed per triangle, and repeated calls give the same bits (the sums
+    have one fixed order, no atomics)."""
+    import torch
+    from opencamlib_amd import capi
+    tris = np.array([[0, 0, 0, 100, 0, 10, 100, 100, 20], [0, 0, 0, 100, 100, 20, 0, 100, 10],
+                     [200, 0, 0, 210, 0, 0, 200, 10, 0]], dtype=float)
+    cl = W.cl_random(200_000, 100.0, z0=-10.0, seed=3)
+    _check_device_vs_oracle(tris, BALL10, cl, "exact64", "fp64", "long-segments")
+    gz = np.random.default_rng(5).normal(size=len(cl))
+    dev = capi.DeviceMesh(tris)
+    fw = _device_forward(dev, BALL10, cl, "mixed")
+    runs = [_device_backward(dev, BALL10, fw, len(tris), gz, "mixed") for _ in range(3)]
+    dev.close()
+    for dz, gtri, gr in runs[1:]:
+        assert np.array_equal(dz, runs[0][0])
+        assert np.array_equal(gtri, runs[0][1])
+        assert gr == runs[0][2]
+    assert np.all(runs[0][1][2] == 0.0)  # the far triangle is under no point
+    del torch
+
+
+@pytest.mark.gpu
 def test_autograd_function_routes_gradients(gpu, grad_scene):
     import torch
     from opencamlib_amd import capi
