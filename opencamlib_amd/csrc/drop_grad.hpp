@@ -17,9 +17,11 @@
 //   - cone tip on a facet (q = 0): z is the plane height under the CL point.
 // Composite cutters use the active sub-cutter (a vertex contact: the
 // sub-cutter owning its radius, compositecutter.cpp:58-61); their dz/dr is
-// NaN.  Cone edges follow the geometric cone contact: for a cone with l > 0
-// the reference's edge formula (conecutter.cpp:115-122, square(length)) is
-// not the geometric contact, and its z has a different slope there.
+// NaN.  A cone edge contact in the reference's hyperbola case is
+// differentiated as the reference writes it (cone_hyper_edge_grad): its
+// formula takes the cutter's length where the cone's contact takes the
+// center height, so for a cone with a shank (l > 0) it is not the cone's
+// envelope and the envelope theorem does not hold there.
 //
 // T = float computes facet and vertex contacts in FP32 on coordinates
 // relative to the contact point; edge contacts always run in FP64.
@@ -58,6 +60,61 @@ OCL_HD void prof_slope(const CutterP& s, T q, T& hp, T& hr) {
 
 OCL_HD double rim_height_slope(const CutterP& s) {
     return s.kind == K_BALL ? 1.0 : (s.kind == K_CONE ? 1.0 / s.tan_a : 0.0);
+}
+
+// The reference's cone edge drop at (x, y) takes its hyperbola case for the
+// edge a -> b (conecutter.cpp:115-117).
+OCL_HD bool cone_edge_hyper(const CutterP& s, V3 a, V3 b, double x, double y) {
+    const double ex = b.x - a.x, ey = b.y - a.y;
+    const double lxy = sqrt(ex * ex + ey * ey);
+    const double d = fabs((ex * (y - a.y) - ey * (x - a.x)) / lxy);
+    const double m = (b.z - a.z) / lxy;
+    const double xu = sqrt(s.radius * s.radius - d * d);
+    const double mu = (s.ch / s.radius) * xu / sqrt(xu * xu + d * d);
+    return fabs(m) <= fabs(mu);
+}
+
+// Derivatives of the reference's cone edge height in its hyperbola case
+// (conecutter.cpp:109-135), z = a.z + M s + G d: s and d the CL point's
+// offset along and distance from the edge's XY line (direction v, left
+// normal n, signed offset c = n.(cl - a), d = |c|), M the edge slope,
+// G = K M^2 - S / tan, K = r / Q, Q = sqrt(L^2 - r^2 M^2),
+// S = sqrt(K^2 M^2 + 1), L = r / tan + l (l, the shank, fixed when r
+// varies).  Differentiated by hand: dz/dcl = M v + G sgn(c) n,
+// dz/dr = d dG/dr, and for the end points, with P = s + d dG/dM and
+// |e| the edge's XY length:
+//   dz/db.xy = -P M v / |e| + (M c - G sgn(c) s) n / |e|,  dz/db.z = P / |e|,
+//   dz/da.xy = -dz/db.xy - dz/dcl,                        dz/da.z = 1 - P / |e|.
+OCL_HD void cone_hyper_edge_grad(const CutterP& cs, V3 a, V3 b, double x, double y, double* g9) {
+    const double T = cs.tan_a, l = cs.length - cs.radius / T, R = cs.radius;
+    const double ex = b.x - a.x, ey = b.y - a.y;
+    const double le = sqrt(ex * ex + ey * ey);
+    const double vx = ex / le, vy = ey / le, nx = -vy, ny = vx;
+    const double M = (b.z - a.z) / le;
+    const double wx = x - a.x, wy = y - a.y;
+    const double s = wx * vx + wy * vy, c = wx * nx + wy * ny;
+    const double sg = c < 0.0 ? -1.0 : 1.0, d = fabs(c);
+    const double L = R / T + l, M2 = M * M;
+    const double Q = sqrt(L * L - R * R * M2);
+    const double K = R / Q, S = sqrt(K * K * M2 + 1.0);
+    const double G = K * M2 - S / T;
+    const double KM = R * R * R * M / (Q * Q * Q);               // dK/dM
+    const double KR = 1.0 / Q - R * (L / T - R * M2) / (Q * Q * Q);  // dK/dr
+    const double GM = KM * M2 + 2.0 * K * M - (K * M2 * KM + K * K * M) / (S * T);
+    const double GR = KR * (M2 - K * M2 / (S * T));
+    const double P = s + d * GM;
+    const double gx = M * vx + G * sg * nx, gy = M * vy + G * sg * ny;
+    const double bn = (M * c - G * sg * s) / le;
+    const double bx = -P * M * vx / le + bn * nx, by = -P * M * vy / le + bn * ny;
+    g9[0] = gx;
+    g9[1] = gy;
+    g9[2] = d * GR;
+    g9[3] = -bx - gx;
+    g9[4] = -by - gy;
+    g9[5] = 1.0 - P / le;
+    g9[6] = bx;
+    g9[7] = by;
+    g9[8] = P / le;
 }
 
 // Sub-cutter of a composite contact: facets and edges carry it in the
@@ -118,6 +175,18 @@ __host__ __device__ __attribute__((always_inline)) inline GradOut contact_grad(c
         const double t = ((P.x - pa.x) * ex + (P.y - pa.y) * ey) / (ex * ex + ey * ey);
         const double ddx = P.x - x, ddy = P.y - y;
         const double dq = sqrt(ddx * ddx + ddy * ddy);
+        if (s.kind == K_CONE && cone_edge_hyper(s, pa, pb, x, y)) {
+            double g9[9];
+            cone_hyper_edge_grad(s, pa, pb, x, y, g9);
+            o.gx = g9[0];
+            o.gy = g9[1];
+            o.gr = g9[2];
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) o.gv[3 * j + q] = (j == a) ? g9[3 + q] : ((j == b) ? g9[6 + q] : 0.0);
+            return o;
+        }
         double egx, egy, egr;
         const bool rim = s.kind == K_CYL || (s.kind == K_CONE && dq >= s.radius * (1.0 - 1e-9));
         if (rim) {

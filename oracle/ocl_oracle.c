@@ -1910,7 +1910,10 @@ long oc_batch_push(oc_mesh* m, const oc_cutter* c, int dir, const double* fib, l
        lambda = (grad_s P.z . grad_s c)/|grad_s c|^2, c = q^2 - r^2, so
        dz/dx = 2 lambda (P.x - x), dz/dP.xy = -2 lambda (P - cl)_xy,
        dz/dr = -d h(r)/dr + 2 lambda r;
-     - cone tip on a facet (q = 0): z is the plane height under the CL point.
+     - cone tip on a facet (q = 0): z is the plane height under the CL point;
+     - cone edge in the reference's hyperbola case: its formula as written
+       (cone_hyper_edge_grad), which for a cone with a shank is not the
+       envelope of the cone.
    out = dz/dx, dz/dy, dz/dr, dz/dp0.xyz, dz/dp1.xyz, dz/dp2.xyz.
    Composite cutters: the derivatives of the active sub-cutter; dz/dr is NaN
    (no single radius).  Returns 1 when there is no contact. */
@@ -1942,6 +1945,82 @@ static double rim_height_slope(const oc_cutter* s) {
     case OC_CONE: return 1.0 / tan(s->angle);
     default: return 0.0;
     }
+}
+
+/* Forward-mode derivatives in 9 directions (x, y, r, edge start xyz, edge
+   end xyz) of the reference's cone edge height in its hyperbola case
+   (conecutter.cpp:109-135).  There z = a.z + M s + G d, with s and d the CL
+   point's offset along and distance from the edge's XY line, M the edge's
+   slope and G = K M^2 - sqrt(K^2 M^2 + 1) / tan(angle),
+   K = r / sqrt(length^2 - r^2 M^2).  The formula takes the cutter's length
+   (center height + shank l) where the cone's true contact takes its center
+   height, so for l > 0 its CC point is not where the slopes match and z is
+   not stationary in the contact parameter: the envelope theorem does not
+   hold, and z is differentiated as written.  (For l = 0, G = -sqrt(ch^2 -
+   r^2 M^2)/r and the result equals the envelope form.) */
+typedef struct {
+    double v, g[9];
+} D9;
+static D9 d9(double v, int i) {
+    D9 r;
+    r.v = v;
+    for (int k = 0; k < 9; ++k) r.g[k] = (k == i) ? 1.0 : 0.0;
+    return r;
+}
+static D9 d9_add(D9 a, D9 b) { for (int k = 0; k < 9; ++k) a.g[k] += b.g[k]; a.v += b.v; return a; }
+static D9 d9_sub(D9 a, D9 b) { for (int k = 0; k < 9; ++k) a.g[k] -= b.g[k]; a.v -= b.v; return a; }
+static D9 d9_mul(D9 a, D9 b) {
+    D9 r;
+    r.v = a.v * b.v;
+    for (int k = 0; k < 9; ++k) r.g[k] = a.g[k] * b.v + a.v * b.g[k];
+    return r;
+}
+static D9 d9_div(D9 a, D9 b) {
+    D9 r;
+    r.v = a.v / b.v;
+    for (int k = 0; k < 9; ++k) r.g[k] = (a.g[k] * b.v - a.v * b.g[k]) / (b.v * b.v);
+    return r;
+}
+static D9 d9_sqrt(D9 a) {
+    D9 r;
+    r.v = sqrt(a.v);
+    for (int k = 0; k < 9; ++k) r.g[k] = a.g[k] / (2.0 * r.v);
+    return r;
+}
+static D9 d9_scale(D9 a, double s) { for (int k = 0; k < 9; ++k) a.g[k] *= s; a.v *= s; return a; }
+
+/* 1 when the reference's cone edge drop at (x, y) takes its hyperbola case
+   for edge a -> b (conecutter.cpp:115-117) */
+static int cone_edge_hyper(const oc_cutter* s, P3 a, P3 b, double x, double y) {
+    const double ex = b.x - a.x, ey = b.y - a.y;
+    const double lxy = sqrt(ex * ex + ey * ey);
+    const double d = fabs((ex * (y - a.y) - ey * (x - a.x)) / lxy);
+    const double m = (b.z - a.z) / lxy;
+    const double xu = sqrt(sq(s->radius) - sq(d));
+    const double mu = (s->center_height / s->radius) * xu / sqrt(sq(xu) + sq(d));
+    return fabs(m) <= fabs(mu);
+}
+
+static void cone_hyper_edge_grad(const oc_cutter* s, P3 a, P3 b, double x, double y, double* g9) {
+    const double tana = tan(s->angle);
+    const double shank = s->length - s->radius / tana; /* l: fixed when r varies */
+    const D9 X = d9(x, 0), Y = d9(y, 1), R = d9(s->radius, 2);
+    const D9 ax = d9(a.x, 3), ay = d9(a.y, 4), az = d9(a.z, 5);
+    const D9 bx = d9(b.x, 6), by = d9(b.y, 7), bz = d9(b.z, 8);
+    const D9 ex = d9_sub(bx, ax), ey = d9_sub(by, ay);
+    const D9 lxy = d9_sqrt(d9_add(d9_mul(ex, ex), d9_mul(ey, ey)));
+    const D9 vx = d9_div(ex, lxy), vy = d9_div(ey, lxy);
+    const D9 M = d9_div(d9_sub(bz, az), lxy);
+    const D9 wx = d9_sub(X, ax), wy = d9_sub(Y, ay);
+    const D9 sa = d9_add(d9_mul(wx, vx), d9_mul(wy, vy));
+    D9 d = d9_sub(d9_mul(vx, wy), d9_mul(vy, wx));
+    if (d.v < 0.0) d = d9_scale(d, -1.0);
+    const D9 L = d9_add(d9_scale(R, 1.0 / tana), d9(shank, -1));
+    const D9 M2 = d9_mul(M, M);
+    const D9 K = d9_div(R, d9_sqrt(d9_sub(d9_mul(L, L), d9_mul(d9_mul(R, R), M2))));
+    const D9 G = d9_sub(d9_mul(K, M2), d9_scale(d9_sqrt(d9_add(d9_mul(d9_mul(K, K), M2), d9(1.0, -1))), 1.0 / tana));
+    const D9 z = d9_add(az, d9_add(d9_mul(M, sa), d9_mul(G, d)));
+    for (int k = 0; k < 9; ++k) g9[k] = z.g[k];
 }
 
 int oc_drop_grad(const oc_cutter* c, const double* tri9, int feat, int cctype, double x, double y,
@@ -1986,6 +2065,21 @@ int oc_drop_grad(const oc_cutter* c, const double* tri9, int feat, int cctype, d
         fs[1] = e2.z;
         cs[0] = 2.0 * (dx * e1.x + dy * e1.y);
         cs[1] = 2.0 * (dx * e2.x + dy * e2.y);
+    }
+    if (m == 1 && s->kind == OC_CONE) {
+        const int a = k - F_EDGE0, b = (a + 1) % 3;
+        if (cone_edge_hyper(s, V[a], V[b], x, y)) {
+            double g9[9];
+            cone_hyper_edge_grad(s, V[a], V[b], x, y, g9);
+            out[0] = g9[0];
+            out[1] = g9[1];
+            out[2] = (c->kind == OC_COMPOSITE) ? NAN : g9[2];
+            for (int j = 0; j < 3; ++j) {
+                out[3 + 3 * a + j] = g9[3 + j];
+                out[3 + 3 * b + j] = g9[6 + j];
+            }
+            return 0;
+        }
     }
     double gx, gy, gr;
     double gxy_v[2]; /* dz/dP.xy, spread by w */

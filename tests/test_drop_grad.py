@@ -24,7 +24,10 @@ SWITCH = 0.01   # distance from a contact switch (mm)
 # cone with l = 0: the reference's edge formula then is the true cone contact
 # (conecutter.cpp:115-122 use length for center_height)
 CONE_L0 = {"kind": "cone", "d": 10.0, "angle": math.pi / 4, "L": 0.0}
-CASES = {"cyl": CYL6, "ball": BALL10, "bull": BULL10, "cone": CONE_L0, "cylball": CYLBALL}
+# with a shank (l = 10): the reference's edge formula is then not the cone's
+# envelope, and the gradient differentiates the formula as written
+CONE_L10 = {"kind": "cone", "d": 10.0, "angle": math.pi / 4, "L": 10.0}
+CASES = {"cyl": CYL6, "ball": BALL10, "bull": BULL10, "cone": CONE_L0, "cone_shank": CONE_L10, "cylball": CYLBALL}
 
 
 def _spec_d(spec, dd):
