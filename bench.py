@@ -46,7 +46,7 @@ PEAK_FP64 = 78.6
 # bounds; add/sub/mul = 1, fma = 2, sqrt/rsqrt/rcp/div = 1), counted by
 # running that code on the host with a counting float type
 # (tests/flops/flop_drop.cpp; tests/test_flop_counts.py keeps these equal).
-F_PAIR = {"cyl": 101, "ball": 109, "bull": 479, "cone": 164}
+F_PAIR = {"cyl": 102, "ball": 110, "bull": 480, "cone": 165}
 # FP64 work per (fiber, candidate triangle) pair of the push contact code,
 # the same rule, averaged over the KD-superset pairs of 40 seeded X fibers of
 # the push workload (tests/flops/flop_push.cpp); BullCutter includes its
