@@ -465,6 +465,40 @@ OCL_HD void exact_feature(const CutterP& c, DropLane& L, const TriX& t, int feat
     }
 }
 
+// Exact FP64 contact of ONE feature of a composite cutter's triangle (a
+// selection-pass winner), as the reference computes it inside its composite
+// drop (compositecutter.cpp:152-194, 58-61): the facet of sub-cutter n
+// (feature bits 4-5) with its window test, a vertex with the composite
+// profile, or edge j of sub-cutter n alone with its window test (the
+// selection found it to be that sub-cutter's highest edge contact).  The
+// lane must start below every candidate (z = -inf).
+OCL_HD void exact_feature_comp(const CutterDev& cd, DropLane& L, const TriX& t, int feat, int& iters) {
+    const int kf = feat & 15, n = (feat >> 4) & 3;
+    iters = 0;
+    L.cur_feat = feat;
+    if (kf == FT_FACET) {
+        DropLane s = sub_lane(L, cd.zoff[n], feat);
+        if (facet_simple(cd.sub[n], s, t) && comp_cc_in_window(cd, n, s)) lane_lift(L, s.z - cd.zoff[n], s.cc, CC_FACET);
+        return;
+    }
+    if (kf < FT_EDGE0) {
+        const V3 p = tri_vtx(t, kf - FT_VTX0);
+        const double q = vxynorm(vsub(v3(L.x, L.y, L.z), p));
+        if (q <= cd.c.radius) lane_lift(L, p.z - cutter_height(cd, q), p, CC_VERTEX);
+        return;
+    }
+    const int j = kf - FT_EDGE0;
+    const V3 p1 = tri_vtx(t, j), p2 = tri_vtx(t, j == 2 ? 0 : j + 1);
+    if (!near0(p1.x - p2.x) || !near0(p1.y - p2.y)) {
+        const double d = xy_line_distance(v3(L.x, L.y, L.z), p1, p2);
+        if (d <= cd.sub[n].radius) {
+            DropLane s = sub_lane(L, cd.zoff[n], feat);
+            if (single_edge_drop(cd.sub[n], s, p1, p2, d, iters) && comp_cc_in_window(cd, n, s))
+                lane_lift(L, s.z - cd.zoff[n], s.cc, CC_EDGE);
+        }
+    }
+}
+
 // millingcutter.cpp:422-433 (XY box of the cutter vs triangle bbox)
 OCL_HD bool cutter_overlaps(double radius, double x, double y, const TriX& t) {
     double mnx = t.p[0].x, mxx = t.p[0].x, mny = t.p[0].y, mxy = t.p[0].y;

@@ -121,6 +121,11 @@ struct SelParams {
     T E;            // z error bound of a well-conditioned contact
     T HPAD;         // rounding allowance of the profile lower bound (culling)
     T CUT;          // cull margin: 2 x the largest contact error bound of the mesh + HPAD
+    // composite cutters (drop_comp.hpp): per sub-cutter kind, radius,
+    // BullCutter radii and radius window, for the culling bounds
+    int ck[2], cwin[2];  // cwin: window test of a CylCutter sub-cutter's edge contacts (at distance R)
+    int cflat[2];        // the sub-cutter's flat bottom has composite height exactly 0 (z offset 0)
+    T cR[2], cr1[2], cr2[2], clo[2], chi[2];
     int cull;       // culling by profile bound enabled.  Cones cull against the
                     // running best only (their facet-first rule makes the
                     // result order dependent, so no floor of the final height
@@ -574,6 +579,93 @@ OCL_HD int feat_slot(int f) {
     return 0;
 }
 
+// One edge contact of a record (edge j, cutter kind K) at the lane (x, y):
+// height z, validity (under the cutter, CC on the segment) and ambiguity
+// (validity within its margins, or an ill-conditioned grazing contact that
+// may still lie on the segment), the height error bound eze and the slack
+// eae of an ambiguous contact, and the contact's offset ccu along the edge
+// from the lane's projection and the lane's signed distance sd from the
+// edge's line.
+template <class T>
+struct EdgeC {
+    T z, ccu, sd, eze, eae;
+    bool val, amb;
+};
+
+template <class T, int K>
+OCL_HD EdgeC<T> edge_contact(const T* rec, int j, int flags, T x, T y, const SelParams<T>& P, bool ve) {
+    typedef FM<T> F;
+    const T* e = rec + R_E0 + kEdge * j;
+    const T s2e = e[E_S2E], invl = e[E_INVL];
+    const T sd = F::fma_(e[E_EX], y, F::fma_(-e[E_EY], x, e[E_C]));
+    const T m = F::fma_(-sd, sd, P.R2);
+    const T u1 = F::fma_(-e[E_EY], y, F::fma_(-e[E_EX], x, e[E_A]));
+    const T zl = F::fma_(-u1, e[E_M], e[E_Z1]);  // edge height at the CL's projection
+    T z, ccu;
+    switch (K) {
+    case K_CYL:
+    case K_BALL: {
+        const T s = F::sqrt_(F::max_(m, (T)0));
+        z = F::fma_(s, e[E_K1], zl);
+        ccu = s * e[E_K2];
+        break;
+    }
+    case K_BULL: {
+        const T d = fabs(sd);
+        // horizontal edge: cc at the projection, z = Z1 - h(d)
+        const T wh = F::max_(d - P.r1, (T)0);
+        const T zh = e[E_Z1] - P.r2 + F::sqrt_(F::max_(P.r22 - wh * wh, (T)0));
+        // general edge: Newton on d = tau (r2 + r1 Q / sqrt(M^2 + tau^2))
+        const T M = e[E_M], Q = e[E_K1], IAM = e[E_K3], AM = fabs(M), M2 = M * M;
+        const T RQ = P.r1 * Q;
+        T tau = F::max_(F::max_(d * F::rcp_(F::fma_(RQ, IAM, P.r2)), (d - P.r1) * P.ir2), (T)0);
+#pragma unroll
+        for (int it = 0; it < (sizeof(T) == 4 ? 6 : 12); ++it) {
+            const T is = F::rsqrt_(F::fma_(tau, tau, M2));
+            const T phi = F::fma_(tau, F::fma_(RQ, is, P.r2), -d);
+            const T dphi = F::fma_(RQ * M2, is * is * is, P.r2);
+            tau = F::min_(F::fma_(-phi, F::rcp_(dphi), tau), (T)1);
+        }
+        const T c = F::sqrt_(F::max_(F::fma_(-tau, tau, (T)1), (T)0));
+        const T is = F::rsqrt_(F::fma_(tau, tau, M2));
+        // With Q^2 = 1 + M^2 the contact's offset along the edge and its
+        // height are c |M| (r1 is + r2 / Q) and c (r1 M^2 is + r2 Q):
+        // no 1/|M| term, so a nearly horizontal edge (|M| just above the
+        // reference's 1e-7 threshold) loses nothing to cancellation.
+        const T zg = F::fma_(c, F::fma_(P.r1 * M2, is, P.r2 * Q), zl - P.r2);
+        const T cg = e[E_K2] * c * AM * F::fma_(P.r1, is, P.r2 * F::rcp_(Q));
+        const bool hz = (flags & (FL_EHOR0 << j)) != 0;
+        z = hz ? zh : zg;
+        ccu = hz ? (T)0 : cg;
+        break;
+    }
+    default: {  // cone
+        const T d = fabs(sd);
+        const T s = F::sqrt_(F::max_(m, (T)0));
+        const T AM = fabs(e[E_M]);
+        const bool hyper = AM * P.R2 <= P.ch * s;
+        const T KH = e[E_K1];
+        const T zhyp = zl - P.ch + F::fma_(d, F::fma_(-e[E_K3], P.cot, AM * KH), P.R * P.cot);
+        const T zrim = F::fma_(AM, s, zl - P.ch);
+        z = hyper ? zhyp : zrim;
+        ccu = e[E_K2] * (hyper ? KH * d : s);
+        break;
+    }
+    }
+    const T t = (ccu - u1) * invl;
+    const T mt = F::min_(t, (T)1 - t);
+    const T dt = P.EPOS * invl;
+    EdgeC<T> r;
+    r.z = z;
+    r.ccu = ccu;
+    r.sd = sd;
+    r.eze = e[E_EZE];
+    r.eae = e[E_EAE];
+    r.val = ve & (m >= (T)0) & (mt >= (T)0);
+    r.amb = ve & (m >= -P.DM) & ((fabs(mt) <= dt) | ((m <= s2e) & (mt >= -dt - P.SQDM * invl)));
+    return r;
+}
+
 // One candidate triangle.  `on` = this lane overlaps the triangle in XY (the
 // reference's overlaps() test).  Branch-free: every update is a select, so a
 // wavefront runs the triangle as one straight-line block of VALU code.
@@ -651,71 +743,9 @@ OCL_HD void sel_triangle(const T* rec, uint32_t k, T x, T y, const SelParams<T>&
     bool eamb = false;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        const T* e = rec + R_E0 + kEdge * j;
-        const T s2e = e[E_S2E], eae = e[E_EAE], eze = e[E_EZE], invl = e[E_INVL];
-        const T sd = F::fma_(e[E_EX], y, F::fma_(-e[E_EY], x, e[E_C]));
-        const T m = F::fma_(-sd, sd, P.R2);
-        const T u1 = F::fma_(-e[E_EY], y, F::fma_(-e[E_EX], x, e[E_A]));
-        const T zl = F::fma_(-u1, e[E_M], e[E_Z1]);  // edge height at the CL's projection
-        T z, ccu;
-        switch (K) {
-        case K_CYL:
-        case K_BALL: {
-            const T s = F::sqrt_(F::max_(m, (T)0));
-            z = F::fma_(s, e[E_K1], zl);
-            ccu = s * e[E_K2];
-            break;
-        }
-        case K_BULL: {
-            const T d = fabs(sd);
-            // horizontal edge: cc at the projection, z = Z1 - h(d)
-            const T wh = F::max_(d - P.r1, (T)0);
-            const T zh = e[E_Z1] - P.r2 + F::sqrt_(F::max_(P.r22 - wh * wh, (T)0));
-            // general edge: Newton on d = tau (r2 + r1 Q / sqrt(M^2 + tau^2))
-            const T M = e[E_M], Q = e[E_K1], IAM = e[E_K3], AM = fabs(M), M2 = M * M;
-            const T RQ = P.r1 * Q;
-            T tau = F::max_(F::max_(d * F::rcp_(F::fma_(RQ, IAM, P.r2)), (d - P.r1) * P.ir2), (T)0);
-#pragma unroll
-            for (int it = 0; it < (sizeof(T) == 4 ? 6 : 12); ++it) {
-                const T is = F::rsqrt_(F::fma_(tau, tau, M2));
-                const T phi = F::fma_(tau, F::fma_(RQ, is, P.r2), -d);
-                const T dphi = F::fma_(RQ * M2, is * is * is, P.r2);
-                tau = F::min_(F::fma_(-phi, F::rcp_(dphi), tau), (T)1);
-            }
-            const T c = F::sqrt_(F::max_(F::fma_(-tau, tau, (T)1), (T)0));
-            const T is = F::rsqrt_(F::fma_(tau, tau, M2));
-            // With Q^2 = 1 + M^2 the contact's offset along the edge and its
-            // height are c |M| (r1 is + r2 / Q) and c (r1 M^2 is + r2 Q):
-            // no 1/|M| term, so a nearly horizontal edge (|M| just above the
-            // reference's 1e-7 threshold) loses nothing to cancellation.
-            const T zg = F::fma_(c, F::fma_(P.r1 * M2, is, P.r2 * Q), zl - P.r2);
-            const T cg = e[E_K2] * c * AM * F::fma_(P.r1, is, P.r2 * F::rcp_(Q));
-            const bool hz = (flags & (FL_EHOR0 << j)) != 0;
-            z = hz ? zh : zg;
-            ccu = hz ? (T)0 : cg;
-            break;
-        }
-        default: {  // cone
-            const T d = fabs(sd);
-            const T s = F::sqrt_(F::max_(m, (T)0));
-            const T AM = fabs(e[E_M]);
-            const bool hyper = AM * P.R2 <= P.ch * s;
-            const T KH = e[E_K1];
-            const T zhyp = zl - P.ch + F::fma_(d, F::fma_(-e[E_K3], P.cot, AM * KH), P.R * P.cot);
-            const T zrim = F::fma_(AM, s, zl - P.ch);
-            z = hyper ? zhyp : zrim;
-            ccu = e[E_K2] * (hyper ? KH * d : s);
-            break;
-        }
-        }
-        const T t = (ccu - u1) * invl;
-        const T mt = F::min_(t, (T)1 - t);
-        const T dt = P.EPOS * invl;
-        const bool val = ve & (m >= (T)0) & (mt >= (T)0);
-        // ambiguous: validity within margin, or an ill-conditioned (grazing)
-        // contact that may still lie on the segment
-        const bool amb = ve & (m >= -P.DM) &
-                         ((fabs(mt) <= dt) | ((m <= s2e) & (mt >= -dt - P.SQDM * invl)));
+        const EdgeC<T> ec = edge_contact<T, K>(rec, j, flags, x, y, P, ve);
+        const T z = ec.z, eze = ec.eze, eae = ec.eae;
+        const bool val = ec.val, amb = ec.amb;
         const T up = val ? z + eze : NEG;
         S.zt[1] = F::max_(S.zt[1], up);
         S.za = F::max_(S.za, amb ? z + eae : NEG);

@@ -15,6 +15,7 @@
 
 #include "../../include/ocl_hip.h"
 #include "cutter.hpp"
+#include "drop_comp.hpp"
 #include "drop_exact.hpp"
 #include "drop_fast.hpp"
 #include "drop_grad.hpp"
@@ -466,7 +467,11 @@ __device__ inline double comp_prof_lower64(const CutterDev& cd, double d2) {
         const double lo = (n == 0) ? 0.0 : cd.rlim[n - 1] - 1e-6, hi = cd.rlim[n] + 1e-6;
         if (d > hi) continue;
         const double r = fmax(d, lo);
-        const double hn = prof_lower64(cd.sub[n], r * r) + cd.zoff[n];
+        double hn = prof_lower64(cd.sub[n], r * r) + cd.zoff[n];
+        // a vertex beyond a CylCutter's or BullCutter's own radius but inside
+        // its window's tolerance gets the height -1 (prof_height)
+        const CutterP& s = cd.sub[n];
+        if ((s.kind == K_CYL || s.kind == K_BULL) && s.radius < hi) hn = fmin(hn, -1.0 + cd.zoff[n]);
         h = fmin(h, hn - 1e-12 * (cd.c.radius + fabs(hn)));
     }
     return h;  // +inf: no window reaches d, the triangle has no accepted contact
@@ -584,22 +589,35 @@ __global__ void k_sel_prep(const double* __restrict__ tri, uint32_t ntri, const 
     build_record<T>(t, cd, o.ox, o.oy, o.oz, er, rec + (size_t)rec_len<T>() * i, fid + 6 * (size_t)i);
 }
 
+// The two sub-cutter records of each triangle for a composite cutter (FP32).
+__global__ void k_sel_prep_comp(const double* __restrict__ tri, uint32_t ntri, const BoxOrigin* __restrict__ org,
+                                const uint32_t* __restrict__ fid, CutterDev cd, SelErr er, float* __restrict__ rec) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntri) return;
+    const TriX t = load_tri(tri, i);
+    const BoxOrigin o = org[i / kLeaf];
+    build_record_comp<float>(t, cd, o.ox, o.oy, o.oz, er, rec + (size_t)(2 * rec_len<float>()) * i,
+                             fid + 6 * (size_t)i);
+}
+
 // Per level-0 box: the largest cull cap of its records (box frame), and the
 // largest contact error bound of the mesh (atomic max of non-negative float
-// bits, which order like unsigned integers).
+// bits, which order like unsigned integers).  nrec: records per triangle
+// (2 for a composite cutter's sub-cutters).
 template <class T>
-__global__ void k_box_cap(const T* __restrict__ rec, uint32_t ntri, uint32_t nbox, T* __restrict__ cap,
-                          unsigned int* __restrict__ emax_bits) {
+__global__ void k_box_cap(const T* __restrict__ rec, uint32_t ntri, uint32_t nbox, uint32_t nrec,
+                          T* __restrict__ cap, unsigned int* __restrict__ emax_bits) {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbox) return;
     T c = -FM<T>::inf();
     float em = 0.0f;
     const uint32_t e = min(ntri, (b + 1) * kLeaf);
-    for (uint32_t i = b * kLeaf; i < e; ++i) {
-        const T* r = rec + (size_t)rec_len<T>() * i;
-        c = FM<T>::max_(c, r[R_ZCAP]);
-        em = fmaxf(em, __double2float_ru((double)r[R_EMAX]));
-    }
+    for (uint32_t i = b * kLeaf; i < e; ++i)
+        for (uint32_t s = 0; s < nrec; ++s) {
+            const T* r = rec + (size_t)rec_len<T>() * (nrec * i + s);
+            c = FM<T>::max_(c, r[R_ZCAP]);
+            em = fmaxf(em, __double2float_ru((double)r[R_EMAX]));
+        }
     cap[b] = c;
     atomicMax(emax_bits, __float_as_uint(em));
 }
@@ -1180,6 +1198,154 @@ static inline uint32_t select_grid(uint32_t n, uint32_t xcd_group) {
     return xcd_grid((n + 63) / 64, xcd_group * kSelRun);
 }
 
+// Arguments of a composite selection pass: the pass's own (P: the
+// composite's constants, rec: two records per triangle) and the
+// sub-cutters' constants.
+struct CompSelArgs {
+    SelArgs<float> A;
+    SelParams<float> PS[2];
+};
+
+// FP32 selection for a two-part composite cutter (sel_triangle_comp), one
+// wavefront of 64 neighbouring points (gid = position in the list).  The
+// level-0 boxes are taken in walk (rank) order and each triangle's two
+// records staged in LDS (my: kLeaf x 2 records).  No vertex floor and no
+// near-tie sets: an undecided point goes on to the exact fold, a decided
+// one to the refine of its winner (refine = 0: FP32 only, the result as is).
+__device__ __attribute__((always_inline)) inline void select_points_comp(const CompSelArgs& C, uint32_t gid,
+                                                                        float* my) {
+    typedef FM<float> F;
+    constexpr uint32_t kLen = 2 * rec_len<float>();
+    const SelArgs<float>& A = C.A;
+    const MeshView& m = A.m;
+    const SelParams<float>& P = A.P;
+    const uint32_t n_eff = A.count ? *A.count : A.n;
+    if ((gid & ~63u) >= n_eff) return;
+    const bool act = gid < n_eff;
+    const uint32_t src = act ? A.list[gid] : 0;
+    const double x = act ? A.cl[3 * (size_t)src] : 0.0;
+    const double y = act ? A.cl[3 * (size_t)src + 1] : 0.0;
+    const double z0 = act ? A.cl[3 * (size_t)src + 2] : 0.0;
+    const double R = A.R;
+    const float qx0 = uniform_f(__double2float_rd(wave_min(act ? x : INFINITY) - R));
+    const float qy0 = uniform_f(__double2float_rd(wave_min(act ? y : INFINITY) - R));
+    const float qx1 = uniform_f(__double2float_ru(wave_max(act ? x : -INFINITY) + R));
+    const float qy1 = uniform_f(__double2float_ru(wave_max(act ? y : -INFINITY) + R));
+    Sel<float> S;
+    sel_init(S, 0.0f);
+    float wzb = 0.0f, wza = -F::inf(), wzr = -F::inf(), wzt[4];
+    for (int i = 0; i < 4; ++i) wzt[i] = -F::inf();
+    uint32_t evals = 0;
+    double ozw = 0.0;
+    bool framed = false;
+    walk_boxes(m.tree, qx0, qy0, qx1, qy1, [&](uint32_t b) {
+        const BoxOrigin bo = A.org[b];
+        if (!framed) {
+            framed = true;
+            ozw = bo.oz;
+            wzb = (float)(z0 - ozw);
+        }
+        const float dz = (float)(bo.oz - ozw);
+        const float xl = (float)(x - bo.ox), yl = (float)(y - bo.oy);
+        // the lanes the box can matter to: its cap less the composite
+        // profile's lower bound at the distance to the box
+        const float bd2 = box_d2<float>(xl, yl, -bo.hx, bo.hx, -bo.hy, bo.hy);
+        const bool rel = act & (A.boxcap[b] + dz - comp_prof_lower<float>(bd2, P) >= wzb - P.CUT);
+        if (!__any(rel)) return;
+        const float lx0 = (float)((double)qx0 - bo.ox), lx1 = (float)((double)qx1 - bo.ox);
+        const float ly0 = (float)((double)qy0 - bo.oy), ly1 = (float)((double)qy1 - bo.oy);
+        const uint32_t t0 = b * kLeaf;
+        const uint32_t nt = min(m.ntri - t0, (uint32_t)kLeaf);
+        {
+            constexpr uint32_t kVec = kLeaf * kLen * sizeof(float) / 16;
+            constexpr uint32_t kIt = (kVec + 63) / 64;
+            const uint32_t nv = nt * (uint32_t)(kLen * sizeof(float) / 16);
+            const float4* srcv = reinterpret_cast<const float4*>(A.rec + (size_t)kLen * t0);
+            float4* dst = reinterpret_cast<float4*>(my);
+#pragma unroll
+            for (uint32_t it = 0; it < kIt; ++it) {
+                const uint32_t i = lane_id() + 64 * it;
+                const float4 v = srcv[min(i, nv - 1)];
+                if (i < nv) dst[i] = v;
+            }
+            wave_lds_sync();
+        }
+        const uint32_t bt_in = S.bt;
+        S.zb = wzb - dz;
+        S.zr = wzr - dz;
+        for (int i = 0; i < 4; ++i) S.zt[i] = -F::inf();
+        S.za = -F::inf();
+        // triangles whose XY box meets the wave's query box, one per lane
+        uint32_t tmask;
+        {
+            const uint32_t jl = lane_id();
+            bool keep = false;
+            if (jl < nt) {
+                const float* r = my + kLen * jl;
+                keep = !((r[R_BX1] < lx0) | (r[R_BX0] > lx1) | (r[R_BY1] < ly0) | (r[R_BY0] > ly1));
+            }
+            tmask = (uint32_t)__ballot(keep);
+        }
+        for (; tmask; tmask &= tmask - 1) {
+            const uint32_t j = (uint32_t)__builtin_ctz(tmask);
+            const uint32_t ku = uniform_u(t0 + j);
+            const float* r = my + kLen * j;
+            const float bx0 = r[R_BX0], bx1 = r[R_BX1], by0 = r[R_BY0], by1 = r[R_BY1];
+            bool on = rel & !((bx1 < xl - P.R) | (bx0 > xl + P.R) | (by1 < yl - P.R) | (by0 > yl + P.R));
+            const float d2 = box_d2<float>(xl, yl, bx0, bx1, by0, by1);
+            on = on & (comp_tri_cap<float>(r, xl, yl, d2, P) >= S.zb - P.CUT);
+            if (!__any(on)) continue;
+            ++evals;
+            sel_triangle_comp<float>(r, ku, xl, yl, P, C.PS, S, on);
+        }
+        if (S.bt != bt_in) wzb = S.zb + dz;
+        wzr = S.zr + dz;
+        for (int i = 0; i < 4; ++i) wzt[i] = F::max_(wzt[i], S.zt[i] + dz);
+        wza = F::max_(wza, S.za + dz);
+    });
+    Sel<float> D = S;
+    D.zb = wzb;
+    for (int i = 0; i < 4; ++i) D.zt[i] = wzt[i];
+    D.za = wza;
+    D.zr = wzr;
+    const bool up = A.refine && framed && sel_escalate(D, (float)(z0 - ozw), P);
+    const NearTieSets& nts = A.nts;
+    if (A.refine) {
+        if (act) {
+            nts.cnt[src] = up ? kSelNone : kSelDecided;
+            nts.bt[src] = D.bt;
+            nts.bf[src] = (uint8_t)D.bf;
+            nts.zsel[src] = framed ? ozw + (double)D.zb : z0;
+            nts.esel[src] = D.eb;
+            nts.zgap[src] = framed ? __double2float_ru((double)D.zr - (double)D.zb) : -INFINITY;
+            A.zfloor[src] = -INFINITY;
+        }
+    } else if (act) {
+        // FP32-only mode: the selection result as is (no CC point)
+        DropLane L = fresh_lane(x, y, z0);
+        if (D.bt != 0xffffffffu) {
+            L.z = ozw + (double)D.zb;
+            L.cc = v3(NAN, NAN, NAN);
+            const int sl = feat_slot(D.bf);
+            L.type = (sl == 0) ? CC_VERTEX : (sl == 1) ? CC_EDGE : CC_FACET;
+            L.tri = D.bt;
+            L.feat = D.bf;
+        }
+        write_drop(A.o, m.order, src, L);
+    }
+    if (A.cnt) {
+        wave_add(&A.cnt->pairs, act ? S.pairs : 0);
+        wave_add(&A.cnt->escalated, (act && up) ? 1 : 0);
+        wave_add(&A.cnt->evals, lane_id() == 0 ? evals : 0);
+    }
+}
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_drop_select_comp(CompSelArgs C) {
+    __shared__ __align__(16) float srec[kLeaf * 2 * rec_len<float>()];
+    const uint32_t gid = xcd_block(C.A.m.xcd_group * kSelRun) * 64u + threadIdx.x;
+    select_points_comp(C, gid, srec);
+}
+
 // The selection passes of a batch of jobs in one persistent launch: the
 // wavefronts pull 64-point chunks from a device work queue that runs over
 // every job's chunks (chunk_off: prefix sums of the jobs' chunk counts,
@@ -1256,7 +1422,8 @@ __device__ __attribute__((always_inline)) inline void resolve_points(
                 DropLane X = fresh_lane(x, y, -INFINITY);
                 X.cur_tri = bt;
                 int it = 0;
-                exact_feature(cd.c, X, t, bf, it);
+                if (cd.c.kind == K_COMPOSITE) exact_feature_comp(cd, X, t, bf, it);
+                else exact_feature(cd.c, X, t, bf, it);
                 brent += it;
                 // the winner's FP64 height must stay consistent with its
                 // selection value and above every other contact's upper value
@@ -1465,9 +1632,13 @@ __device__ inline void coop_gather(const MeshView& m, uint32_t* qbuf, double x, 
 // contact, the cone rim contact and the best vertex/edge contact; the fold
 // then applies them in rank order exactly as MillingCutter::dropCutter does
 // against the running z (a lifting facet skips the triangle's vertices and
-// edges; the cone rim is tried only when the tip did not lift).
+// edges; the cone rim is tried only when the tip did not lift).  Comp: a
+// composite cutter, whose facet contact is the highest valid one of its
+// sub-cutters' and whose edge contacts are its sub-cutters' window-tested
+// ones (compositecutter.cpp:152-194); the fold is the same.
 // w0: the wavefront's index in the grid, nwaves: the grid's wavefronts;
 // qbuf: the wavefront's kCoopQueue-entry LDS queue.
+template <bool Comp>
 __device__ __attribute__((always_inline)) inline void coop_exact_points(
     const MeshView& m, const CutterDev& cd, const double* __restrict__ cl, const uint32_t* __restrict__ list,
     uint32_t n_eff, const double* __restrict__ zfloor, const double* __restrict__ boxcap,
@@ -1489,15 +1660,21 @@ __device__ __attribute__((always_inline)) inline void coop_exact_points(
         // order dependent: they skip only triangles whose contacts all stay
         // below the running height (which at a triangle's turn is at least
         // the running height when it is gathered), as those cannot lift.
-        const bool cull = zfloor && (c.kind == K_CYL || c.kind == K_BALL || c.kind == K_BULL);
-        const bool crun = cull || (c.kind == K_CONE && has_contact_cap(c));
+        // Composites are order dependent too (the edges' below() test): they
+        // skip triangles whose contacts all stay below the running height.
+        const bool cull = !Comp && zfloor && (c.kind == K_CYL || c.kind == K_BALL || c.kind == K_BULL);
+        const bool crun = Comp ? comp_has_cap(cd) : (cull || (c.kind == K_CONE && has_contact_cap(c)));
         const double zf = cull ? zfloor[src] : -INFINITY;
         auto thr = [&]() {
             const double t = fmax(zf, zr);
             return t - 1e-9 * (1.0 + fabs(t));
         };
-        auto keep = [&](uint32_t, const TriX& t) { return !crun || contact_cap(c, t, x, y) >= thr(); };
-        const BoxCull bcl{crun ? boxcap : nullptr, org};
+        auto keep = [&](uint32_t, const TriX& t) {
+            if (!crun) return true;
+            if constexpr (Comp) return t.maxz - comp_prof_lower64(cd, tri_box_d2(t, x, y)) >= thr();
+            return contact_cap(c, t, x, y) >= thr();
+        };
+        const BoxCull bcl{(crun && !Comp) ? boxcap : nullptr, org};
         coop_gather(m, qbuf, x, y, c, bcl, thr, keep, [&](uint32_t* q, uint32_t nb) {
             const bool has = lane < nb;
             const uint32_t k = has ? q[lane] : 0u;
@@ -1508,13 +1685,19 @@ __device__ __attribute__((always_inline)) inline void coop_exact_points(
                 maxz = t.maxz;
                 A.cur_tri = B.cur_tri = V.cur_tri = k;
                 int it = 0;
-                exact_feature(c, A, t, FT_FACET, it);
-                if (c.kind == K_CONE) exact_feature(c, B, t, FT_FACET_RIM, it);
+                if constexpr (Comp) {
+                    A.cur_feat = 0;
+                    facet_composite(cd, A, t);
+                } else {
+                    exact_feature(c, A, t, FT_FACET, it);
+                    if (c.kind == K_CONE) exact_feature(c, B, t, FT_FACET_RIM, it);
+                }
                 V.cur_feat = 0;
                 vertex_drop(cd, V, t);
                 if (V.z < t.maxz) {
                     int it2 = 0;
-                    edge_simple(c, V, t, it2);
+                    if constexpr (Comp) edge_composite(cd, V, t, it2);
+                    else edge_simple(c, V, t, it2);
                     brent += it2;
                 }
             }
@@ -1605,8 +1788,23 @@ __global__ void __launch_bounds__(256) k_drop_coop_exact(MeshView m, CutterDev c
     __shared__ uint32_t qmem[kCoopWaves][kCoopQueue];
     const uint32_t wv = threadIdx.x >> 6;
     unsigned long long brent = 0;
-    coop_exact_points(m, cd, cl, list, *count, zfloor, boxcap, org, o, qmem[wv], brent,
-                      xcd_block(m.xcd_group) * kCoopWaves + wv, gridDim.x * kCoopWaves);
+    coop_exact_points<false>(m, cd, cl, list, *count, zfloor, boxcap, org, o, qmem[wv], brent,
+                             xcd_block(m.xcd_group) * kCoopWaves + wv, gridDim.x * kCoopWaves);
+    if (cnt) wave_add(&cnt->brent, brent);
+}
+
+// The exact tier for a composite cutter (no floors, no box caps).
+__global__ void __launch_bounds__(256) k_drop_coop_exact_comp(MeshView m, CutterDev cd,
+                                                              const double* __restrict__ cl,
+                                                              const uint32_t* __restrict__ list,
+                                                              const uint32_t* __restrict__ count,
+                                                              const BoxOrigin* __restrict__ org, DropOut o,
+                                                              Counters* cnt) {
+    __shared__ uint32_t qmem[kCoopWaves][kCoopQueue];
+    const uint32_t wv = threadIdx.x >> 6;
+    unsigned long long brent = 0;
+    coop_exact_points<true>(m, cd, cl, list, *count, nullptr, nullptr, org, o, qmem[wv], brent,
+                            xcd_block(m.xcd_group) * kCoopWaves + wv, gridDim.x * kCoopWaves);
     if (cnt) wave_add(&cnt->brent, brent);
 }
 
@@ -1670,7 +1868,8 @@ __global__ void __launch_bounds__(256) k_drop_coop_exact_jobs(const PostJob* __r
         const uint32_t nq = J.qcount[0];
         const uint32_t w0 = (blockIdx.x * kCoopWaves + wv + nwaves - first % nwaves) % nwaves;
         unsigned long long brent = 0;
-        coop_exact_points(J.m, J.cd, J.cl, J.q1, nq, J.zfloor, J.cap64, J.org, J.o, qmem[wv], brent, w0, nwaves);
+        coop_exact_points<false>(J.m, J.cd, J.cl, J.q1, nq, J.zfloor, J.cap64, J.org, J.o, qmem[wv], brent, w0,
+                                 nwaves);
         if (J.cnt3) wave_add(&J.cnt3->brent, brent);
         first += nq;
     }
@@ -1904,14 +2103,23 @@ static int ensure_records(ocl_mesh* m, const CutterDev& cd, std::shared_ptr<RecS
     RecSet& rs = *slot;
     rs.ok = false;
     const uint32_t n = (uint32_t)m->ntri, nbox = m->cnt[0];
+    const bool comp = cd.c.kind == K_COMPOSITE;
+    if (comp && sizeof(T) != 4) return fail(OCL_E_UNSUPPORTED, "composite selection records are FP32");
+    const uint32_t nrec = comp ? 2u : 1u;
     int rc;
-    if ((rc = rs.rec.ensure(sizeof(T) * rec_len<T>() * m->ntri)) || (rc = rs.cap.ensure(sizeof(T) * nbox + 16)) ||
-        (rc = rs.emax.ensure(16)))
+    if ((rc = rs.rec.ensure(sizeof(T) * rec_len<T>() * nrec * m->ntri)) ||
+        (rc = rs.cap.ensure(sizeof(T) * nbox + 16)) || (rc = rs.emax.ensure(16)))
         return rc;
-    k_sel_prep<T><<<(n + 255) / 256, 256, 0, st>>>(m->tri.as<double>(), n, m->origins.as<BoxOrigin>(),
-                                                   m->fid.as<uint32_t>(), cd, er, rs.rec.as<T>());
+    if constexpr (sizeof(T) == 4) {
+        if (comp)
+            k_sel_prep_comp<<<(n + 255) / 256, 256, 0, st>>>(m->tri.as<double>(), n, m->origins.as<BoxOrigin>(),
+                                                             m->fid.as<uint32_t>(), cd, er, rs.rec.as<float>());
+    }
+    if (!comp)
+        k_sel_prep<T><<<(n + 255) / 256, 256, 0, st>>>(m->tri.as<double>(), n, m->origins.as<BoxOrigin>(),
+                                                       m->fid.as<uint32_t>(), cd, er, rs.rec.as<T>());
     HIP_TRY(hipMemsetAsync(rs.emax.p, 0, 4, st));
-    k_box_cap<T><<<(nbox + 255) / 256, 256, 0, st>>>(rs.rec.as<T>(), n, nbox, rs.cap.as<T>(),
+    k_box_cap<T><<<(nbox + 255) / 256, 256, 0, st>>>(rs.rec.as<T>(), n, nbox, nrec, rs.cap.as<T>(),
                                                      rs.emax.as<unsigned int>());
     HIP_TRY(hipGetLastError());
     float em = 0.0f;
@@ -2001,6 +2209,8 @@ struct FastPlan {
     uint32_t *q1, *q2, *qnt, *qcount;  // qcount: q1, q2, qnt entries
     SelParams<double> P64;
     SelParams<float> P32;
+    SelParams<float> PS32[2];  // composite: the sub-cutters' FP32 constants
+    bool comp;                 // composite FP32 selection (no FP64 records)
     double etie;
 };
 
@@ -2057,16 +2267,28 @@ static void fast_bind(FastPlan& F, ocl_mesh* m, const WorkPtrs& w, const MeshVie
     F.nts.q = F.qnt;
     F.nts.qn = F.qcount + 2;
     const double se = env_scale("OCL_SEL_E_SCALE", 1.0);
-    const SelErr er64 = make_err<double>(cd, se);
-    F.P64 = make_params<double>(cd, er64);
-    set_cut(F.P64, *F.r64);
-    F.etie = (double)F.P64.E;
+    F.comp = cd.c.kind == K_COMPOSITE;
+    if (!F.comp) {
+        const SelErr er64 = make_err<double>(cd, se);
+        F.P64 = make_params<double>(cd, er64);
+        set_cut(F.P64, *F.r64);
+        F.etie = (double)F.P64.E;
+    }
     if (F.r32) {
         const SelErr er32 = make_err<float>(cd, se);
-        F.P32 = make_params<float>(cd, er32);
+        F.P32 = F.comp ? comp_sel_params<float>(cd, er32, F.PS32) : make_params<float>(cd, er32);
         set_cut(F.P32, *F.r32);
         F.etie = (double)F.P32.E;
     }
+}
+
+// The precision modes whose pipeline starts with a selection pass: FP32 or
+// FP64 for Cyl/Ball/Bull/Cone, FP32 for a composite cutter of two Cyl/Ball/
+// Bull parts (comp_fast); every other composite takes the exact fold.
+static bool selects(const CutterDev& cd, int precision, size_t ntri) {
+    if (precision == OCL_PREC_EXACT64 || ntri == 0) return false;
+    if (cd.c.kind != K_COMPOSITE) return true;
+    return comp_fast(cd) && (precision == OCL_PREC_FP32 || precision == OCL_PREC_MIXED);
 }
 
 static bool uses_fp32_records(int precision) {
@@ -2086,8 +2308,11 @@ static int fast_prepare(ocl_mesh* m, DropWork& w, const MeshView& mv, const Cutt
         (rc = w.selrec.ensure(20ull * n + 64)) || (rc = w.ensure_counters()))
         return rc;
     const double se = env_scale("OCL_SEL_E_SCALE", 1.0);
-    if ((rc = ensure_records<double>(m, cd, m->rs64, make_err<double>(cd, se), st))) return rc;
-    F.r64 = m->rs64;
+    F.r64.reset();
+    if (cd.c.kind != K_COMPOSITE) {
+        if ((rc = ensure_records<double>(m, cd, m->rs64, make_err<double>(cd, se), st))) return rc;
+        F.r64 = m->rs64;
+    }
     F.r32.reset();
     if (uses_fp32_records(precision)) {
         if ((rc = ensure_records<float>(m, cd, m->rs32, make_err<float>(cd, se), st))) return rc;
@@ -2121,6 +2346,14 @@ static int fast_post(const FastPlan& F, hipStream_t st) {
     k_drop_resolve<<<(n + 255) / 256, 256, 0, st>>>(mv, cd, F.cl, F.perm, n, F.nts, F.etie, F.o, F.q1, F.qcount,
                                                      F.cnt2);
     HIP_TRY(hipGetLastError());
+    if (F.comp) {
+        // composite: no near-tie sets, the exact fold of what is undecided
+        k_drop_coop_exact_comp<<<kCoopBlocks, 256, 0, st>>>(mv, cd, F.cl, F.q1, F.qcount,
+                                                            m->origins.as<BoxOrigin>(), F.o, F.cnt3);
+        HIP_TRY(hipGetLastError());
+        *F.exact_slot = 0;
+        return OCL_OK;
+    }
     k_drop_neartie<<<kNearTieBlocks, 256, 0, st>>>(
         mv, cd, F.cl, F.qnt, F.qcount + 2, F.nts, F.o, F.q1, F.qcount, F.cnt2);
     HIP_TRY(hipGetLastError());
@@ -2169,7 +2402,15 @@ static int launch_drop_fast(const FastPlan& F, hipStream_t st, bool timed) {
     case K_BALL: if (f32) k_drop_select<float, K_BALL><<<nb, 64, 0, st>>>(A32); else k_drop_select<double, K_BALL><<<nb, 64, 0, st>>>(A64); break;
     case K_BULL: if (f32) k_drop_select<float, K_BULL><<<nb, 64, 0, st>>>(A32); else k_drop_select<double, K_BULL><<<nb, 64, 0, st>>>(A64); break;
     case K_CONE: if (f32) k_drop_select<float, K_CONE><<<nb, 64, 0, st>>>(A32); else k_drop_select<double, K_CONE><<<nb, 64, 0, st>>>(A64); break;
-    default: return fail(OCL_E_UNSUPPORTED, "selection pass has no composite specialisation");
+    default: {
+        if (!F.comp || !f32) return fail(OCL_E_UNSUPPORTED, "composite selection is FP32 only");
+        CompSelArgs C;
+        C.A = A32;
+        C.PS[0] = F.PS32[0];
+        C.PS[1] = F.PS32[1];
+        k_drop_select_comp<<<nb, 64, 0, st>>>(C);
+        break;
+    }
     }
     HIP_TRY(hipGetLastError());
     if (timed) HIP_TRY(hipEventRecord(m->ev_s1, st));
@@ -2468,7 +2709,7 @@ int ocl_hip_drop_device(ocl_mesh* m, const ocl_cutter_desc* desc, const double* 
     }
     const MeshView mv = m->view();
     bool fast = false;
-    if (precision == OCL_PREC_EXACT64 || m->ntri == 0 || cd.c.kind == K_COMPOSITE) {
+    if (!selects(cd, precision, m->ntri)) {
         if (m->ntri > 0)
             k_drop_exact<<<xcd_grid((n32 + 63) / 64, mv.xcd_group * 4u), 64, 0, st>>>(mv, cd, d_cl, perm, nullptr,
                                                                                     n32, o, cnt);
@@ -2616,9 +2857,11 @@ extern "C" int ocl_hip_drop_device_noalloc(ocl_mesh* m, const ocl_cutter_desc* d
     CutterDev cd;
     int rc = cutter_from_desc(desc, cd);
     if (rc) return rc;
-    const bool exact = precision == OCL_PREC_EXACT64 || m->ntri == 0 || cd.c.kind == K_COMPOSITE;
+    const bool exact = !selects(cd, precision, m->ntri);
     // the selection records are mesh state built by an allocating call
-    if (!exact && (!records_hold(m->rs64, cd) || (uses_fp32_records(precision) && !records_hold(m->rs32, cd))))
+    const bool need64 = cd.c.kind != K_COMPOSITE;
+    if (!exact && ((need64 && !records_hold(m->rs64, cd)) ||
+                   (uses_fp32_records(precision) && !records_hold(m->rs32, cd))))
         return fail(OCL_E_ARG, "the mesh holds no selection records for this cutter and precision: call "
                                "ocl_hip_drop_device once with them first (that call builds them)");
     if (n == 0) return OCL_OK;
@@ -2658,7 +2901,7 @@ extern "C" int ocl_hip_drop_device_noalloc(ocl_mesh* m, const ocl_cutter_desc* d
         return OCL_OK;
     }
     FastPlan F;
-    F.r64 = m->rs64;
+    if (need64) F.r64 = m->rs64;
     if (uses_fp32_records(precision)) F.r32 = m->rs32;
     fast_bind(F, m, a.w, mv, cd, d_cl, perm, n32, precision, o, nullptr);
     return launch_drop_fast(F, st, false);
@@ -3374,16 +3617,19 @@ struct ocl_batch {
     uint32_t nzero = 0, nsel = 0, grid = 0, nexact = 0, exchunks = 0, npost = 0, postchunks = 0;
     uint32_t* sorted_vals = nullptr;
     std::vector<uint32_t> sel_order;  // job indices of the queue, largest first
+    std::vector<uint32_t> comp_sel;   // composite selection jobs (launched one by one)
     hipStream_t cap = nullptr;
+    hipStream_t side = nullptr;  // the composite jobs' passes, beside the others
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr, s0 = nullptr, s1 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, s0 = nullptr, s1 = nullptr, fork = nullptr, join = nullptr;
     ~ocl_batch() {
         for (const ocl_amd::BatchJob& J : jobs) --J.m->batches;
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (cap) (void)hipStreamDestroy(cap);
-        for (hipEvent_t e : {e0, e1, s0, s1})
+        if (side) (void)hipStreamDestroy(side);
+        for (hipEvent_t e : {e0, e1, s0, s1, fork, join})
             if (e) (void)hipEventDestroy(e);
     }
 };
@@ -3410,6 +3656,28 @@ static int batch_record(ocl_batch* b, hipStream_t st, bool timed) {
     if (b->nexact)
         k_drop_exact_jobs<<<b->exchunks, 64, 0, st>>>(b->exjobs.as<ExactJob>(), b->exoff.as<uint32_t>(),
                                                                  b->nexact);
+    // Composite jobs: their selection and later passes on the side stream,
+    // where they overlap the other jobs' (a composite job with few points
+    // on a large mesh is a long chain of few wavefronts).
+    const bool forked = !b->comp_sel.empty();
+    if (forked) {
+        HIP_TRY(hipEventRecord(b->fork, st));
+        HIP_TRY(hipStreamWaitEvent(b->side, b->fork, 0));
+        for (uint32_t j : b->comp_sel) {
+            const BatchJob& J = b->jobs[j];
+            CompSelArgs C;
+            C.A = fast_args32(J.F);
+            C.PS[0] = J.F.PS32[0];
+            C.PS[1] = J.F.PS32[1];
+            k_drop_select_comp<<<select_grid(J.n, J.mv.xcd_group), 64, 0, b->side>>>(C);
+            HIP_TRY(hipGetLastError());
+        }
+        for (uint32_t j : b->comp_sel) {
+            int rc = fast_post(b->jobs[j].F, b->side);
+            if (rc) return rc;
+        }
+        HIP_TRY(hipEventRecord(b->join, b->side));
+    }
     if (timed) HIP_TRY(hipEventRecord(b->s0, st));
     if (b->nsel) {
         if (b->tune.per_job_launch) {
@@ -3455,11 +3723,12 @@ static int batch_record(ocl_batch* b, hipStream_t st, bool timed) {
         HIP_TRY(hipGetLastError());
     } else {
         for (const BatchJob& J : b->jobs) {
-            if (!J.fast) continue;
+            if (!J.fast || J.n == 0 || J.F.comp) continue;
             int rc = fast_post(J.F, st);
             if (rc) return rc;
         }
     }
+    if (forked) HIP_TRY(hipStreamWaitEvent(st, b->join, 0));
     if (timed) HIP_TRY(hipEventRecord(b->e1, st));
     return OCL_OK;
 }
@@ -3511,7 +3780,7 @@ int ocl_hip_batch_create(const ocl_drop_job* jobs, size_t njobs, int precision, 
         J.base = (uint32_t)total;
         J.o = DropOut{q.d_z_out, q.d_cc_xyz_out, q.d_cc_type_out, q.d_tri_out, q.d_feat_out};
         J.mv = q.mesh->view();
-        J.fast = !(precision == OCL_PREC_EXACT64 || q.mesh->ntri == 0 || J.cd.c.kind == K_COMPOSITE);
+        J.fast = selects(J.cd, precision, q.mesh->ntri);
         ocl_mesh* m = q.mesh;
         DropWork& w = *J.w;
         if ((rc = w.ensure_counters())) return rc;
@@ -3555,7 +3824,8 @@ int ocl_hip_batch_create(const ocl_drop_job* jobs, size_t njobs, int precision, 
         if ((rc = fast_prepare(J.m, *J.w, J.mv, J.cd, J.cl, b->sorted_vals + J.base, J.n, precision, J.o,
                                J.w->counters.as<Counters>(), st, J.F)))
             return rc;
-        sel.push_back((uint32_t)(&J - b->jobs.data()));
+        // composite selections have a kernel of their own, outside the queue
+        (J.F.comp ? b->comp_sel : sel).push_back((uint32_t)(&J - b->jobs.data()));
     }
     std::stable_sort(sel.begin(), sel.end(), [&](uint32_t a, uint32_t c) { return b->jobs[a].n > b->jobs[c].n; });
     b->sel_order = sel;
@@ -3625,6 +3895,8 @@ int ocl_hip_batch_create(const ocl_drop_job* jobs, size_t njobs, int precision, 
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     b->grid = std::max(1u, std::min((uint32_t)cus * bpc, coff.back()));
     for (hipEvent_t* e : {&b->e0, &b->e1, &b->s0, &b->s1}) HIP_TRY(hipEventCreate(e));
+    for (hipEvent_t* e : {&b->fork, &b->join}) HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    if (!b->comp_sel.empty()) HIP_TRY(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
     HIP_TRY(hipDeviceSynchronize());
     if (b->tune.use_graph && b->total > 0) {
         HIP_TRY(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));

@@ -12,6 +12,7 @@
 #include "../../include/ocl_hip.h"
 #include "cutter.hpp"
 #include "drop_exact.hpp"
+#include "drop_comp.hpp"
 #include "drop_fast.hpp"
 #include "feature_ids.hpp"
 #include "kd_rank.hpp"
@@ -34,6 +35,23 @@ bool cutter_of(const ocl_cutter_desc* d, CutterDev& cd) {
         default: return false;
         }
     };
+    if (d->c.type == OCL_CUTTER_COMPOSITE) {  // as the library's cutter_from_desc
+        if (d->nsub < 1 || d->nsub > kMaxSub) return false;
+        cd.c.kind = K_COMPOSITE;
+        cd.nsub = d->nsub;
+        double r = 0;
+        for (int i = 0; i < d->nsub; ++i) {
+            if (!simple(d->sub[i], cd.sub[i])) return false;
+            cd.rlim[i] = d->radius_limit[i];
+            cd.hlim[i] = d->height_limit[i];
+            cd.zoff[i] = d->zoffset[i];
+            r = std::max(r, d->radius_limit[i]);
+        }
+        cd.c.radius = r;
+        cd.c.diameter = 2 * r;
+        cd.c.length = d->c.length;
+        return true;
+    }
     return simple(d->c, cd.c);
 }
 
@@ -44,10 +62,13 @@ SelParams<T> params(const CutterDev& cd, const SelErr& er) {
 
 template <class T, int K>
 void run_points(const std::vector<TriX>& tx, const std::vector<BoxOrigin>& org, const std::vector<double>& bbx,
-                const CutterDev& cd, const SelParams<T>& P, const double* cl, size_t n, const std::vector<T>& rec,
+                const CutterDev& cd, const SelParams<T>& P, const SelParams<T>* PS, const double* cl, size_t n,
+                const std::vector<T>& rec,
                 const std::vector<T>& boxcap, const uint32_t* order, double* z_out, int* type_out, int* esc_out,
                 int* feat_out, int* tri_out) {
     typedef FM<T> F;
+    constexpr bool kComp = K == K_COMPOSITE;
+    const size_t stride = (kComp ? 2 : 1) * (size_t)rec_len<T>();
     const double R = cd.c.radius;
     const size_t ntri = tx.size();
 #pragma omp parallel for schedule(dynamic, 64)
@@ -62,7 +83,7 @@ void run_points(const std::vector<TriX>& tx, const std::vector<BoxOrigin>& org, 
         const size_t nbox = (ntri + kLeafH - 1) / kLeafH;
         // floor of the final height from the vertices of the boxes under the point
         T zc = -F::inf();
-        if (P.cull && K != K_CONE)
+        if (P.cull && K != K_CONE && !kComp)
             for (size_t b = 0; b < nbox; ++b) {
                 const double* bb = &bbx[4 * b];
                 if (bb[1] < x || bb[0] > x || bb[3] < y || bb[2] > y) continue;
@@ -76,7 +97,7 @@ void run_points(const std::vector<TriX>& tx, const std::vector<BoxOrigin>& org, 
                 const T xl = (T)(x - bo.ox), yl = (T)(y - bo.oy);
                 const size_t e = std::min(ntri, (b + 1) * kLeafH);
                 for (size_t k = b * kLeafH; k < e; ++k)
-                    zc = F::max_(zc, vertex_floor<T, K>(&rec[(size_t)rec_len<T>() * k], xl, yl, P, true) + dz);
+                    zc = F::max_(zc, vertex_floor<T, K>(&rec[stride * k], xl, yl, P, true) + dz);
             }
         for (size_t b = 0; b < nbox; ++b) {
             const double* bb = &bbx[4 * b];
@@ -91,7 +112,8 @@ void run_points(const std::vector<TriX>& tx, const std::vector<BoxOrigin>& org, 
             const T xl = (T)(x - bo.ox), yl = (T)(y - bo.oy);
             if (P.cull) {
                 const T d2 = box_d2<T>(xl, yl, -(T)bo.hx, (T)bo.hx, -(T)bo.hy, (T)bo.hy);
-                if (boxcap[b] + dz - prof_lower<T, K>(d2, P) < F::max_(wzb, zc) - P.CUT) continue;
+                const T pl = kComp ? comp_prof_lower<T>(d2, P) : prof_lower<T, K>(d2, P);
+                if (boxcap[b] + dz - pl < F::max_(wzb, zc) - P.CUT) continue;
             }
             const uint32_t bt_in = S.bt;
             S.zb = wzb - dz;
@@ -100,14 +122,16 @@ void run_points(const std::vector<TriX>& tx, const std::vector<BoxOrigin>& org, 
             S.za = -F::inf();
             const size_t e = std::min(ntri, (b + 1) * kLeafH);
             for (size_t k = b * kLeafH; k < e; ++k) {
-                const T* r = &rec[(size_t)rec_len<T>() * k];
+                const T* r = &rec[stride * k];
                 if (r[R_BX1] < xl - P.R || r[R_BX0] > xl + P.R || r[R_BY1] < yl - P.R || r[R_BY0] > yl + P.R)
                     continue;
                 if (P.cull) {
                     const T d2 = box_d2<T>(xl, yl, r[R_BX0], r[R_BX1], r[R_BY0], r[R_BY1]);
-                    if (tri_cap<T, K>(r, xl, yl, d2, P) < F::max_(S.zb, zc - dz) - P.CUT) continue;
+                    const T cap = kComp ? comp_tri_cap<T>(r, xl, yl, d2, P) : tri_cap<T, K>(r, xl, yl, d2, P);
+                    if (cap < F::max_(S.zb, zc - dz) - P.CUT) continue;
                 }
-                sel_triangle<T, K>(r, (uint32_t)k, xl, yl, P, S, true);
+                if constexpr (kComp) sel_triangle_comp<T>(r, (uint32_t)k, xl, yl, P, PS, S, true);
+                else sel_triangle<T, K>(r, (uint32_t)k, xl, yl, P, S, true);
             }
             if (S.bt != bt_in) wzb = S.zb + dz;
             wzr = S.zr + dz;
@@ -140,7 +164,8 @@ void run_points(const std::vector<TriX>& tx, const std::vector<BoxOrigin>& org, 
             X.x = x; X.y = y; X.z = -INFINITY; X.cc = v3(0, 0, 0); X.type = CC_NONE;
             X.tri = 0xffffffffu; X.feat = FT_NONE; X.cur_tri = D.bt; X.cur_feat = 0;
             int it = 0;
-            exact_feature(cd.c, X, t, D.bf, it);
+            if (kComp) exact_feature_comp(cd, X, t, D.bf, it);
+            else exact_feature(cd.c, X, t, D.bf, it);
             zr = ozw + (double)D.zb;
             ty = X.type;
             ft = D.bf;
@@ -213,29 +238,41 @@ extern "C" int ocl_selcheck_drop(const double* tris, size_t ntri, unsigned bucke
         org[b].hx = std::nextafterf((float)(0.5 * ((double)fx1 - (double)fx0)), INFINITY);
         org[b].hy = std::nextafterf((float)(0.5 * ((double)fy1 - (double)fy0)), INFINITY);
     }
+    const bool comp = cd.c.kind == K_COMPOSITE;
+    if (comp && (!comp_fast(cd) || !use_float)) return OCL_E_UNSUPPORTED;
     auto go = [&](auto tag) {
         typedef decltype(tag) T;
         SelErr er;
         er.u = FM<T>::eps() * escale;
         er.R = cd.c.radius;
         er.zmag = 4.0 * cd.c.radius + 10.0;
-        SelParams<T> P = params<T>(cd, er);
-        std::vector<T> rec((size_t)rec_len<T>() * ntri);
+        SelParams<T> PS[2];
+        SelParams<T> P = comp ? comp_sel_params<T>(cd, er, PS) : params<T>(cd, er);
+        const size_t stride = (comp ? 2 : 1) * (size_t)rec_len<T>();
+        std::vector<T> rec(stride * ntri);
         std::vector<T> boxcap(nbox, -FM<T>::inf());
         double emax = 0;
         for (size_t k = 0; k < ntri; ++k) {
             const BoxOrigin& o = org[k / kLeafH];
-            T* r = &rec[(size_t)rec_len<T>() * k];
-            build_record<T>(tx[k], cd, o.ox, o.oy, o.oz, er, r, &fid[6 * k]);
-            boxcap[k / kLeafH] = std::max(boxcap[k / kLeafH], r[R_ZCAP]);
-            emax = std::max(emax, (double)r[R_EMAX]);
+            T* r = &rec[stride * k];
+            if (comp) {
+                build_record_comp<T>(tx[k], cd, o.ox, o.oy, o.oz, er, r, &fid[6 * k]);
+                const T* r1 = r + rec_len<T>();
+                boxcap[k / kLeafH] = std::max(boxcap[k / kLeafH], std::max(r[R_ZCAP], r1[R_ZCAP]));
+                emax = std::max(emax, std::max((double)r[R_EMAX], (double)r1[R_EMAX]));
+            } else {
+                build_record<T>(tx[k], cd, o.ox, o.oy, o.oz, er, r, &fid[6 * k]);
+                boxcap[k / kLeafH] = std::max(boxcap[k / kLeafH], r[R_ZCAP]);
+                emax = std::max(emax, (double)r[R_EMAX]);
+            }
         }
         P.CUT = (T)(2.0 * emax + (double)P.HPAD);
         switch (cd.c.kind) {
-        case K_CYL: run_points<T, K_CYL>(tx, org, bbx, cd, P, cl, n, rec, boxcap, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
-        case K_BALL: run_points<T, K_BALL>(tx, org, bbx, cd, P, cl, n, rec, boxcap, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
-        case K_BULL: run_points<T, K_BULL>(tx, org, bbx, cd, P, cl, n, rec, boxcap, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
-        default: run_points<T, K_CONE>(tx, org, bbx, cd, P, cl, n, rec, boxcap, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
+        case K_CYL: run_points<T, K_CYL>(tx, org, bbx, cd, P, PS, cl, n, rec, boxcap, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
+        case K_BALL: run_points<T, K_BALL>(tx, org, bbx, cd, P, PS, cl, n, rec, boxcap, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
+        case K_BULL: run_points<T, K_BULL>(tx, org, bbx, cd, P, PS, cl, n, rec, boxcap, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
+        case K_COMPOSITE: run_points<T, K_COMPOSITE>(tx, org, bbx, cd, P, PS, cl, n, rec, boxcap, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
+        default: run_points<T, K_CONE>(tx, org, bbx, cd, P, PS, cl, n, rec, boxcap, order.data(), z_out, type_out, esc_out, feat_out, tri_out); break;
         }
     };
     if (use_float) go(0.0f);
@@ -374,6 +411,8 @@ extern "C" int ocl_selcheck_tricap(const double* tris, size_t ntri, const ocl_cu
                                    size_t n, int use_float, double* cap_out, double* hi_out) {
     CutterDev cd;
     if (!cutter_of(desc, cd)) return OCL_E_UNSUPPORTED;
+    const bool comp = cd.c.kind == K_COMPOSITE;
+    if (comp && (!comp_fast(cd) || !use_float)) return OCL_E_UNSUPPORTED;
     auto go = [&](auto tag) {
         typedef decltype(tag) T;
         typedef FM<T> F;
@@ -381,8 +420,9 @@ extern "C" int ocl_selcheck_tricap(const double* tris, size_t ntri, const ocl_cu
         er.u = F::eps();
         er.R = cd.c.radius;
         er.zmag = 4.0 * cd.c.radius + 10.0;
-        const SelParams<T> P = params<T>(cd, er);
-        T r[kFastRec32];
+        SelParams<T> PS[2];
+        const SelParams<T> P = comp ? comp_sel_params<T>(cd, er, PS) : params<T>(cd, er);
+        T r[2 * kFastRec32];
         for (size_t k = 0; k < ntri; ++k) {
             const double* q = tris + 9 * k;
             TriX t;
@@ -393,10 +433,32 @@ extern "C" int ocl_selcheck_tricap(const double* tris, size_t ntri, const ocl_cu
             t.maxz = std::fmax(std::fmax(q[2], q[5]), q[8]);
             const double ox = (q[0] + q[3] + q[6]) / 3.0, oy = (q[1] + q[4] + q[7]) / 3.0;
             const double oz = 0.5 * (std::fmin(std::fmin(q[2], q[5]), q[8]) + t.maxz);
-            build_record<T>(t, cd, ox, oy, oz, er, r);
+            if (comp) build_record_comp<T>(t, cd, ox, oy, oz, er, r, nullptr);
+            else build_record<T>(t, cd, ox, oy, oz, er, r);
             for (size_t i = 0; i < n; ++i) {
                 const T xl = (T)(pts[3 * i] - ox), yl = (T)(pts[3 * i + 1] - oy);
                 const T d2 = box_d2<T>(xl, yl, r[R_BX0], r[R_BX1], r[R_BY0], r[R_BY1]);
+                if (comp) {
+                    // the lower of the triangle cap and the box-level bound
+                    // (the largest record cap less the profile's lower bound)
+                    const T* r1 = r + rec_len<T>();
+                    const T cb = F::max_(r[R_ZCAP], r1[R_ZCAP]) - comp_prof_lower<T>(d2, P);
+                    const T cap = F::min_(comp_tri_cap<T>(r, xl, yl, d2, P), cb);
+                    T hmax = -F::inf();
+                    const T zf0 = F::fma_(r[R_GY], yl, F::fma_(r[R_GX], xl, r[R_F0]));
+                    const T zf1 = F::fma_(r1[R_GY], yl, F::fma_(r1[R_GX], xl, r1[R_F0]));
+                    const T starts[4] = {-F::inf(), zf0, zf1, F::max_(zf0, zf1)};
+                    for (T z0 : starts) {
+                        Sel<T> S;
+                        sel_init(S, z0);
+                        T hi = -F::inf();
+                        sel_triangle_comp<T>(r, 0, xl, yl, P, PS, S, true, &hi);
+                        hmax = F::max_(hmax, hi);
+                    }
+                    cap_out[i * ntri + k] = (double)cap + oz;
+                    hi_out[i * ntri + k] = (double)hmax + oz;
+                    continue;
+                }
                 T cap = 0;
                 switch (cd.c.kind) {
                 case K_CYL: cap = tri_cap<T, K_CYL>(r, xl, yl, d2, P); break;
