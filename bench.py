@@ -52,6 +52,10 @@ F_PAIR = {"cyl": 102, "ball": 110, "bull": 480, "cone": 165}
 # the push workload (tests/flops/flop_push.cpp); BullCutter includes its
 # edges' Brent solves (15.2 iterations per pair on that sample).
 F_PUSH = {"bull": 2430, "cone": 1244}
+# The same count over the pairs the mixed-precision push keeps (the sample's
+# pairs less those the containment bound leaves out, replayed in rank order
+# in batches of 64 as the kernel takes them).
+F_PUSH_KEPT = {"bull": 2430, "cone": 1244}
 
 SPECS = {
     "ball": {"kind": "ball", "d": 10.0, "L": 20.0},
@@ -144,6 +148,17 @@ def traffic_from_profile(cutter):
     for line in open(path):
         if line.startswith("TRAFFIC_BYTES_PER_LAUNCH"):
             return float(line.split()[-1])
+    return None
+
+
+def push_traffic(cutter):
+    """HBM bytes per launch of the push kernel (mixed precision, X and Y
+    fibers of the push workload), from the committed rocprofv3 summary."""
+    path = os.path.join(ROOT, "profiles", "push_fibers_c2_counters.md")
+    if os.path.exists(path):
+        for line in open(path):
+            if line.startswith(f"TRAFFIC_BYTES_PER_LAUNCH k_push_fibers<{cutter}>"):
+                return float(line.split()[-1])
     return None
 
 
@@ -265,22 +280,38 @@ def bench_push(capi, W, cpu_fibers, do_cpu):
             fibs["x"].append(xf)
             fibs["y"].append(yf)
         fibs = {d: np.concatenate(v) for d, v in fibs.items()}
-        dev.push_xy(spec, fibs["x"], fibs["y"])  # warm-up: trees and pool sizes
+        # every candidate pair, as the reference pushes them (EXACT64)
+        dev.push_xy(spec, fibs["x"], fibs["y"], precision="exact64")  # warm-up: trees and pool sizes
+        ra = dev.push_xy(spec, fibs["x"], fibs["y"], precision="exact64")
+        sa = ra["stats"]
+        # the product default (MIXED): pairs that cannot change their fiber
+        # left out, same intervals
+        dev.push_xy(spec, fibs["x"], fibs["y"], precision="mixed")
         t0 = time.perf_counter()
-        r = dev.push_xy(spec, fibs["x"], fibs["y"])
+        r = dev.push_xy(spec, fibs["x"], fibs["y"], precision="mixed")
         wall = time.perf_counter() - t0
         st = r["stats"]
         kms = st["kernel_ms"]
+        same = all(np.array_equal(r[d]["offsets"], ra[d]["offsets"]) and np.array_equal(r[d]["lohi"], ra[d]["lohi"])
+                   and np.array_equal(r[d]["cc"], ra[d]["cc"]) for d in ("x", "y"))
         nint = len(r["x"]["lohi"]) + len(r["y"]["lohi"])
         nf = len(fibs["x"]) + len(fibs["y"])
-        ach = st["pairs"] * F_PUSH[name] / (kms * 1e-3) / 1e12
-        line = {"fibers": nf, "intervals": nint, "pairs": int(st["pairs"]), "brent_iters": int(st["brent_iters"]),
+        ach = st["pairs"] * F_PUSH_KEPT[name] / (kms * 1e-3) / 1e12
+        ref_tf = sa["pairs"] * F_PUSH[name] / (kms * 1e-3) / 1e12
+        line = {"fibers": nf, "intervals": nint, "pairs": int(st["pairs"]), "pairs_skipped": int(st["skipped"]),
+                "brent_iters": int(st["brent_iters"]),
                 "kernel_ms": kms, "fibers_per_s": nf / (kms / 1e3), "intervals_per_s": nint / (kms / 1e3),
                 "wall_s_incl_host": wall,
+                "every_pair": {"kernel_ms": sa["kernel_ms"], "pairs": int(sa["pairs"]),
+                               "brent_iters": int(sa["brent_iters"]),
+                               "fibers_per_s": nf / (sa["kernel_ms"] / 1e3), "same_intervals": same},
+                "speedup_vs_every_pair": sa["kernel_ms"] / kms,
                 "roofline": {"bound": "valu", "kernel": f"k_push_fibers<{name}>", "achieved": ach,
                              "peak": PEAK_FP64, "unit": "TFLOP/s (FP64)", "frac": ach / PEAK_FP64,
-                             "flop_per_pair": F_PUSH[name], "pairs_per_launch": int(st["pairs"]),
-                             "kernel_ms": kms, "traffic": None}}
+                             "flop_per_pair": F_PUSH_KEPT[name], "pairs_per_launch": int(st["pairs"]),
+                             "kernel_ms": kms, "traffic": push_traffic(name),
+                             "reference_equivalent": {"pairs": int(sa["pairs"]), "flop_per_pair": F_PUSH[name],
+                                                      "tflops": ref_tf, "frac": ref_tf / PEAK_FP64}}}
         if do_cpu:
             from oracle import oracle as O
             if om is None:
@@ -294,7 +325,8 @@ def bench_push(capi, W, cpu_fibers, do_cpu):
                                     "sample": f"{cpu_fibers} seeded X fibers of the same set, FP64 oracle with "
                                               "the reference's KD-search candidates"}
         out[name] = line
-        log(f"push {name}: {nf} fibers, {nint} intervals, kernel {kms:.1f} ms")
+        log(f"push {name}: {nf} fibers, {nint} intervals, kernel {kms:.1f} ms (every pair {sa['kernel_ms']:.1f} ms), "
+            f"{st['pairs']} pairs + {st['skipped']} skipped, same intervals {same}")
     dev.close()
     return out
 
@@ -442,6 +474,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the backward, push and batched-jobs sections")
     ap.add_argument("--cpu-push-fibers", type=int, default=200)
+    ap.add_argument("--only", choices=("push", "batch", "backward"),
+                    help="run one extra section alone and print its JSON object (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run 3 device copies of 256 MiB (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -464,6 +498,19 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration: 3 x 268435456 bytes read + written")
         del src, dst
+    if args.only == "push":
+        print(json.dumps({"push": bench_push(capi, W, args.cpu_push_fibers, not args.no_cpu)}), flush=True)
+        return
+    if args.only == "batch":
+        print(json.dumps({"batch": bench_batch(torch, capi, W, max(3, args.steps // 2))}), flush=True)
+        return
+    if args.only == "backward":
+        tris = W.config1_mesh()
+        mesh = capi.DeviceMesh(tris)
+        print(json.dumps({"backward": bench_backward(torch, capi, W, mesh, len(tris), 1_000_000,
+                                                     max(3, args.steps // 2))}), flush=True)
+        mesh.close()
+        return
     spec = SPECS[args.cutter]
     t0 = time.perf_counter()
     tris = W.config1_mesh()

@@ -89,6 +89,8 @@ typedef struct {
     uint64_t wave_evals;    /* wavefront evaluations of the per-pair contact
                                code in the first selection kernel (pairs /
                                (64 wave_evals) = lane utilisation) */
+    uint64_t skipped;       /* push: candidate pairs left out because they
+                               cannot change their fiber (not in pairs) */
 } ocl_stats;
 
 /* Mesh: ntri triangles, 9 doubles each (p0, p1, p2).  bucket = the KD-tree
@@ -269,7 +271,13 @@ int ocl_hip_batch_autotune(const ocl_drop_job* jobs, size_t njobs, int precision
  * Pairs: every triangle the reference's KD search returns for the fiber
  * (kdtree.hpp:273-304, a superset of the box overlaps, all of them pushed as
  * in batchpushcutter.cpp:170-181); with OCL_PUSH_EXACT_FILTER only the
- * triangles whose boxes overlap the search box. */
+ * triangles whose boxes overlap the search box.  OCL_PREC_EXACT64 pushes
+ * every such pair; the other modes leave out a pair whose interval bound
+ * (the triangle's extent along the fiber widened by the cutter's profile
+ * width) lies strictly inside an interval the fiber already holds, or whose
+ * triangle lies below the fiber: Fiber::addInterval would ignore it
+ * (fiber.cpp:58-66), so the results are the same (stats->skipped counts
+ * them; the reference notes the idea at batchpushcutter.cpp:177). */
 int ocl_hip_push_batch(ocl_mesh* mesh, const ocl_cutter_desc* cutter, int dir, const double* fib_p1p2, size_t nf,
                        int precision, uint64_t* offsets_out, uint64_t* total_out, ocl_stats* stats, void* stream);
 

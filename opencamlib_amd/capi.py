@@ -41,7 +41,7 @@ class Stats(C.Structure):
     _fields_ = [("points", C.c_uint64), ("pairs", C.c_uint64), ("escalated", C.c_uint64),
                 ("escalated_exact", C.c_uint64),
                 ("brent_iters", C.c_uint64), ("intervals", C.c_uint64), ("kernel_ms", C.c_double),
-                ("select_ms", C.c_double), ("wave_evals", C.c_uint64)]
+                ("select_ms", C.c_double), ("wave_evals", C.c_uint64), ("skipped", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -22,6 +22,7 @@
 #include "feature_ids.hpp"
 #include "kd_rank.hpp"
 #include "mesh.hpp"
+#include "push_exact.hpp"
 #include "radix.hpp"
 #include "stl_read.hpp"
 #include "walk.hpp"
@@ -189,6 +190,19 @@ __global__ void k_tri_prep(const double* __restrict__ src, double* __restrict__ 
     r[13] = 0.0;
     r[14] = 0.0;
     r[15] = 0.0;
+}
+
+// Push trees: the pair-bound record (push_bound_rec) in the record's
+// padding, for fibers along `axis`.
+__global__ void k_push_boundrec(double* __restrict__ rec, uint32_t n, int axis) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double* r = rec + (size_t)kRec * i;
+    double b[3];
+    push_bound_rec(r, axis, b);
+    r[13] = b[0];
+    r[14] = b[1];
+    r[15] = b[2];
 }
 
 // Level-0 boxes: XY box of kLeaf consecutive triangles, rounded outward.
@@ -2948,6 +2962,7 @@ static int build_tree(ocl_mesh* m, ocl_tree& T, const int dims[4], int cu) {
     HIP_TRY(hipMemcpy(T.order.p, order.data(), sizeof(uint32_t) * ntri, hipMemcpyHostToDevice));
     const uint32_t n32 = (uint32_t)ntri;
     k_tri_prep<<<(n32 + 255) / 256, 256>>>(staging.as<double>(), T.tri.as<double>(), n32);
+    k_push_boundrec<<<(n32 + 255) / 256, 256>>>(T.tri.as<double>(), n32, cu ^ 1);
     float4* boxes = T.boxes.as<float4>();
     k_box_leaf_uz<<<(T.cnt[0] + 255) / 256, 256>>>(T.tri.as<double>(), n32, boxes, T.cnt[0], cu);
     for (int l = 1; l <= T.top; ++l)
@@ -3053,8 +3068,8 @@ static PushSide push_side(ocl_mesh* m, int dir, int superset) {
 // those fibers only, with list capacity 256, then 4x as long as some list
 // outgrows it.  Adds the device time of every launch to *ms and the counters
 // to *hc.
-static int push_run(ocl_mesh* m, const CutterDev& cd, int superset, const int* dirs, int nd, hipStream_t st,
-                    float* ms, PushCounters* hc) {
+static int push_run(ocl_mesh* m, const CutterDev& cd, int superset, int skip, const int* dirs, int nd,
+                    hipStream_t st, float* ms, PushCounters* hc) {
     int rc;
     if ((rc = m->push_cnt.ensure(sizeof(PushCounters))) || (rc = m->push_queue.ensure(64))) return rc;
     HIP_TRY(hipMemsetAsync(m->push_cnt.p, 0, sizeof(PushCounters), st));
@@ -3089,11 +3104,11 @@ static int push_run(ocl_mesh* m, const CutterDev& cd, int superset, const int* d
         const uint32_t ns = (uint32_t)nt;
         const PushSide* sd = m->push_sides.as<const PushSide>();
         switch (cd.c.kind) {
-        case K_CYL: k_push_fibers<K_CYL><<<nb, 256, 0, st>>>(sd, ns, q, superset, cd, pc); break;
-        case K_BALL: k_push_fibers<K_BALL><<<nb, 256, 0, st>>>(sd, ns, q, superset, cd, pc); break;
-        case K_BULL: k_push_fibers<K_BULL><<<nb, 256, 0, st>>>(sd, ns, q, superset, cd, pc); break;
-        case K_CONE: k_push_fibers<K_CONE><<<nb, 256, 0, st>>>(sd, ns, q, superset, cd, pc); break;
-        default: k_push_fibers<K_COMPOSITE><<<nb, 256, 0, st>>>(sd, ns, q, superset, cd, pc); break;
+        case K_CYL: k_push_fibers<K_CYL><<<nb, 256, 0, st>>>(sd, ns, q, superset, skip, cd, pc); break;
+        case K_BALL: k_push_fibers<K_BALL><<<nb, 256, 0, st>>>(sd, ns, q, superset, skip, cd, pc); break;
+        case K_BULL: k_push_fibers<K_BULL><<<nb, 256, 0, st>>>(sd, ns, q, superset, skip, cd, pc); break;
+        case K_CONE: k_push_fibers<K_CONE><<<nb, 256, 0, st>>>(sd, ns, q, superset, skip, cd, pc); break;
+        default: k_push_fibers<K_COMPOSITE><<<nb, 256, 0, st>>>(sd, ns, q, superset, skip, cd, pc); break;
         }
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(m->ev1, st));
@@ -3151,10 +3166,13 @@ static int push_side_finish(ocl_mesh* m, int dir, uint64_t* offsets_out) {
     return OCL_OK;
 }
 
-static int push_precision(int& precision, int& superset) {
+// EXACT64 pushes every candidate, as the reference does; the other modes
+// leave out the candidates that cannot change their fiber (same results).
+static int push_precision(int& precision, int& superset, int& skip) {
     superset = (precision & OCL_PUSH_EXACT_FILTER) ? 0 : 1;
     precision &= ~OCL_PUSH_EXACT_FILTER;
     if (precision < OCL_PREC_EXACT64 || precision > OCL_PREC_MIXED) return fail(OCL_E_ARG, "unknown precision mode");
+    skip = precision != OCL_PREC_EXACT64;
     return OCL_OK;
 }
 
@@ -3164,6 +3182,7 @@ static void push_stats(ocl_stats* stats, uint64_t fibers, uint64_t rows, const P
     stats->pairs = hc.pairs;
     stats->intervals = rows;
     stats->brent_iters = hc.brent;
+    stats->skipped = hc.skipped;
     stats->kernel_ms = ms;
     stats->select_ms = ms;
 }
@@ -3196,8 +3215,8 @@ int ocl_hip_push_batch(ocl_mesh* m, const ocl_cutter_desc* desc, int dir, const 
     if (!m) return fail(OCL_E_ARG, "mesh is NULL");
     if (dir != 0 && dir != 1) return fail(OCL_E_ARG, "dir must be 0 (X fibers) or 1 (Y fibers)");
     if (nf > 0 && !offsets_out) return fail(OCL_E_ARG, "offset pointer is NULL");
-    int superset, rc;
-    if ((rc = push_precision(precision, superset))) return rc;
+    int superset, skip, rc;
+    if ((rc = push_precision(precision, superset, skip))) return rc;
     CutterDev cd;
     if ((rc = cutter_from_desc(desc, cd))) return rc;
     hipStream_t st = (hipStream_t)stream_;
@@ -3211,7 +3230,7 @@ int ocl_hip_push_batch(ocl_mesh* m, const ocl_cutter_desc* desc, int dir, const 
     }
     PushCounters hc{};
     float ms = 0;
-    if (m->ntri > 0 && (rc = push_run(m, cd, superset, &dir, 1, st, &ms, &hc))) return rc;
+    if (m->ntri > 0 && (rc = push_run(m, cd, superset, skip, &dir, 1, st, &ms, &hc))) return rc;
     if ((rc = push_side_finish(m, dir, offsets_out))) return rc;
     if (total_out) *total_out = m->pw[dir].total;
     push_stats(stats, nf, m->pw[dir].total, hc, ms);
@@ -3224,8 +3243,8 @@ int ocl_hip_push_xy(ocl_mesh* m, const ocl_cutter_desc* desc, const double* xfib
     using namespace ocl_amd;
     if (!m) return fail(OCL_E_ARG, "mesh is NULL");
     if ((nx > 0 && !x_offsets_out) || (ny > 0 && !y_offsets_out)) return fail(OCL_E_ARG, "offset pointers are NULL");
-    int superset, rc;
-    if ((rc = push_precision(precision, superset))) return rc;
+    int superset, skip, rc;
+    if ((rc = push_precision(precision, superset, skip))) return rc;
     CutterDev cd;
     if ((rc = cutter_from_desc(desc, cd))) return rc;
     hipStream_t st = (hipStream_t)stream_;
@@ -3237,7 +3256,7 @@ int ocl_hip_push_xy(ocl_mesh* m, const ocl_cutter_desc* desc, const double* xfib
     if (ny > 0) dirs[nd++] = 1;
     PushCounters hc{};
     float ms = 0;
-    if (nd > 0 && m->ntri > 0 && (rc = push_run(m, cd, superset, dirs, nd, st, &ms, &hc))) return rc;
+    if (nd > 0 && m->ntri > 0 && (rc = push_run(m, cd, superset, skip, dirs, nd, st, &ms, &hc))) return rc;
     if (x_offsets_out && (rc = push_side_finish(m, 0, x_offsets_out))) return rc;
     if (y_offsets_out && (rc = push_side_finish(m, 1, y_offsets_out))) return rc;
     if (totals_out) {

@@ -752,6 +752,79 @@ OCL_HD bool comp_push_part(const CutterDev& cd, const FibX& f, PIval& i, const T
     return nc > 0;
 }
 
+// ------------------------------------------------------------ pair bounds
+// A bound on the interval one (fiber, triangle) pair can add, used to leave
+// out pairs that cannot change the fiber (no counterpart in the reference,
+// whose pushCutter3 notes the idea as a to-do, batchpushcutter.cpp:177).
+// Fiber::addInterval ignores an interval strictly inside an interval the
+// fiber already holds (Fiber::contains with the strict Interval::inside,
+// fiber.cpp:58-66, interval.cpp:106-111), and the fiber's intervals only
+// grow and merge, so a pair whose bound lies strictly inside one of them
+// when it is queued changes nothing whenever it would have been pushed.
+//
+// Every contact of the push code has its CC point on the triangle and its
+// CL point within the cutter's profile width at the CC height (at most the
+// radius) of the CC point in XY: vertices and the CylCutter's z-slice
+// points are on the triangle; edge contacts pass isInside(p1, p2) on the
+// edge's line; BullCutter edge contacts are offset-ellipse points, r1 from
+// a point r2 from the edge line, for any root the solve returns; facet
+// contacts are tested with the XY barycentric test (tri_contains).  That
+// test is only trustworthy for a triangle whose XY projection is well
+// conditioned: on a vertical or sliver triangle the reference accepts CC
+// points far outside it, so such triangles get no bound.  Every contact
+// has its CC point at or above the fiber (vertex and edge height tests,
+// facet heights fz + ch - nl nz), so a well-conditioned triangle entirely
+// below the fiber adds nothing.
+//
+// Record (three values per triangle, in the push tree's record padding):
+// conditioned flag, lowest and highest vertex coordinate along the fiber
+// axis (0 = x for X fibers, 1 = y for Y fibers).
+OCL_HD void push_bound_rec(const double* p9, int axis, double out[3]) {
+    const double ax = p9[0], ay = p9[1], bx = p9[3], by = p9[4], cx = p9[6], cy = p9[7];
+    // the denominators of the barycentric test, as tri_contains computes them
+    const double du = ay * cx - ax * cy + (cy - ay) * bx + (ax - cx) * by;
+    const double dv = ax * by - ay * bx + (ay - by) * cx + (bx - ax) * cy;
+    double m = 1.0;
+    for (int k = 0; k < 3; ++k) m = fmax(m, fmax(fabs(p9[3 * k]), fabs(p9[3 * k + 1])));
+    const double m2 = m * m;
+    const double e0 = hypot(bx - ax, by - ay), e1 = hypot(cx - bx, cy - by), e2 = hypot(ax - cx, ay - cy);
+    // Rounding moves the test's decisions by about 50 eps m^2 / (shortest
+    // edge) in XY and its denominators by 8 eps m^2: with these floors an
+    // accepted CC point lies within 6e-5 mm of the triangle, well inside the
+    // bound's 1e-3 mm margin.
+    const bool ok = fabs(du) >= 1e-9 * m2 && fabs(dv) >= 1e-9 * m2 && fmin(e0, fmin(e1, e2)) >= 1e-10 * m2;
+    out[0] = ok ? 1.0 : 0.0;
+    out[1] = fmin(fmin(p9[axis], p9[3 + axis]), p9[6 + axis]);
+    out[2] = fmax(fmax(p9[axis], p9[3 + axis]), p9[6 + axis]);
+}
+
+// Bound of one pair from the triangle's record (rec3 = push_bound_rec, zmax =
+// highest vertex) for the fiber from a to b along its axis at height fz.
+// Returns 0: no bound (composite cutters, ill-conditioned triangles);
+// 1: the pair adds no interval; 2: any interval it adds lies in [tl, tu]
+// (fiber parameter).  Ball and cone bounds use the profile width at the
+// highest reachable CC height (their contacts are exact geometric
+// tangencies); the other cutters use the radius.
+OCL_HD int push_bound(const CutterP& c, const double rec3[3], double zmax, double fz, double a, double b, double& tl,
+                      double& tu) {
+    if (c.kind == K_COMPOSITE || rec3[0] == 0.0 || !(b != a)) return 0;
+    const double hb = zmax - fz + 1e-4 + 1e-9 * (fabs(zmax) + fabs(fz));
+    if (hb < 0.0) return 1;
+    double w = c.radius;
+    if (c.kind == K_BALL || c.kind == K_CONE) w = fmin(prof_width(c, hb), c.radius);
+    w += 1e-3 + 1e-9 * (fabs(rec3[1]) + fabs(rec3[2]) + c.radius);
+    const double inv = 1.0 / (b - a);
+    double t0 = (rec3[1] - w - a) * inv, t1 = (rec3[2] + w - a) * inv;
+    if (t0 > t1) {
+        const double q = t0;
+        t0 = t1;
+        t1 = q;
+    }
+    tl = t0 - 1e-12 * (1.0 + fabs(t0));
+    tu = t1 + 1e-12 * (1.0 + fabs(t1));
+    return 2;
+}
+
 // MillingCutter::pushCutter (millingcutter.cpp:377-382): vertices, facet,
 // edges, all applied to the same interval.  K = the cutter kind, fixed at
 // compile time so each kernel carries only its cutter's code.

@@ -113,7 +113,7 @@ OCL_HD bool plist_insert(PEntry* E, uint32_t& n, uint32_t cap, const PEntry& iv)
 }
 
 struct PushCounters {
-    unsigned long long pairs, intervals, brent;
+    unsigned long long pairs, intervals, brent, skipped;
 };
 
 // One wavefront per fiber (persistent grid): the wave walks the tree with
@@ -140,6 +140,14 @@ struct PushCounters {
 // then costs the longest solve of each round of 64 instead of the longest
 // lane's six solves in a row.
 //
+// skip = 1: a candidate whose interval bound (push_bound) lies strictly
+// inside an interval of the fiber's list when it is queued, or whose
+// triangle lies below the fiber, is left out: it cannot change the list
+// (push_bound's comment), so the results are those of pushing every
+// candidate.  The test reads the list lane 0 keeps in LDS, which only grows
+// and merges while the fiber's candidates are taken in rank order; the retry
+// pass does not skip.
+//
 // Merging: a batch's non-empty intervals are first merged among themselves
 // in parallel (sort by lower end, components by a prefix max), so lane 0
 // inserts a few components rather than every interval; one by one, the
@@ -154,7 +162,7 @@ constexpr uint32_t kLdsList = 64;
 template <int K>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_push_fibers(const PushSide* __restrict__ sides, uint32_t nsides, uint32_t* __restrict__ queue, int superset,
-              CutterDev cd, PushCounters* cnt) {
+              int skip, CutterDev cd, PushCounters* cnt) {
     constexpr int kS = (K == K_BULL) ? kBullSolves : 1;
     __shared__ uint32_t qmem[kPushWaves][128];
     __shared__ PEntry emem[kPushWaves][64];
@@ -167,7 +175,7 @@ k_push_fibers(const PushSide* __restrict__ sides, uint32_t nsides, uint32_t* __r
     uint32_t* qbuf = qmem[wv];
     PEntry* ebuf = emem[wv];
     const double R = cd.c.radius, L = cd.c.length;
-    unsigned long long pairs = 0, brent = 0, nint = 0;
+    unsigned long long pairs = 0, brent = 0, nint = 0, skipped = 0;
     // the sides' descriptors stay in global memory (read with uniform
     // addresses): two copies as kernel arguments, picked per fiber, cost
     // registers and spills
@@ -379,6 +387,29 @@ k_push_fibers(const PushSide* __restrict__ sides, uint32_t nsides, uint32_t* __r
                         cand = !(hi_a < a0 || lo_a > a1 || t.maxz < v0 || lo_z > v1);
                     }
                 }
+                if (skip && in_lds) {
+                    // the list's size is lane 0's; its entries are in LDS,
+                    // written before the last batch's closing wave_lds_sync
+                    // (the retry pass, whose list is in global memory, pushes
+                    // every candidate)
+                    const uint32_t nl = uniform_u((uint32_t)__shfl((int)n, 0, 64));
+                    if (cand) {
+                        const double* r = pm.tri + (size_t)kRec * k;
+                        const double rb[3] = {r[13], r[14], r[15]};
+                        double tl = 0.0, tu = 0.0;
+                        const int bd = push_bound(cd.c, rb, r[12], f.p1.z, dir == 0 ? f.p1.x : f.p1.y,
+                                                  dir == 0 ? f.p2.x : f.p2.y, tl, tu);
+                        bool drop = bd == 1;
+                        if (bd == 2)
+                            for (uint32_t j = 0; j < nl && EL[j].lo < tl; ++j)
+                                if (tu < EL[j].hi) {
+                                    drop = true;
+                                    break;
+                                }
+                        skipped += drop ? 1 : 0;
+                        cand = !drop;
+                    }
+                }
                 const unsigned long long bal = __ballot(cand);
                 const uint32_t pos = qn + __popcll(bal & ((1ull << lane) - 1ull));
                 if (cand) qbuf[pos] = k;
@@ -429,6 +460,7 @@ k_push_fibers(const PushSide* __restrict__ sides, uint32_t nsides, uint32_t* __r
         wave_add(&cnt->pairs, pairs);
         wave_add(&cnt->intervals, nint);
         wave_add(&cnt->brent, brent);
+        wave_add(&cnt->skipped, skipped);
     }
 }
 

@@ -550,3 +550,20 @@ extern "C" int ocl_selcheck_feature_ids(const double* tris, size_t ntri, uint32_
     std::memcpy(out, ids.data(), sizeof(uint32_t) * ids.size());
     return OCL_OK;
 }
+
+// The push kernel's pair bound (push_bound_rec + push_bound) for one
+// (fiber, triangle) pair: returns 0 (no bound), 1 (no interval) or 2 (any
+// interval in [out[0], out[1]]); dir 0 = X fiber, 1 = Y fiber.
+extern "C" int ocl_selcheck_push_bound(const double* tri9, const ocl_cutter_desc* desc, const double* fib6, int dir,
+                                       double* out) {
+    CutterDev cd;
+    if (!cutter_of(desc, cd)) return -1;
+    double rec[3];
+    push_bound_rec(tri9, dir == 0 ? 0 : 1, rec);
+    const double zmax = std::max(std::max(tri9[2], tri9[5]), tri9[8]);
+    double tl = 0.0, tu = 0.0;
+    const int r = push_bound(cd.c, rec, zmax, fib6[2], fib6[dir == 0 ? 0 : 1], fib6[dir == 0 ? 3 : 4], tl, tu);
+    out[0] = tl;
+    out[1] = tu;
+    return r;
+}

@@ -1,0 +1,157 @@
+"""The push kernel's pair bound, checked on the CPU against the oracle.
+
+The mixed/FP64 push leaves out a (fiber, triangle) pair when a bound on the
+interval it could add lies strictly inside an interval the fiber already
+holds, or when its triangle lies below the fiber (push_bound in
+csrc/push_exact.hpp, host build in libocl_selcheck.so).  That is exact only
+if the bound holds every interval the reference's pushCutter
+(millingcutter.cpp:377-382) adds.  Checked here over the KD-search candidates
+of seeded fibers of the sphere scene, of two of the reference's STL meshes,
+and over random and degenerate triangles near the fiber (vertical facets and
+slivers, where the reference's facet test accepts CC points off the
+triangle and the bound must refuse): a "no interval" answer must meet an
+empty oracle interval, a bound must contain the oracle's interval.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+from cutters import ALL, scaled
+from opencamlib_amd import capi, workloads as W
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def selcheck():
+    L = C.CDLL(os.path.join(ROOT, "opencamlib_amd", "lib", "libocl_selcheck.so"))
+    L.ocl_selcheck_push_bound.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+    L.ocl_selcheck_push_bound.restype = C.c_int
+    return L
+
+
+def check_pairs(L, spec, pairs):
+    """pairs: iterable of (fib6, dir, tri9).  Returns counts per answer."""
+    desc = capi.cutter_desc(spec)
+    cut = O.cutter_from_spec(spec)
+    out = np.zeros(2)
+    counts = [0, 0, 0]
+    for fib, d, t in pairs:
+        fib = np.ascontiguousarray(fib, dtype=np.float64)
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        r = L.ocl_selcheck_push_bound(t.ctypes.data, C.addressof(desc), fib.ctypes.data, d, out.ctypes.data)
+        assert r in (0, 1, 2)
+        counts[r] += 1
+        if r == 0:
+            continue
+        ne, iv = O.push_triangle(cut, fib, t)
+        if r == 1:
+            assert not ne, (spec, fib.tolist(), t.tolist(), iv[:2])
+        elif ne:
+            assert out[0] <= iv[0] and iv[1] <= out[1], (spec, fib.tolist(), t.tolist(), iv[:2], out)
+    return counts
+
+
+def kd_pairs(tris, spec, fibers, d):
+    om = O.Mesh(tris)
+    cut = O.cutter_from_spec(spec)
+    for fib in fibers:
+        for k in om.kd_search_push(cut, fib, d):
+            yield fib, d, tris[k]
+
+
+@pytest.mark.parametrize("name", ["bull", "cone", "ball", "cyl"])
+def test_bound_holds_on_sphere_scene_candidates(name):
+    spec = {"bull": {"kind": "bull", "d": 10.0, "r": 2.0, "L": 20.0},
+            "cone": {"kind": "cone", "d": 10.0, "angle": math.pi / 4, "L": 10.0},
+            "ball": {"kind": "ball", "d": 10.0, "L": 20.0},
+            "cyl": {"kind": "cyl", "d": 6.0, "L": 20.0}}[name]
+    tris = W.config2_mesh().reshape(-1, 9)
+    rng = np.random.default_rng(21)
+    levels = W.config2_levels()
+    L = selcheck()
+    total = [0, 0, 0]
+    for trial in range(6):
+        z = levels[int(rng.integers(len(levels)))]
+        xf, yf = W.waterline_fibers(tris.reshape(-1, 3, 3), spec["d"] / 2, z, 2000)
+        d = trial % 2
+        fib = (xf if d == 0 else yf)[rng.choice(2000, 3, replace=False)]
+        c = check_pairs(L, spec, kd_pairs(tris, spec, fib, d))
+        total = [a + b for a, b in zip(total, c)]
+    # the vertical facets of the icospheres get no bound; most pairs do
+    assert total[1] + total[2] > 0.9 * sum(total), total
+    assert total[0] > 0, total
+
+
+@pytest.mark.parametrize("mesh,scale", [("cone_on_side", 3.0), ("wheel_in_box", 10.0)])
+def test_bound_holds_on_stl_candidates(mesh, scale):
+    tris = capi.stl_read(os.path.join(GOLDEN, mesh + ".stl")).reshape(-1, 9)
+    L = selcheck()
+    for kind in ("bull", "cone"):
+        spec = scaled(ALL[kind], scale)
+        z0, z1 = tris[:, 2::3].min(), tris[:, 2::3].max()
+        for f in (0.3, 0.7):
+            xf, yf = W.waterline_fibers(tris.reshape(-1, 3, 3), spec["d"] / 2, z0 + f * (z1 - z0), 40)
+            check_pairs(L, spec, kd_pairs(tris, spec, xf[::8], 0))
+            check_pairs(L, spec, kd_pairs(tris, spec, yf[::8], 1))
+
+
+def random_pairs(rng, n, d):
+    """Triangles of 0.05-8 mm around random fibers, a third of them vertical
+    (two corners on one vertical line) or slivers (one corner almost on the
+    opposite edge)."""
+    for _ in range(n):
+        z = rng.uniform(-2, 4)
+        a = rng.uniform(-3, 3)
+        fib = np.array([-20.0, a, z, 20.0, a, z]) if d == 0 else np.array([a, -20.0, z, a, 20.0, z])
+        c = rng.uniform(-6, 6, 3) + np.array([0.0, 0.0, z])
+        size = 10 ** rng.uniform(-1.3, 0.9)
+        p = c + size * rng.normal(size=(3, 3))
+        kind = rng.integers(3)
+        if kind == 1:
+            p[2, :2] = p[1, :2]
+        elif kind == 2:
+            p[2] = p[0] + rng.uniform(0.1, 0.9) * (p[1] - p[0]) + 1e-7 * rng.normal(size=3)
+        yield fib, d, p.reshape(9)
+
+
+@pytest.mark.parametrize("name", ["cyl", "ball", "bull", "cone"])
+def test_bound_holds_on_random_and_degenerate_triangles(name):
+    spec = ALL[name]
+    L = selcheck()
+    rng = np.random.default_rng(7 + len(name))
+    counts = [0, 0, 0]
+    for d in (0, 1):
+        c = check_pairs(L, spec, random_pairs(rng, 3000, d))
+        counts = [a + b for a, b in zip(counts, c)]
+    assert counts[0] > 0 and counts[2] > 0, counts
+
+
+def test_vertical_facet_gets_no_bound():
+    """A vertical icosphere facet of the sphere scene (first X fiber of the
+    lowest waterline level): the reference's facet test accepts a CC point
+    on the line through its XY projection, and the interval starts 31 mm
+    before the triangle's extent less the radius.  The bound must refuse
+    this triangle."""
+    spec = {"kind": "bull", "d": 10.0, "r": 2.0, "L": 20.0}
+    tris = W.config2_mesh().reshape(-1, 9)
+    xf, _ = W.waterline_fibers(tris.reshape(-1, 3, 3), 5.0, W.config2_levels()[0], 2000)
+    fib, t = np.ascontiguousarray(xf[0]), np.ascontiguousarray(tris[144436])
+    ne, iv = O.push_triangle(O.cutter_from_spec(spec), fib, t)
+    x = fib[0] + iv[0] * (fib[3] - fib[0])
+    assert ne and x < t[0::3].min() - 5.0 - 10.0
+    L = selcheck()
+    out = np.zeros(2)
+    desc = capi.cutter_desc(spec)
+    assert L.ocl_selcheck_push_bound(t.ctypes.data, C.addressof(desc), fib.ctypes.data, 0, out.ctypes.data) == 0
+
+
+def test_composite_cutters_get_no_bound():
+    L = selcheck()
+    spec = ALL["cylball"]
+    counts = check_pairs(L, spec, random_pairs(np.random.default_rng(3), 200, 0))
+    assert counts == [200, 0, 0]

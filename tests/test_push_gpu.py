@@ -37,12 +37,14 @@ def split(res, nf):
     return [(res["lohi"][offs[i]:offs[i + 1]], res["cc"][offs[i]:offs[i + 1]]) for i in range(nf)]
 
 
-def compare_push(tris, spec, fib, direction, label, exact_filter=False, dev=None):
+def compare_push(tris, spec, fib, direction, label, exact_filter=False, dev=None, precision="exact64"):
+    """precision "exact64" pushes every candidate pair; the other modes leave
+    out the pairs that cannot change their fiber (same results)."""
     search_mode = 2 if exact_filter else 0
     own = dev is None
     if own:
         dev = capi.DeviceMesh(tris)
-    got = dev.push(spec, fib, direction, exact_filter=exact_filter)
+    got = dev.push(spec, fib, direction, precision=precision, exact_filter=exact_filter)
     if own:
         dev.close()
     ref = O.Mesh(tris).batch_push(O.cutter_from_spec(spec), fib, 0 if direction == "x" else 1,
@@ -66,10 +68,16 @@ def compare_push(tris, spec, fib, direction, label, exact_filter=False, dev=None
                 bad_end += 1
             if not np.array_equal(gc[:, [3, 7]], rc[:, [3, 7]]) or np.max(np.abs(gc - rc)) > max(tol, 1e-12) * 1e3:
                 bad_cc += 1
-    print(f"[push] {label} {spec['kind']} dir={direction}: fibers={nf} intervals={len(got['lohi'])} "
+    print(f"[push] {label} {spec['kind']} {precision} dir={direction}: fibers={nf} intervals={len(got['lohi'])} "
           f"count_mismatch={bad_count} endpoint_mismatch={bad_end} cc_mismatch={bad_cc} max|dt|={worst:.3e} "
           f"stats={got['stats']}", flush=True)
+    if precision == "exact64":
+        assert got["stats"]["skipped"] == 0
+    compare_push.last_stats = got["stats"]
     return bad_count, bad_end, bad_cc
+
+
+PRECISIONS = ["exact64", "mixed"]
 
 
 @pytest.fixture(scope="module")
@@ -77,37 +85,44 @@ def small_scene():
     return W.spheres_scene(4, 2, 60.0, seed=5)
 
 
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("direction", ["x", "y"])
 @pytest.mark.parametrize("name", ["cyl", "ball", "bull", "cone", "cylball"])
-def test_push_small_scene(gpu, small_scene, name, direction):
+def test_push_small_scene(gpu, small_scene, name, direction, precision):
     spec = ALL[name]
     R = max(spec.get("d", 10.0), 10.0) / 2
     fib = fibers_for(small_scene, R, [1.0, 4.5, 9.0, 17.0], 60, direction)
-    c, e, k = compare_push(small_scene, spec, fib, direction, "small")
+    c, e, k = compare_push(small_scene, spec, fib, direction, "small", precision=precision)
     assert c == 0 and e == 0 and k == 0
 
 
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("spec", [BULL10, CONE10], ids=["bull", "cone"])
 @pytest.mark.parametrize("direction", ["x", "y"])
-def test_push_config2_subset(gpu, spec, direction):
+def test_push_config2_subset(gpu, spec, direction, precision):
     """The 204,802-triangle sphere scene; 300 fibers per direction at three
-    of the twenty waterline levels."""
+    of the twenty waterline levels.  In mixed precision most candidate
+    pairs cannot change their fiber and are left out."""
     tris = W.config2_mesh()
     levels = W.config2_levels()
     fib = fibers_for(tris, spec["d"] / 2, [levels[0], levels[7], levels[15]], 300, direction)
     fib = fib[::3]
-    c, e, k = compare_push(tris, spec, fib, direction, "C2")
+    c, e, k = compare_push(tris, spec, fib, direction, "C2", precision=precision)
     assert c == 0 and e == 0 and k == 0
+    st = compare_push.last_stats
+    if precision != "exact64":
+        assert st["skipped"] > 0.3 * (st["pairs"] + st["skipped"]), st
 
 
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("name", ["bull", "cone"])
-def test_push_exact_filter_option(gpu, small_scene, name):
+def test_push_exact_filter_option(gpu, small_scene, name, precision):
     """Only pairs whose search boxes overlap: same as the oracle's exact
     filter.  On this scene the KD superset adds intervals (shaft contacts of
     triangles above f.z + L), so the two options differ."""
     spec = ALL[name]
     fib = fibers_for(small_scene, 5.0, [2.0, 6.0], 60, "x")
-    c, e, k = compare_push(small_scene, spec, fib, "x", "exact-filter", exact_filter=True)
+    c, e, k = compare_push(small_scene, spec, fib, "x", "exact-filter", exact_filter=True, precision=precision)
     assert c == 0 and e == 0 and k == 0
 
 
@@ -143,9 +158,10 @@ def comb(n, pitch):
     return np.array(tris, dtype=float)
 
 
+@pytest.mark.parametrize("precision", PRECISIONS)
 @pytest.mark.parametrize("direction", ["x", "y"])
 @pytest.mark.parametrize("kind", ["bull", "cone"])
-def test_push_comb_overflow_and_capacity(gpu, kind, direction):
+def test_push_comb_overflow_and_capacity(gpu, kind, direction, precision):
     """Fibers across 40 ridges hold 40 disjoint intervals (the sparse ones
     2): lists that fit the 64-entry LDS list and go to the compact pool.  A
     comb of 100 ridges gives 100-interval lists that outgrow LDS and run
@@ -168,18 +184,18 @@ def test_push_comb_overflow_and_capacity(gpu, kind, direction):
     dense, sparse = fibers(125.0), fibers(8.0)
     dev = capi.DeviceMesh(tris)
     for label, fib, nmin in (("comb-dense", dense, 40), ("comb-sparse", sparse, 2), ("comb-dense-again", dense, 40)):
-        got = dev.push(spec, fib, direction)
+        got = dev.push(spec, fib, direction, precision=precision)
         longest = int(np.max(np.diff(got["offsets"])))
         assert longest >= nmin, (label, longest)
-        c, e, k = compare_push(tris, spec, fib, direction, label, dev=dev)
+        c, e, k = compare_push(tris, spec, fib, direction, label, dev=dev, precision=precision)
         assert c == 0 and e == 0 and k == 0, label
     dev.close()
     tris = mesh(100)
     dev = capi.DeviceMesh(tris)
     fib = fibers(305.0)
-    got = dev.push(spec, fib, direction)
+    got = dev.push(spec, fib, direction, precision=precision)
     assert int(np.max(np.diff(got["offsets"]))) >= 100
-    c, e, k = compare_push(tris, spec, fib, direction, "comb-100", dev=dev)
+    c, e, k = compare_push(tris, spec, fib, direction, "comb-100", dev=dev, precision=precision)
     assert c == 0 and e == 0 and k == 0
     dev.close()
     # 2,000 fibers of 40 intervals: 80,000 entries, more than the first
@@ -194,6 +210,6 @@ def test_push_comb_overflow_and_capacity(gpu, kind, direction):
     else:
         fib = np.stack([a, np.full_like(a, -5.0), z, a, np.full_like(a, 125.0), z], axis=1)
     for label in ("pool-full", "pool-sized"):
-        c, e, k = compare_push(tris, spec, fib, direction, label, dev=dev)
+        c, e, k = compare_push(tris, spec, fib, direction, label, dev=dev, precision=precision)
         assert c == 0 and e == 0 and k == 0, label
     dev.close()

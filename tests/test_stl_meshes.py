@@ -111,10 +111,11 @@ def test_stl_mesh_drop_matches_oracle(gpu, name, kind, mode):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["exact64", "mixed"])
 @pytest.mark.parametrize("direction", ["x", "y"])
 @pytest.mark.parametrize("kind", ["bull", "cone"])
 @pytest.mark.parametrize("name", list(MESHES))
-def test_stl_mesh_push_matches_oracle(gpu, name, kind, direction):
+def test_stl_mesh_push_matches_oracle(gpu, name, kind, direction, precision):
     from test_push_gpu import compare_push
     tris = capi.stl_read(path_of(name))
     spec = scaled(ALL[kind], MESHES[name])
@@ -124,5 +125,5 @@ def test_stl_mesh_push_matches_oracle(gpu, name, kind, direction):
         xf, yf = W.waterline_fibers(tris, spec["d"] / 2.0, z0 + f * (z1 - z0), 100)
         fib.append(xf if direction == "x" else yf)
     fib = np.concatenate(fib)
-    c, e, k = compare_push(tris, spec, fib, direction, name)
+    c, e, k = compare_push(tris, spec, fib, direction, name, precision=precision)
     assert c == 0 and e == 0 and k == 0
