@@ -116,6 +116,7 @@ struct ocl_tree {
     ocl_amd::DevBuf tri, order, boxes, levels;
     ocl_amd::DevBuf acc;      // per rank: the KD search's leaf acceptance bounds (kd_rank_order)
     ocl_amd::DevBuf kdboxes;  // acceptance boxes, same level layout as boxes
+    ocl_amd::DevBuf reach;    // reach boxes (k_box_leaf_reach), same level layout
     uint32_t cnt[ocl_amd::kMaxLevels] = {};
     size_t box_off[ocl_amd::kMaxLevels] = {};
     int top = 0;
@@ -250,6 +251,34 @@ __global__ void k_box_leaf_uz(const double* __restrict__ rec, uint32_t ntri, flo
         }
     }
     out[b] = make_float4(__double2float_rd(u0), __double2float_rd(z0), __double2float_ru(u1), __double2float_ru(z1));
+}
+
+// Level-0 "reach boxes" of the push trees for the pair bound (push_bound):
+// the extent across the fiber and the highest vertex of the box's
+// well-conditioned triangles (x = lowest across, z = highest across, w =
+// highest vertex, y unused = -inf); an ill-conditioned triangle, which gets
+// no bound, makes the box reach everything.  A box none of whose triangles
+// can add an interval to a fiber fails the fiber's reach query.
+__global__ void k_box_leaf_reach(const double* __restrict__ rec, uint32_t ntri, float4* __restrict__ out,
+                                 uint32_t nbox, int cu) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    double u0 = INFINITY, u1 = -INFINITY, z1 = -INFINITY;
+    const uint32_t e = min(ntri, (b + 1) * kLeaf);
+    for (uint32_t i = b * kLeaf; i < e; ++i) {
+        const double* r = rec + (size_t)kRec * i;
+        if (r[13] == 0.0) {
+            u0 = -INFINITY;
+            u1 = INFINITY;
+            z1 = INFINITY;
+        }
+        for (int k = 0; k < 3; ++k) {
+            u0 = fmin(u0, r[3 * k + cu]);
+            u1 = fmax(u1, r[3 * k + cu]);
+        }
+        z1 = fmax(z1, r[12]);
+    }
+    out[b] = make_float4(__double2float_rd(u0), -INFINITY, __double2float_ru(u1), __double2float_ru(z1));
 }
 
 // Level-0 "acceptance boxes" of the push trees: the KD search reaches a
@@ -2972,6 +3001,11 @@ static int build_tree(ocl_mesh* m, ocl_tree& T, const int dims[4], int cu) {
     k_kdbox_leaf<<<(T.cnt[0] + 255) / 256, 256>>>(T.acc.as<double>(), n32, kb, T.cnt[0]);
     for (int l = 1; l <= T.top; ++l)
         k_box_up<<<(T.cnt[l] + 255) / 256, 256>>>(kb + T.box_off[l - 1], T.cnt[l - 1], kb + T.box_off[l], T.cnt[l]);
+    if ((rc = T.reach.ensure(sizeof(float4) * total))) return rc;
+    float4* rb = T.reach.as<float4>();
+    k_box_leaf_reach<<<(T.cnt[0] + 255) / 256, 256>>>(T.tri.as<double>(), n32, rb, T.cnt[0], cu);
+    for (int l = 1; l <= T.top; ++l)
+        k_box_up<<<(T.cnt[l] + 255) / 256, 256>>>(rb + T.box_off[l - 1], T.cnt[l - 1], rb + T.box_off[l], T.cnt[l]);
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipGetLastError());
     T.built = true;
@@ -3055,6 +3089,7 @@ static PushSide push_side(ocl_mesh* m, int dir, int superset) {
     S.pm = tree_view(T, m->ntri);
     if (superset) S.pm.tree.box = T.kdboxes.as<const float4>();  // walk the KD acceptance boxes
     S.acc = T.acc.as<const double>();
+    S.reach = T.reach.as<const float4>();
     S.fib = W.fib.as<const double>();
     S.perm = W.rcap ? W.retry.as<const uint32_t>() : W.perm.as<const uint32_t>();
     S.nf = W.rcap ? W.nretry : W.nf;

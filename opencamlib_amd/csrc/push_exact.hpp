@@ -799,17 +799,24 @@ OCL_HD void push_bound_rec(const double* p9, int axis, double out[3]) {
 }
 
 // Bound of one pair from the triangle's record (rec3 = push_bound_rec, zmax =
-// highest vertex) for the fiber from a to b along its axis at height fz.
-// Returns 0: no bound (composite cutters, ill-conditioned triangles);
-// 1: the pair adds no interval; 2: any interval it adds lies in [tl, tu]
-// (fiber parameter).  Ball and cone bounds use the profile width at the
-// highest reachable CC height (their contacts are exact geometric
-// tangencies); the other cutters use the radius.
-OCL_HD int push_bound(const CutterP& c, const double rec3[3], double zmax, double fz, double a, double b, double& tl,
-                      double& tu) {
+// highest vertex, c0/c1 = lowest/highest vertex coordinate across the
+// fiber) for the fiber from a to b along its axis, at height fz and
+// coordinate ac across it.  Returns 0: no bound (composite cutters,
+// ill-conditioned triangles); 1: the pair adds no interval (the triangle
+// lies below the fiber, or farther than the radius from it across: a CL
+// point on the fiber is within the radius of its CC point in XY, so its CC
+// point is within the radius of the fiber across it; the reference's KD
+// search returns such triangles, kdtree.hpp:273-304); 2: any interval the
+// pair adds lies in [tl, tu] (fiber parameter).  Ball and cone bounds use
+// the profile width at the highest reachable CC height (their contacts are
+// exact geometric tangencies); the other cutters use the radius.
+OCL_HD int push_bound(const CutterP& c, const double rec3[3], double zmax, double c0, double c1, double fz, double ac,
+                      double a, double b, double& tl, double& tu) {
     if (c.kind == K_COMPOSITE || rec3[0] == 0.0 || !(b != a)) return 0;
     const double hb = zmax - fz + 1e-4 + 1e-9 * (fabs(zmax) + fabs(fz));
     if (hb < 0.0) return 1;
+    const double ma = c.radius + 1e-3 + 1e-9 * (fabs(c0) + fabs(c1) + fabs(ac) + c.radius);
+    if (c1 < ac - ma || c0 > ac + ma) return 1;
     double w = c.radius;
     if (c.kind == K_BALL || c.kind == K_CONE) w = fmin(prof_width(c, hb), c.radius);
     w += 1e-3 + 1e-9 * (fabs(rec3[1]) + fabs(rec3[2]) + c.radius);

@@ -21,7 +21,7 @@ struct PEntry {
 };
 
 // Where the fibers' lists go.  First pass (rcap == 0): a list is built in
-// LDS (kLdsList entries) and copied into a compact pool at an offset taken
+// LDS (lds_list<K>() entries) and copied into a compact pool at an offset taken
 // from a counter; a fiber whose list outgrows LDS, or does not fit in the
 // pool, is appended to `retry` (and written with 0 entries).  Retry pass
 // (rcap > 0, over the `retry` fibers only): slot s of the pass builds its list
@@ -48,6 +48,7 @@ struct PushLists {
 struct PushSide {
     MeshView pm;
     const double* acc;
+    const float4* reach;  // reach boxes (pair bound), same layout as pm.tree.box
     const double* fib;
     const uint32_t* perm;
     uint32_t nf;
@@ -142,11 +143,13 @@ struct PushCounters {
 //
 // skip = 1: a candidate whose interval bound (push_bound) lies strictly
 // inside an interval of the fiber's list when it is queued, or whose
-// triangle lies below the fiber, is left out: it cannot change the list
-// (push_bound's comment), so the results are those of pushing every
-// candidate.  The test reads the list lane 0 keeps in LDS, which only grows
-// and merges while the fiber's candidates are taken in rank order; the retry
-// pass does not skip.
+// triangle lies below the fiber or beyond the radius across it, is left
+// out: it cannot change the list (push_bound's comment), so the results are
+// those of pushing every candidate.  The walk first skips whole level-0
+// boxes by their reach boxes (k_box_leaf_reach).  The containment test
+// reads the list lane 0 keeps in LDS, which only grows and merges while the
+// fiber's candidates are taken in rank order; the retry pass, whose list is
+// in global memory, only leaves out pairs that cannot reach the fiber.
 //
 // Merging: a batch's non-empty intervals are first merged among themselves
 // in parallel (sort by lower end, components by a prefix max), so lane 0
@@ -157,7 +160,12 @@ struct PushCounters {
 // pass that merges in global memory (PushLists).
 constexpr int kPushWaves = 4;  // wavefronts per 256-thread block
 constexpr int kBullSolves = 6; // Brent solves per pair: 3 edges x 2 roots
-constexpr uint32_t kLdsList = 64;
+// Interval list entries per fiber in LDS: 128 (61 KB per 4-wavefront block,
+// two blocks per CU); the BullCutter's solve tables leave room for 64.
+template <int K>
+constexpr uint32_t lds_list() {
+    return K == K_BULL ? 64u : 128u;
+}
 
 template <int K>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
@@ -169,6 +177,7 @@ k_push_fibers(const PushSide* __restrict__ sides, uint32_t nsides, uint32_t* __r
     __shared__ uint32_t tmem[kPushWaves][kS * 64];  // Bull: listed solves (lane << 3 | edge << 1 | root)
     __shared__ double bmem[kPushWaves][kS][64];     // Bull: brackets (lo, hi per edge)
     __shared__ double rmem[kPushWaves][kS][64];     // Bull: roots (d1, d2 per edge)
+    constexpr uint32_t kLdsList = lds_list<K>();
     __shared__ PEntry lmem[kPushWaves][kLdsList];   // the fiber's interval list
     const uint32_t lane = lane_id();
     const uint32_t wv = threadIdx.x >> 6;
@@ -367,9 +376,22 @@ k_push_fibers(const PushSide* __restrict__ sides, uint32_t nsides, uint32_t* __r
         };
         // the walk pauses whenever 64 candidates are queued, so the batch
         // code is emitted once
+        // with skip, the walk also needs the box's reach box to meet the
+        // fiber's reach query, a superset of push_bound's own tests for
+        // every triangle of the box: across +- (R + margin), highest vertex
+        // at or above the fiber less the slack
+        const bool prune = skip && K != K_COMPOSITE;
+        const double ma = R + 2e-3 + 1e-8 * (fabs(across) + R + 1.0);
+        const float ra0 = __double2float_rd(across - ma), ra1 = __double2float_ru(across + ma);
+        const float rz0 = __double2float_rd(f.p1.z - 2e-4 - 3e-9 * (fabs(f.p1.z) + 1.0));
+        const float4* __restrict__ reach = S.reach;
+        auto hit = [&](uint32_t i) -> bool {
+            return box_hits(pm.tree.box[i], qa0, qv0, qa1, qv1) &&
+                   (!prune || box_hits(reach[i], ra0, rz0, ra1, INFINITY));
+        };
         BoxWalk walk;
         for (;;) {
-            walk_boxes_resumable(pm.tree, qa0, qv0, qa1, qv1, walk, [](uint32_t) {}, [&](uint32_t b, uint32_t) -> bool {
+            walk_boxes_resumable_if(pm.tree, hit, walk, [](uint32_t) {}, [&](uint32_t b, uint32_t) -> bool {
                 const uint32_t k = b * kLeaf + lane;
                 bool cand = false;
                 if (lane < (uint32_t)kLeaf && k < pm.ntri) {
@@ -387,25 +409,33 @@ k_push_fibers(const PushSide* __restrict__ sides, uint32_t nsides, uint32_t* __r
                         cand = !(hi_a < a0 || lo_a > a1 || t.maxz < v0 || lo_z > v1);
                     }
                 }
-                if (skip && in_lds) {
+                if (skip) {
                     // the list's size is lane 0's; its entries are in LDS,
                     // written before the last batch's closing wave_lds_sync
-                    // (the retry pass, whose list is in global memory, pushes
-                    // every candidate)
-                    const uint32_t nl = uniform_u((uint32_t)__shfl((int)n, 0, 64));
+                    // (the retry pass, whose list is in global memory, tests
+                    // only whether a pair can reach its fiber)
+                    const uint32_t nl = in_lds ? uniform_u((uint32_t)__shfl((int)n, 0, 64)) : 0u;
                     if (cand) {
                         const double* r = pm.tri + (size_t)kRec * k;
                         const double rb[3] = {r[13], r[14], r[15]};
+                        const int ca = dir == 0 ? 1 : 0;  // coordinate across the fiber
+                        const double c0 = fmin(fmin(r[ca], r[3 + ca]), r[6 + ca]);
+                        const double c1 = fmax(fmax(r[ca], r[3 + ca]), r[6 + ca]);
                         double tl = 0.0, tu = 0.0;
-                        const int bd = push_bound(cd.c, rb, r[12], f.p1.z, dir == 0 ? f.p1.x : f.p1.y,
+                        const int bd = push_bound(cd.c, rb, r[12], c0, c1, f.p1.z, across, dir == 0 ? f.p1.x : f.p1.y,
                                                   dir == 0 ? f.p2.x : f.p2.y, tl, tu);
                         bool drop = bd == 1;
-                        if (bd == 2)
-                            for (uint32_t j = 0; j < nl && EL[j].lo < tl; ++j)
-                                if (tu < EL[j].hi) {
-                                    drop = true;
-                                    break;
-                                }
+                        if (bd == 2) {
+                            // the last entry starting below tl (the list is
+                            // sorted by lower end and disjoint)
+                            uint32_t lo = 0, hi = nl;
+                            while (lo < hi) {
+                                const uint32_t mid = (lo + hi) >> 1;
+                                if (EL[mid].lo < tl) lo = mid + 1;
+                                else hi = mid;
+                            }
+                            drop = lo > 0 && tu < EL[lo - 1].hi;
+                        }
                         skipped += drop ? 1 : 0;
                         cand = !drop;
                     }

@@ -561,8 +561,12 @@ extern "C" int ocl_selcheck_push_bound(const double* tri9, const ocl_cutter_desc
     double rec[3];
     push_bound_rec(tri9, dir == 0 ? 0 : 1, rec);
     const double zmax = std::max(std::max(tri9[2], tri9[5]), tri9[8]);
+    const int ca = dir == 0 ? 1 : 0;
+    const double c0 = std::min(std::min(tri9[ca], tri9[3 + ca]), tri9[6 + ca]);
+    const double c1 = std::max(std::max(tri9[ca], tri9[3 + ca]), tri9[6 + ca]);
     double tl = 0.0, tu = 0.0;
-    const int r = push_bound(cd.c, rec, zmax, fib6[2], fib6[dir == 0 ? 0 : 1], fib6[dir == 0 ? 3 : 4], tl, tu);
+    const int r = push_bound(cd.c, rec, zmax, c0, c1, fib6[2], fib6[ca], fib6[dir == 0 ? 0 : 1],
+                             fib6[dir == 0 ? 3 : 4], tl, tu);
     out[0] = tl;
     out[1] = tu;
     return r;

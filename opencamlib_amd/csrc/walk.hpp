@@ -122,9 +122,12 @@ struct BoxWalk {
     bool started = false, done = false;
 };
 
-template <class Block, class Visit>
-__device__ inline void walk_boxes_resumable(const BoxTree& T, float qx0, float qy0, float qx1, float qy1, BoxWalk& w,
-                                            Block&& block, Visit&& visit) {
+// hit(i) tests box i of the hierarchy's box arrays (all levels, as T.box)
+// against the walk's query: a node is entered, and a level-0 box visited,
+// only where it holds.
+template <class Hit, class Block, class Visit>
+__device__ inline void walk_boxes_resumable_if(const BoxTree& T, Hit&& hit, BoxWalk& w, Block&& block,
+                                               Visit&& visit) {
     const uint32_t lane = lane_id();
     auto get = [&](int l) -> uint32_t {
         return (uint32_t)(((l < 4) ? (w.mlo >> (16 * l)) : (w.mhi >> (16 * (l - 4)))) & 0xffffull);
@@ -135,9 +138,9 @@ __device__ inline void walk_boxes_resumable(const BoxTree& T, float qx0, float q
     };
     auto load_mask = [&](int l, uint32_t base) -> uint32_t {
         const uint32_t cnt = T.lvl[2 * l], off = T.lvl[2 * l + 1];
-        bool hit = false;
-        if (lane < (uint32_t)kFan && base + lane < cnt) hit = box_hits(T.box[off + base + lane], qx0, qy0, qx1, qy1);
-        return (uint32_t)__ballot(hit) & 0xffffu;
+        bool h = false;
+        if (lane < (uint32_t)kFan && base + lane < cnt) h = hit(off + base + lane);
+        return (uint32_t)__ballot(h) & 0xffffu;
     };
     const int top = T.top;
     const uint32_t top_cnt = T.lvl[2 * top];
@@ -177,6 +180,13 @@ __device__ inline void walk_boxes_resumable(const BoxTree& T, float qx0, float q
         put(w.l, load_mask(w.l, w.base));
         if (w.l == 0) block(w.base);
     }
+}
+
+template <class Block, class Visit>
+__device__ inline void walk_boxes_resumable(const BoxTree& T, float qx0, float qy0, float qx1, float qy1, BoxWalk& w,
+                                            Block&& block, Visit&& visit) {
+    walk_boxes_resumable_if(
+        T, [&](uint32_t i) { return box_hits(T.box[i], qx0, qy0, qx1, qy1); }, w, block, visit);
 }
 
 template <class Visit>

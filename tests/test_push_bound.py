@@ -2,7 +2,8 @@
 
 The mixed/FP64 push leaves out a (fiber, triangle) pair when a bound on the
 interval it could add lies strictly inside an interval the fiber already
-holds, or when its triangle lies below the fiber (push_bound in
+holds, or when its triangle lies below the fiber or farther than the radius
+from it across (the reference's KD search returns such triangles; push_bound in
 csrc/push_exact.hpp, host build in libocl_selcheck.so).  That is exact only
 if the bound holds every interval the reference's pushCutter
 (millingcutter.cpp:377-382) adds.  Checked here over the KD-search candidates
@@ -82,9 +83,10 @@ def test_bound_holds_on_sphere_scene_candidates(name):
         fib = (xf if d == 0 else yf)[rng.choice(2000, 3, replace=False)]
         c = check_pairs(L, spec, kd_pairs(tris, spec, fib, d))
         total = [a + b for a, b in zip(total, c)]
-    # the vertical facets of the icospheres get no bound; most pairs do
+    # the vertical facets of the icospheres get no bound; most pairs do, and
+    # a third or more of the KD search's candidates lie beyond the radius
     assert total[1] + total[2] > 0.9 * sum(total), total
-    assert total[0] > 0, total
+    assert total[0] > 0 and total[1] > 0.3 * sum(total), total
 
 
 @pytest.mark.parametrize("mesh,scale", [("cone_on_side", 3.0), ("wheel_in_box", 10.0)])
