@@ -787,7 +787,8 @@ OCL_HD void push_bound_rec(const double* p9, int axis, double out[3]) {
     double m = 1.0;
     for (int k = 0; k < 3; ++k) m = fmax(m, fmax(fabs(p9[3 * k]), fabs(p9[3 * k + 1])));
     const double m2 = m * m;
-    const double e0 = hypot(bx - ax, by - ay), e1 = hypot(cx - bx, cy - by), e2 = hypot(ax - cx, ay - cy);
+    const double e0 = sqrt(sqr(bx - ax) + sqr(by - ay)), e1 = sqrt(sqr(cx - bx) + sqr(cy - by));
+    const double e2 = sqrt(sqr(ax - cx) + sqr(ay - cy));
     // Rounding moves the test's decisions by about 50 eps m^2 / (shortest
     // edge) in XY and its denominators by 8 eps m^2: with these floors an
     // accepted CC point lies within 6e-5 mm of the triangle, well inside the

@@ -84,3 +84,74 @@ def test_push_flops_per_pair_match_the_bench_table(name):
         per_it = flops(list(ob)[:6] + [0]) / ob[6]
         print(f"one Brent solve: {ob[6]} iterations, {per_it:.1f} flops/iteration")
         assert 40 <= per_it <= 200
+
+
+def kept_pairs(tris, spec, fibs, om, cut):
+    """The candidate pairs the mixed-precision push keeps: each fiber's
+    KD-search candidates in rank order, less those push_bound leaves out (no
+    interval possible, or the bound strictly inside the fiber's list as
+    built from the batches of 64 kept pairs before), replayed on the host."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from test_push_bound import selcheck
+    from opencamlib_amd import capi
+    from oracle import oracle as O
+    L = selcheck()
+    desc = capi.cutter_desc(spec)
+    out = np.zeros(2)
+    keep = []
+    for j, fib in enumerate(fibs):
+        fib = np.ascontiguousarray(fib)
+        lst, queue = [], []
+
+        def flush():
+            nonlocal lst
+            for lo, hi in queue:
+                nl = []
+                for e in lst:
+                    if e[1] < lo or e[0] > hi:
+                        nl.append(e)
+                    else:
+                        lo, hi = min(lo, e[0]), max(hi, e[1])
+                nl.append((lo, hi))
+                lst = sorted(nl)
+            queue.clear()
+
+        for k in om.kd_search_push(cut, fib, 0):
+            t = np.ascontiguousarray(tris[k])
+            r = L.ocl_selcheck_push_bound(t.ctypes.data, C.addressof(desc), fib.ctypes.data, 0, out.ctypes.data)
+            if r == 1 or (r == 2 and any(e[0] < out[0] and out[1] < e[1] for e in lst)):
+                continue
+            keep.append((j, int(k)))
+            ne, iv = O.push_triangle(cut, fib, t)
+            if ne:
+                queue.append((iv[0], iv[1]))
+            if len(keep) % 64 == 0:
+                flush()
+        flush()
+    return keep
+
+
+@pytest.mark.parametrize("name", ["bull", "cone"])
+def test_push_flops_per_kept_pair_match_the_bench_table(name):
+    """The same count over the pairs the mixed-precision push evaluates
+    (bench.F_PUSH_KEPT, the mixed push line's roofline)."""
+    from opencamlib_amd import workloads as W
+    from oracle import oracle as O
+    L = build("flop_push")
+    kind, spec, par = PUSH[name]
+    tris = np.ascontiguousarray(W.config2_mesh())
+    om = O.Mesh(tris)
+    cut = O.cutter_from_spec(spec)
+    fibs = np.concatenate([W.waterline_fibers(tris, spec["d"] / 2, z, 2000)[0] for z in W.config2_levels()])
+    idx = np.random.default_rng(5).choice(len(fibs), 40, replace=False)
+    fs = np.ascontiguousarray(fibs[idx])
+    pairs = kept_pairs(tris.reshape(-1, 9), spec, fs, om, cut)
+    pr = np.ascontiguousarray(np.array(pairs, dtype=np.uint32))
+    sp = np.array(par, dtype=np.float64)
+    o = (C.c_ulonglong * 7)()
+    assert L.fc_push_pairs(kind, sp.ctypes.data_as(C.c_void_p), tris.ctypes.data_as(C.c_void_p),
+                           fs.ctypes.data_as(C.c_void_p), pr.ctypes.data_as(C.c_void_p), C.c_size_t(len(pr)), o) == 0
+    per_pair = flops(list(o)[:6] + [0]) / len(pr)
+    print(f"FP64 push {name}, kept pairs: {len(pr)}, {per_pair:.1f} flops/pair, {o[6] / len(pr):.2f} Brent "
+          f"iterations/pair")
+    assert abs(per_pair - bench.F_PUSH_KEPT[name]) <= 0.03 * bench.F_PUSH_KEPT[name]
