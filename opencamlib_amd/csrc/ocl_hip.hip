@@ -70,6 +70,7 @@ struct DevBuf {
 // instead of overwriting one a batch replays.
 struct RecSet {
     DevBuf rec, cap, emax;
+    DevBuf capup;  // FP32: highest reachable height per hierarchy box, absolute (k_capup_*)
     bool ok = false;
     CutterDev key{};
     double emax_h = 0.0;
@@ -313,6 +314,26 @@ __global__ void k_box_up(const float4* __restrict__ in, uint32_t nin, float4* __
         r.z = fmaxf(r.z, c.z);
         r.w = fmaxf(r.w, c.w);
     }
+    out[b] = r;
+}
+
+// Absolute caps of the hierarchy boxes (FP32 selection): level 0 is the
+// box's cap (highest reachable height, relative to its frame) plus the
+// frame's height, rounded up; a higher box takes the largest of its
+// children.
+__global__ void k_capup_leaf(const float* __restrict__ cap, const BoxOrigin* __restrict__ org, uint32_t nbox,
+                             float* __restrict__ out) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbox) return;
+    const float c = cap[b];
+    out[b] = (c == -INFINITY) ? -INFINITY : __double2float_ru((double)c + org[b].oz);
+}
+__global__ void k_capup_up(const float* __restrict__ in, uint32_t nin, float* __restrict__ out, uint32_t nout) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nout) return;
+    float r = -INFINITY;
+    const uint32_t e = min(nin, (b + 1) * kFan);
+    for (uint32_t i = b * kFan; i < e; ++i) r = fmaxf(r, in[i]);
     out[b] = r;
 }
 
@@ -722,6 +743,7 @@ struct SelArgs {
     const T* rec;
     const BoxOrigin* org;
     const T* boxcap;
+    const float* capup;      // FP32: absolute cap per hierarchy box (all levels), or NULL
     SelParams<T> P;
     double R;                // cutter radius
     const double* cl;
@@ -1090,8 +1112,28 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
         walk_boxes(m.tree, qx0, qy0, qx1, qy1, [&](uint32_t b) { eval_box(b, org[b], boxcap[b]); });
     } else {
     BoxWalk walk;
+    const float* __restrict__ capup = A.capup;
+    const bool wprune = K != K_CONE && P.cull && capup != nullptr;
     for (;;) {
-        walk_boxes_resumable(m.tree, qx0, qy0, qx1, qy1, walk, [](uint32_t) {}, [&](uint32_t b, uint32_t) -> bool {
+        // The walk leaves out boxes, and whole subtrees, whose absolute cap
+        // less the profile height at their distance from the hull stays
+        // below every lane's threshold: the same test the sorted list's
+        // early exit makes, with the thresholds known so far (they only
+        // rise), so none of them would have been evaluated.
+        double wthr = -INFINITY;
+        if (wprune && framed)
+            wthr = (double)wave_min_t<T>(act ? F::max_(W.zb, zc) : F::inf()) + ozw - (double)P.CUT;
+        auto hit = [&](uint32_t i) -> bool {
+            const float4 bx = m.tree.box[i];
+            if (!box_hits(bx, qx0, qy0, qx1, qy1)) return false;
+            if (!(wthr > -INFINITY)) return true;
+            const double gx = fmax(fmax((double)bx.x - hull_x1, hull_x0 - (double)bx.z), 0.0);
+            const double gy = fmax(fmax((double)bx.y - hull_y1, hull_y0 - (double)bx.w), 0.0);
+            const double key =
+                (double)capup[i] - (double)prof_lower<T, K>((T)__double2float_rd(gx * gx + gy * gy), P);
+            return key + 1e-5 * (1.0 + fabs(key)) >= wthr;
+        };
+        walk_boxes_resumable_if(m.tree, hit, walk, [](uint32_t) {}, [&](uint32_t b, uint32_t) -> bool {
             if (lane_id() == 0) bl_b[nlist] = b;
             ++nlist;
             return nlist < kBoxList;
@@ -2165,6 +2207,17 @@ static int ensure_records(ocl_mesh* m, const CutterDev& cd, std::shared_ptr<RecS
     k_box_cap<T><<<(nbox + 255) / 256, 256, 0, st>>>(rs.rec.as<T>(), n, nbox, nrec, rs.cap.as<T>(),
                                                      rs.emax.as<unsigned int>());
     HIP_TRY(hipGetLastError());
+    if constexpr (sizeof(T) == 4) {
+        // absolute caps of every hierarchy box, for the walk's best-first cull
+        const size_t total = m->box_off[m->top] + m->cnt[m->top];
+        if ((rc = rs.capup.ensure(sizeof(float) * total + 16))) return rc;
+        float* cu = rs.capup.as<float>();
+        k_capup_leaf<<<(nbox + 255) / 256, 256, 0, st>>>(rs.cap.as<float>(), m->origins.as<BoxOrigin>(), nbox, cu);
+        for (int l = 1; l <= m->top; ++l)
+            k_capup_up<<<(m->cnt[l] + 255) / 256, 256, 0, st>>>(cu + m->box_off[l - 1], m->cnt[l - 1],
+                                                                 cu + m->box_off[l], m->cnt[l]);
+        HIP_TRY(hipGetLastError());
+    }
     float em = 0.0f;
     HIP_TRY(hipMemcpyAsync(&em, rs.emax.p, 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -2188,6 +2241,7 @@ static SelArgs<T> sel_args(ocl_mesh* m, const MeshView& mv, const T* rec, const 
     A.rec = rec;
     A.org = m->origins.as<BoxOrigin>();
     A.boxcap = cap;
+    A.capup = nullptr;
     A.P = P;
     A.R = cd.c.radius;
     A.cl = d_cl;
@@ -2366,8 +2420,11 @@ static int fast_prepare(ocl_mesh* m, DropWork& w, const MeshView& mv, const Cutt
 }
 
 static SelArgs<float> fast_args32(const FastPlan& F) {
-    return sel_args<float>(F.m, F.mv, F.r32->rec.as<float>(), F.r32->cap.as<float>(), F.P32, F.cd, F.cl,
-                           F.perm, F.n, F.o, F.zfloor, F.nts, F.precision == OCL_PREC_MIXED ? 1 : 0, F.cnt);
+    SelArgs<float> A = sel_args<float>(F.m, F.mv, F.r32->rec.as<float>(), F.r32->cap.as<float>(), F.P32, F.cd,
+                                       F.cl, F.perm, F.n, F.o, F.zfloor, F.nts,
+                                       F.precision == OCL_PREC_MIXED ? 1 : 0, F.cnt);
+    A.capup = F.r32->capup.as<const float>();
+    return A;
 }
 
 static SelArgs<double> fast_args64(const FastPlan& F) {
