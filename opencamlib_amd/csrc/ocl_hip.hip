@@ -150,7 +150,7 @@ struct ocl_mesh {
     ocl_amd::DevBuf fid;         // 6 feature IDs per triangle (rank order; feature_ids.hpp)
     ocl_amd::DevBuf boxzmax;     // highest vertex per level-0 box
     ocl_amd::DevBuf gr_part;     // backward: per-block partial sums of dz/dr
-    ocl_amd::DevBuf gt_kv, gt_hist, gt_val, gt_part;  // backward: per-vertex sums in a fixed order
+    ocl_amd::DevBuf gt_kv, gt_hist, gt_part;  // backward: per-vertex sums in a fixed order
     double zmid = 0, zhalf = 0, xyb = 0;
     // selection records of the last cutter, per precision (shared with the
     // batches that use them)
@@ -3484,12 +3484,34 @@ int ocl_hip_drop_batch(ocl_mesh* m, const ocl_cutter_desc* desc, const double* c
 // ============================================================= backward
 namespace ocl_amd {
 
+// Sort keys of the backward's per-vertex sums: the point's contact triangle
+// (input index), or ntri when the point has no contact.
+__global__ void __launch_bounds__(256) k_grad_keys(const uint8_t* __restrict__ type, const uint32_t* __restrict__ tri,
+                                                   uint32_t n, uint32_t ntri, uint32_t* __restrict__ gkey,
+                                                   uint32_t* __restrict__ gidx) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t t = tri[i];
+    gkey[i] = (t != 0xffffffffu && t < ntri && type[i] != CC_NONE) ? t : ntri;
+    gidx[i] = i;
+}
+
+constexpr uint32_t kSegBlock = 256;  // positions per block of the per-triangle sums
+
 // One lane per CL point: derivatives of its dropped z from the recorded
-// active contact (drop_grad.hpp).  Per-vertex terms: the point's 9 weighted
-// derivatives go to gv[9 i ..] with key[i] = its contact triangle (input
-// index; ntri: none), and k_grad_seg / k_grad_carry add them per triangle in
-// point order after a stable sort by key, so every sum has one fixed order
-// (FP64 atomics made the last bits vary run to run).
+// active contact (drop_grad.hpp).
+// Per-vertex terms (gtri non-NULL): the lanes take the points in the order
+// of a stable sort by contact triangle (skey, sidx: ascending triangle,
+// equal triangles in point order), put their 9 weighted terms in LDS, and
+// the first lane of each run of equal triangles in the block adds the run's
+// terms in order.  A run that is a whole segment (it starts the triangle's
+// segment and the segment ends inside the block) goes straight to gtri; a
+// run that continues past the block end, or started before the block,
+// leaves its sum in seg_part (slot 2b for the run at the block's first
+// position, slot 2b+1 for the block's last run) and k_grad_carry adds the
+// pieces of each such segment in block order.  Every sum has one fixed
+// order (FP64 atomics made the last bits vary run to run), and the terms
+// never leave the chip.
 // dz/dr: each block writes its partial sum to gr_part[block]; k_sum_into
 // adds them up (one same-address atomic per wavefront serialised ~15 k
 // atomics per million points).
@@ -3498,17 +3520,20 @@ __global__ void __launch_bounds__(256) k_drop_grad(MeshView m, const uint32_t* _
                                                    const double* __restrict__ cl, const double* __restrict__ cc,
                                                    const uint8_t* __restrict__ type, const uint32_t* __restrict__ tri,
                                                    const uint8_t* __restrict__ feat, const double* __restrict__ gz,
-                                                   uint32_t n, double* __restrict__ dz, double* __restrict__ gv,
-                                                   uint32_t* __restrict__ gkey, uint32_t* __restrict__ gidx,
+                                                   uint32_t n, double* __restrict__ dz,
+                                                   const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sidx,
+                                                   double* __restrict__ gtri, double* __restrict__ seg_part,
                                                    double* __restrict__ gr_part) {
     __shared__ double wsum[4];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double gsh[9 * kSegBlock];
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = (p < n) ? (sidx ? sidx[p] : p) : 0u;
     double acc_r = 0.0;
-    if (i < n) {
+    GradOut o{};
+    double g = 0.0;
+    if (p < n) {
         const uint32_t t = tri[i];
-        uint32_t key = m.ntri;
         const int ty = type[i];
-        GradOut o{};
         if (t != 0xffffffffu && t < m.ntri && ty != CC_NONE) {
             const double* r = m.tri + (size_t)kRec * rank_of[t];
             const V3 V[3] = {v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), v3(r[6], r[7], r[8])};
@@ -3526,22 +3551,40 @@ __global__ void __launch_bounds__(256) k_drop_grad(MeshView m, const uint32_t* _
             } else {
                 o = contact_grad<T>(cd.c, V, f, ty, x, y, P);
             }
-            const double g = gz ? gz[i] : 1.0;
-            if (gv) {
-#pragma unroll
-                for (int j = 0; j < 9; ++j) gv[9 * (size_t)i + j] = g * o.gv[j];
-                key = t;
-            }
+            g = gz ? gz[i] : 1.0;
             acc_r = g * o.gr;
-        }
-        if (gkey) {
-            gkey[i] = key;
-            gidx[i] = i;
         }
         if (dz) {
             dz[3 * (size_t)i] = o.gx;
             dz[3 * (size_t)i + 1] = o.gy;
             dz[3 * (size_t)i + 2] = o.gr;
+        }
+    }
+    if (gtri) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) gsh[9 * threadIdx.x + j] = g * o.gv[j];
+        __syncthreads();
+        const uint32_t b0 = blockIdx.x * kSegBlock, bend = min(n, b0 + kSegBlock);
+        if (p < n) {
+            const uint32_t k = skey[p];
+            const bool first = threadIdx.x == 0;
+            if (k < m.ntri && (first || skey[p - 1] != k)) {
+                double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                uint32_t q = p;
+                for (; q < bend && skey[q] == k; ++q)
+#pragma unroll
+                    for (int j = 0; j < 9; ++j) a[j] += gsh[9 * (q - b0) + j];
+                const bool starts = p == 0 || skey[p - 1] != k;
+                const bool ends = q == n || skey[q] != k;
+                if (starts && ends) {
+#pragma unroll
+                    for (int j = 0; j < 9; ++j) gtri[9 * (size_t)k + j] += a[j];
+                } else {
+                    double* d = seg_part + 9 * (2 * (size_t)blockIdx.x + (first ? 0 : 1));
+#pragma unroll
+                    for (int j = 0; j < 9; ++j) d[j] = a[j];
+                }
+            }
         }
     }
     if (gr_part) {
@@ -3552,43 +3595,6 @@ __global__ void __launch_bounds__(256) k_drop_grad(MeshView m, const uint32_t* _
     }
 }
 
-// Per-triangle sums of the sorted terms (key[p], the contact triangle of
-// point idx[p], ascending; equal keys in point order).  Blocks of 256
-// positions: the first position of each run of equal keys inside a block
-// adds the run's terms in order.  A run that is a whole segment (it starts
-// the key's segment and the segment ends inside the block) goes straight to
-// gtri; a run that continues past the block end, or started before the
-// block, leaves its sum in part: slot 2b for the run at the block's first
-// position, slot 2b+1 for the block's last run, and k_grad_carry adds the
-// pieces of each such segment in block order.
-constexpr uint32_t kSegBlock = 256;
-__global__ void __launch_bounds__(256) k_grad_seg(const uint32_t* __restrict__ key, const uint32_t* __restrict__ idx,
-                                                  const double* __restrict__ gv, uint32_t n, uint32_t ntri,
-                                                  double* __restrict__ gtri, double* __restrict__ part) {
-    const uint32_t p = blockIdx.x * kSegBlock + threadIdx.x;
-    const uint32_t bend = min(n, (blockIdx.x + 1) * kSegBlock);
-    if (p >= n) return;
-    const uint32_t k = key[p];
-    const bool first = threadIdx.x == 0;
-    if (k >= ntri || (!first && key[p - 1] == k)) return;
-    double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    uint32_t q = p;
-    for (; q < bend && key[q] == k; ++q) {
-        const double* v = gv + 9 * (size_t)idx[q];
-#pragma unroll
-        for (int j = 0; j < 9; ++j) a[j] += v[j];
-    }
-    const bool starts = p == 0 || key[p - 1] != k;
-    const bool ends = q == n || key[q] != k;
-    if (starts && ends) {
-#pragma unroll
-        for (int j = 0; j < 9; ++j) gtri[9 * (size_t)k + j] += a[j];
-    } else {
-        double* d = part + 9 * (2 * (size_t)blockIdx.x + (first ? 0 : 1));
-#pragma unroll
-        for (int j = 0; j < 9; ++j) d[j] = a[j];
-    }
-}
 // One thread per block b: the segments that start in block b and cross its
 // end (the run at the block's first position when it starts its segment, or
 // the block's last run) collect their pieces from the following blocks'
@@ -3664,35 +3670,38 @@ extern "C" int ocl_hip_drop_backward(ocl_mesh* m, const ocl_cutter_desc* desc, c
         if ((rc = m->gr_part.ensure(8ull * nb + 64))) return rc;
         part = m->gr_part.as<double>();
     }
-    double* gv = nullptr;
-    uint32_t *gkey = nullptr, *gidx = nullptr;
     const uint32_t nseg = (n32 + kSegBlock - 1) / kSegBlock;
+    const uint32_t *ks = nullptr, *is = nullptr;
+    double* gp = nullptr;
     if (d_grad_tri) {
+        // stable sort of the points by contact triangle (keys up to ntri);
+        // the gradient kernel then takes them in that order
         if ((rc = m->gt_kv.ensure(8ull * n + 64)) || (rc = m->gt_hist.ensure(4ull * rs_scratch_words(n32))) ||
-            (rc = m->gt_val.ensure(72ull * n + 64)) || (rc = m->gt_part.ensure(144ull * nseg + 64)))
+            (rc = m->gt_part.ensure(144ull * nseg + 64)))
             return rc;
-        gv = m->gt_val.as<double>();
-        gkey = m->gt_kv.as<uint32_t>();
-        gidx = gkey + n32;
-    }
-    if (precision == OCL_PREC_FP32 || precision == OCL_PREC_MIXED)
-        k_drop_grad<float><<<nb, 256, 0, st>>>(mv, rk, cd, d_cl, d_cc, d_type, d_tri, d_feat, d_grad_z, n32, d_dz,
-                                               gv, gkey, gidx, part);
-    else
-        k_drop_grad<double><<<nb, 256, 0, st>>>(mv, rk, cd, d_cl, d_cc, d_type, d_tri, d_feat, d_grad_z, n32, d_dz,
-                                                gv, gkey, gidx, part);
-    HIP_TRY(hipGetLastError());
-    if (d_grad_tri) {
-        // stable sort by contact triangle (keys up to ntri), then the
-        // per-triangle sums in point order
+        uint32_t* gkey = m->gt_kv.as<uint32_t>();
+        uint32_t* gidx = gkey + n32;
         const uint32_t nt32 = (uint32_t)m->ntri;
+        k_grad_keys<<<nb, 256, 0, st>>>(d_type, d_tri, n32, nt32, gkey, gidx);
+        HIP_TRY(hipGetLastError());
         uint32_t bits = 1;
         while (bits < 32 && (nt32 >> bits) != 0) ++bits;
-        uint32_t *ks = nullptr, *is = nullptr;
-        rs_sort(gkey, gidx, n32, bits, m->gt_hist.as<uint32_t>(), &ks, &is, st);
-        double* gp = m->gt_part.as<double>();
-        k_grad_seg<<<nseg, kSegBlock, 0, st>>>(ks, is, gv, n32, nt32, d_grad_tri, gp);
-        k_grad_carry<<<(nseg + 255) / 256, 256, 0, st>>>(ks, n32, nt32, nseg, d_grad_tri, gp);
+        uint32_t *ks_, *is_;
+        rs_sort(gkey, gidx, n32, bits, m->gt_hist.as<uint32_t>(), &ks_, &is_, st);
+        ks = ks_;
+        is = is_;
+        gp = m->gt_part.as<double>();
+    }
+    static_assert(kSegBlock == 256, "one gradient block per sum block");
+    if (precision == OCL_PREC_FP32 || precision == OCL_PREC_MIXED)
+        k_drop_grad<float><<<nb, 256, 0, st>>>(mv, rk, cd, d_cl, d_cc, d_type, d_tri, d_feat, d_grad_z, n32, d_dz,
+                                               ks, is, d_grad_tri, gp, part);
+    else
+        k_drop_grad<double><<<nb, 256, 0, st>>>(mv, rk, cd, d_cl, d_cc, d_type, d_tri, d_feat, d_grad_z, n32, d_dz,
+                                                ks, is, d_grad_tri, gp, part);
+    HIP_TRY(hipGetLastError());
+    if (d_grad_tri) {
+        k_grad_carry<<<(nseg + 255) / 256, 256, 0, st>>>(ks, n32, (uint32_t)m->ntri, nseg, d_grad_tri, gp);
         HIP_TRY(hipGetLastError());
     }
     if (d_grad_r) {
