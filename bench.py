@@ -622,7 +622,16 @@ def main():
                 "roofline": {"bound": "valu", "kernel": "k_drop_select<float,bull>", "achieved": b_ach,
                              "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": b_ach / PEAK_FP32,
                              "frac_unpacked": b_ach / PEAK_FP32_UNPACKED, "pairs_per_launch": int(b_st["pairs"]),
-                             "flop_per_pair": F_PAIR["bull"], "kernel_ms": b_sel, "traffic": None}}
+                             "flop_per_pair": F_PAIR["bull"], "kernel_ms": b_sel,
+                             "lane_utilisation": (b_st["pairs"] / (64.0 * b_st["wave_evals"])
+                                                  if b_st["wave_evals"] else None),
+                             "traffic": traffic_from_profile("bull"),
+                             "traffic_source": "profiles/drop_select_bull_c1_hbm_bytes.md"}}
+        bc = valu_from_profile("bull")
+        if bc:
+            bull["roofline"]["counters"] = {"valu_busy": bc.get("VALU_BUSY_FRACTION"),
+                                            "executed_fp32_tflops": bc.get("EXECUTED_FP32_TFLOPS"),
+                                            "source": "profiles/drop_select_bull_c1_valu_counters.md"}
         if not args.no_cpu:
             from oracle import oracle as O
             idx = np.sort(np.random.default_rng(100).choice(n, 20_000, replace=False))
