@@ -3269,7 +3269,7 @@ static int push_run(ocl_mesh* m, const CutterDev& cd, int superset, int skip, co
         }
         if ((rc = m->push_sides.ensure(sizeof(S)))) return rc;
         HIP_TRY(hipMemcpyAsync(m->push_sides.p, S, sizeof(PushSide) * nt, hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemsetAsync(m->push_queue.p, 0, 4, st));
+        HIP_TRY(hipMemsetAsync(m->push_queue.p, 0, 4 * kXcds, st));
         HIP_TRY(hipEventRecord(m->ev0, st));
         const uint32_t nb = (uint32_t)std::min<size_t>((nfib + kPushWaves - 1) / kPushWaves, 2048);
         uint32_t* q = m->push_queue.as<uint32_t>();

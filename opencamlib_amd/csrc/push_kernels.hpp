@@ -190,15 +190,32 @@ k_push_fibers(const PushSide* __restrict__ sides, uint32_t nsides, uint32_t* __r
     // registers and spills
     const uint32_t nf0 = sides[0].nf;
     const uint32_t total = nf0 + (nsides > 1 ? sides[1].nf : 0u);
-    // fibers cost very different amounts (a fiber through a sphere against
+    // Fibers cost very different amounts (a fiber through a sphere against
     // one that misses every triangle), so each wavefront takes the next
-    // fiber of the sorted order from a device counter when it is done,
-    // instead of a fixed share
+    // fiber from a device counter when it is done, instead of a fixed share.
+    // The sorted fiber order is cut into one contiguous range per XCD
+    // (blocks go to the XCDs round-robin), each with its own counter
+    // (queue[0..kXcds)): the wavefronts of an XCD take neighbouring fibers,
+    // whose candidate triangles overlap, so they share that XCD's L2; an
+    // XCD whose range is done takes fibers from the next ranges.
+    const uint32_t xcd = blockIdx.x % kXcds;
+    uint32_t tried = 0;  // ranges found empty (uniform)
     for (;;) {
-        uint32_t w = 0;
-        if (lane == 0) w = atomicAdd(queue, 1u);
-        w = uniform_u((uint32_t)__shfl((int)w, 0, 64));
-        if (w >= total) break;
+        uint32_t w = 0xffffffffu;
+        while (tried < kXcds) {
+            const uint32_t r = (xcd + tried) % kXcds;
+            const uint32_t r0 = (uint32_t)((uint64_t)total * r / kXcds);
+            const uint32_t r1 = (uint32_t)((uint64_t)total * (r + 1) / kXcds);
+            uint32_t j = 0;
+            if (lane == 0) j = atomicAdd(queue + r, 1u);
+            j = uniform_u((uint32_t)__shfl((int)j, 0, 64));
+            if (j < r1 - r0) {
+                w = r0 + j;
+                break;
+            }
+            ++tried;
+        }
+        if (w == 0xffffffffu) break;
         const bool second = w >= nf0;
         const PushSide& S = sides[second ? 1 : 0];
         const MeshView pm = S.pm;
