@@ -1511,8 +1511,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
 // every job's chunks (chunk_off: prefix sums of the jobs' chunk counts,
 // largest jobs first so the tail is short) and run the job's pass on them.
 // Cutter kinds are dispatched per chunk; composite jobs are not queued.
+// Three wavefronts per SIMD: with the four kinds' code in one body, a budget
+// of four (128 VGPRs) spilled 17 VGPRs to scratch in the FP32 pass and took
+// the C3 batch's selection from 20.5 to 21.2 ms.
 template <class T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 k_drop_select_jobs(const SelArgs<T>* __restrict__ jobs, const uint32_t* __restrict__ chunk_off, uint32_t njobs,
                    uint32_t* __restrict__ queue) {
     __shared__ __align__(16) T srec[4][kLeaf * rec_len<T>()];
