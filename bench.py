@@ -151,15 +151,21 @@ def traffic_from_profile(cutter):
     return None
 
 
+def profile_line(name, key, kernel):
+    """The value of a `KEY kernel value` line of a committed rocprofv3
+    summary under profiles/, or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(path):
+        for line in open(path):
+            if line.startswith(f"{key} {kernel} "):
+                return float(line.split()[-1])
+    return None
+
+
 def push_traffic(cutter):
     """HBM bytes per launch of the push kernel (mixed precision, X and Y
     fibers of the push workload), from the committed rocprofv3 summary."""
-    path = os.path.join(ROOT, "profiles", "push_fibers_c2_counters.md")
-    if os.path.exists(path):
-        for line in open(path):
-            if line.startswith(f"TRAFFIC_BYTES_PER_LAUNCH k_push_fibers<{cutter}>"):
-                return float(line.split()[-1])
-    return None
+    return profile_line("push_fibers_c2_counters.md", "TRAFFIC_BYTES_PER_LAUNCH", f"k_push_fibers<{cutter}>")
 
 
 def valu_from_profile(cutter):
@@ -310,6 +316,12 @@ def bench_push(capi, W, cpu_fibers, do_cpu):
                              "peak": PEAK_FP64, "unit": "TFLOP/s (FP64)", "frac": ach / PEAK_FP64,
                              "flop_per_pair": F_PUSH_KEPT[name], "pairs_per_launch": int(st["pairs"]),
                              "kernel_ms": kms, "traffic": push_traffic(name),
+                             "traffic_source": "profiles/push_fibers_c2_counters.md",
+                             "counters": {"valu_busy": profile_line("push_fibers_c2_counters.md", "VALU_BUSY_FRACTION",
+                                                                    f"k_push_fibers<{name}>"),
+                                          "executed_fp64_tflops": profile_line(
+                                              "push_fibers_c2_counters.md", "EXECUTED_FP64_TFLOPS",
+                                              f"k_push_fibers<{name}>")},
                              "reference_equivalent": {"pairs": int(sa["pairs"]), "flop_per_pair": F_PUSH[name],
                                                       "tflops": ref_tf, "frac": ref_tf / PEAK_FP64}}}
         if do_cpu:
@@ -457,7 +469,15 @@ def bench_batch(torch, capi, W, steps):
                          "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": sel_ach / PEAK_FP32,
                          "frac_unpacked": sel_ach / PEAK_FP32_UNPACKED, "pairs_per_launch": int(sel_pairs),
                          "flop_per_pair": "per job: " + json.dumps(F_PAIR), "kernel_ms": st["select_ms"],
-                         "eager_ms_per_step": st["kernel_ms"], "traffic": None}}
+                         "eager_ms_per_step": st["kernel_ms"],
+                         "traffic": profile_line("batch_c3_counters.md", "TRAFFIC_BYTES_PER_LAUNCH",
+                                                 "k_drop_select_jobs<float>"),
+                         "traffic_source": "profiles/batch_c3_counters.md",
+                         "counters": {"valu_busy": profile_line("batch_c3_counters.md", "VALU_BUSY_FRACTION",
+                                                                "k_drop_select_jobs<float>"),
+                                      "executed_fp32_tflops": profile_line("batch_c3_counters.md",
+                                                                           "EXECUTED_FP32_TFLOPS",
+                                                                           "k_drop_select_jobs<float>")}}}
 
 
 def main():
