@@ -70,7 +70,7 @@ struct DevBuf {
 // instead of overwriting one a batch replays.
 struct RecSet {
     DevBuf rec, cap, emax;
-    DevBuf capup;  // FP32: highest reachable height per hierarchy box, absolute (k_capup_*)
+    DevBuf capup;  // highest reachable height per hierarchy box, absolute (k_capup_*)
     DevBuf topv;   // FP32, simple cutters: highest vertex per level-0 box, box frame (k_box_topv)
     bool ok = false;
     CutterDev key{};
@@ -322,11 +322,12 @@ __global__ void k_box_up(const float4* __restrict__ in, uint32_t nin, float4* __
 // box's cap (highest reachable height, relative to its frame) plus the
 // frame's height, rounded up; a higher box takes the largest of its
 // children.
-__global__ void k_capup_leaf(const float* __restrict__ cap, const BoxOrigin* __restrict__ org, uint32_t nbox,
+template <class T>
+__global__ void k_capup_leaf(const T* __restrict__ cap, const BoxOrigin* __restrict__ org, uint32_t nbox,
                              float* __restrict__ out) {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbox) return;
-    const float c = cap[b];
+    const T c = cap[b];
     out[b] = (c == -INFINITY) ? -INFINITY : __double2float_ru((double)c + org[b].oz);
 }
 __global__ void k_capup_up(const float* __restrict__ in, uint32_t nin, float* __restrict__ out, uint32_t nout) {
@@ -1710,6 +1711,7 @@ constexpr int kCoopWaves = 4;  // wavefronts per 256-thread block
 struct BoxCull {
     const double* cap;
     const BoxOrigin* org;
+    const float* capup;  // absolute caps of every hierarchy box (the cap array's records), or NULL
 };
 
 // The walk's level-0 boxes are tested four at a time, one triangle per lane
@@ -1741,12 +1743,25 @@ __device__ inline void coop_gather(const MeshView& m, uint32_t* qbuf, double x, 
             phy = o.hy;
         }
     };
+    // With the hierarchy's absolute caps the walk enters no box, at any
+    // level, whose cap less the profile height at its distance from the
+    // point stays below the threshold (the contacts of every triangle under
+    // it do too; the threshold only rises while the point is folded).
+    auto hit = [&](uint32_t i) -> bool {
+        const float4 bx = m.tree.box[i];
+        if (!box_hits(bx, qx0, qy0, qx1, qy1)) return false;
+        if (!bc.capup) return true;
+        const double gx = fmax(fmax((double)bx.x - x, x - (double)bx.z), 0.0);
+        const double gy = fmax(fmax((double)bx.y - y, y - (double)bx.w), 0.0);
+        const double d2 = fmax(fma(gx, gx, gy * gy) * (1.0 - 1e-12) - 1e-12, 0.0);
+        return (double)bc.capup[i] - prof_lower64(c, d2) >= thr();
+    };
     BoxWalk w;
     for (;;) {
         // up to four level-0 boxes that survive the box-level cull
         uint32_t pb0 = 0, pb1 = 0, pb2 = 0, pb3 = 0, npend = 0;  // wave-uniform
         if (!w.done)
-            walk_boxes_resumable(m.tree, qx0, qy0, qx1, qy1, w, block, [&](uint32_t b, uint32_t jb) -> bool {
+            walk_boxes_resumable_if(m.tree, hit, w, block, [&](uint32_t b, uint32_t jb) -> bool {
                 if (bc.cap) {
                     // every contact in the box is below cap + oz - h(d)
                     const double ox = readlane_d(pox, jb), oy = readlane_d(poy, jb), oz = readlane_d(poz, jb);
@@ -1806,7 +1821,7 @@ __device__ __attribute__((always_inline)) inline void coop_exact_points(
     const MeshView& m, const CutterDev& cd, const double* __restrict__ cl, const uint32_t* __restrict__ list,
     uint32_t n_eff, const double* __restrict__ zfloor, const double* __restrict__ boxcap,
     const BoxOrigin* __restrict__ org, const DropOut& o, uint32_t* qbuf, unsigned long long& brent, uint32_t w0,
-    uint32_t nwaves) {
+    uint32_t nwaves, const float* __restrict__ capup = nullptr) {
     const uint32_t lane = lane_id();
     for (uint32_t w = w0; w < n_eff; w += nwaves) {
         const uint32_t src = uniform_u(list[w]);
@@ -1837,7 +1852,7 @@ __device__ __attribute__((always_inline)) inline void coop_exact_points(
             if constexpr (Comp) return t.maxz - comp_prof_lower64(cd, tri_box_d2(t, x, y)) >= thr();
             return contact_cap(c, t, x, y) >= thr();
         };
-        const BoxCull bcl{(crun && !Comp) ? boxcap : nullptr, org};
+        const BoxCull bcl{(crun && !Comp) ? boxcap : nullptr, org, (crun && !Comp && boxcap) ? capup : nullptr};
         coop_gather(m, qbuf, x, y, c, bcl, thr, keep, [&](uint32_t* q, uint32_t nb) {
             const bool has = lane < nb;
             const uint32_t k = has ? q[lane] : 0u;
@@ -1947,12 +1962,12 @@ __global__ void __launch_bounds__(256) k_drop_coop_exact(MeshView m, CutterDev c
                                                          const double* __restrict__ zfloor,
                                                          const double* __restrict__ boxcap,
                                                          const BoxOrigin* __restrict__ org, DropOut o,
-                                                         Counters* cnt) {
+                                                         Counters* cnt, const float* __restrict__ capup) {
     __shared__ uint32_t qmem[kCoopWaves][kCoopQueue];
     const uint32_t wv = threadIdx.x >> 6;
     unsigned long long brent = 0;
     coop_exact_points<false>(m, cd, cl, list, *count, zfloor, boxcap, org, o, qmem[wv], brent,
-                             xcd_block(m.xcd_group) * kCoopWaves + wv, gridDim.x * kCoopWaves);
+                             xcd_block(m.xcd_group) * kCoopWaves + wv, gridDim.x * kCoopWaves, capup);
     if (cnt) wave_add(&cnt->brent, brent);
 }
 
@@ -1992,6 +2007,7 @@ struct PostJob {
     const BoxOrigin* org;
     Counters* cnt2;
     Counters* cnt3;
+    const float* capup;  // absolute hierarchy caps of the job's FP64 records, or NULL
 };
 
 __global__ void __launch_bounds__(256) k_drop_resolve_jobs(const PostJob* __restrict__ jobs,
@@ -2032,7 +2048,7 @@ __global__ void __launch_bounds__(256) k_drop_coop_exact_jobs(const PostJob* __r
         const uint32_t w0 = (blockIdx.x * kCoopWaves + wv + nwaves - first % nwaves) % nwaves;
         unsigned long long brent = 0;
         coop_exact_points<false>(J.m, J.cd, J.cl, J.q1, nq, J.zfloor, J.cap64, J.org, J.o, qmem[wv], brent, w0,
-                                 nwaves);
+                                 nwaves, J.capup);
         if (J.cnt3) wave_add(&J.cnt3->brent, brent);
         first += nq;
     }
@@ -2285,16 +2301,19 @@ static int ensure_records(ocl_mesh* m, const CutterDev& cd, std::shared_ptr<RecS
     k_box_cap<T><<<(nbox + 255) / 256, 256, 0, st>>>(rs.rec.as<T>(), n, nbox, nrec, rs.cap.as<T>(),
                                                      rs.emax.as<unsigned int>());
     HIP_TRY(hipGetLastError());
-    if constexpr (sizeof(T) == 4) {
-        // absolute caps of every hierarchy box, for the walk's best-first cull
+    {
+        // absolute caps of every hierarchy box: the FP32 selection's walk
+        // cull (FP32 records) and the exact tier's (FP64 records)
         const size_t total = m->box_off[m->top] + m->cnt[m->top];
         if ((rc = rs.capup.ensure(sizeof(float) * total + 16))) return rc;
-        if (!comp) {
-            if ((rc = rs.topv.ensure(sizeof(float4) * nbox + 16))) return rc;
-            k_box_topv<<<(nbox + 255) / 256, 256, 0, st>>>(rs.rec.as<float>(), n, nbox, rs.topv.as<float4>());
+        if constexpr (sizeof(T) == 4) {
+            if (!comp) {
+                if ((rc = rs.topv.ensure(sizeof(float4) * nbox + 16))) return rc;
+                k_box_topv<<<(nbox + 255) / 256, 256, 0, st>>>(rs.rec.as<float>(), n, nbox, rs.topv.as<float4>());
+            }
         }
         float* cu = rs.capup.as<float>();
-        k_capup_leaf<<<(nbox + 255) / 256, 256, 0, st>>>(rs.cap.as<float>(), m->origins.as<BoxOrigin>(), nbox, cu);
+        k_capup_leaf<T><<<(nbox + 255) / 256, 256, 0, st>>>(rs.cap.as<T>(), m->origins.as<BoxOrigin>(), nbox, cu);
         for (int l = 1; l <= m->top; ++l)
             k_capup_up<<<(m->cnt[l] + 255) / 256, 256, 0, st>>>(cu + m->box_off[l - 1], m->cnt[l - 1],
                                                                  cu + m->box_off[l], m->cnt[l]);
@@ -2562,8 +2581,9 @@ static int fast_post(const FastPlan& F, hipStream_t st) {
         exact_list = F.q2;
         exact_n = F.qcount + 1;
     }
+    const float* capup = F.r64->capup.p ? F.r64->capup.as<const float>() : nullptr;
     k_drop_coop_exact<<<kCoopBlocks, 256, 0, st>>>(mv, cd, F.cl, exact_list, exact_n, F.zfloor, cap64, org, F.o,
-                                                   F.cnt3);
+                                                   F.cnt3, capup);
     HIP_TRY(hipGetLastError());
     *F.exact_slot = (int)(exact_n - F.qcount);
     return OCL_OK;
@@ -4060,7 +4080,8 @@ int ocl_hip_batch_create(const ocl_drop_job* jobs, size_t njobs, int precision, 
             const FastPlan& F = J.F;
             ocl_mesh* m = F.m;
             pj.push_back(PostJob{F.mv, F.cd, F.cl, F.perm, F.n, F.nts, F.etie, F.o, F.q1, F.qcount, F.zfloor,
-                                 F.r64->cap.as<double>(), m->origins.as<BoxOrigin>(), F.cnt2, F.cnt3});
+                                 F.r64->cap.as<double>(), m->origins.as<BoxOrigin>(), F.cnt2, F.cnt3,
+                                 F.r64->capup.p ? F.r64->capup.as<const float>() : nullptr});
             poff.push_back(poff.back() + (F.n + 63) / 64);
             J.w->exact_slot = 0;
         }
