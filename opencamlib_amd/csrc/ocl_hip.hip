@@ -852,6 +852,10 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
     // XY hull of the wave's points
     const double hull_x0 = wave_min(act ? x : INFINITY), hull_x1 = wave_max(act ? x : -INFINITY);
     const double hull_y0 = wave_min(act ? y : INFINITY), hull_y1 = wave_max(act ? y : -INFINITY);
+    // XY hull of the points of the lane's row of 16 (neighbours in the
+    // selection order; +-inf for a row without points)
+    const double row_x0 = row_min_t<double>(act ? x : INFINITY), row_x1 = row_max_t<double>(act ? x : -INFINITY);
+    const double row_y0 = row_min_t<double>(act ? y : INFINITY), row_y1 = row_max_t<double>(act ? y : -INFINITY);
     // Floor of every lane's final height from the vertices of the triangles
     // right under the wavefront's points (boxes overlapping their XY hull).
     T zc = -F::inf();
@@ -977,8 +981,6 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
             const bool rel = act & (cap + dz - prof_lower<T, K>(d2, P) >= F::max_(W.zb, zc) - P.CUT);
             if (!__any(rel)) return;
         }
-        const T lx0 = (T)((double)qx0 - bo.ox), lx1 = (T)((double)qx1 - bo.ox);
-        const T ly0 = (T)((double)qy0 - bo.oy), ly1 = (T)((double)qy1 - bo.oy);
         // stage the box's records in LDS with coalesced 16-byte loads: all
         // of them are in flight together instead of one dependent load chain
         // per triangle
@@ -1006,29 +1008,41 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
         S.zr = W.zr - dz;
         for (int i = 0; i < 4; ++i) S.zt[i] = -F::inf();
         S.za = -F::inf();
-        // wave-level pre-test, one lane per triangle of the box: its XY box
-        // must meet the wave's query box, and (profile cull) its cap at the
-        // distance to the hull of the wave's points must reach the lowest
-        // threshold of any lane
+        // Pre-test of the box's triangles, one lane per (row of 16 points,
+        // triangle): the per-lane tests of the triangle loop below, made for
+        // the XY hull of the row's points and the row's lowest threshold.
+        // Each bound is at least the one of any point of the row (rounding
+        // is monotone: the hull's coordinates in the box frame bound the
+        // points' own), so a triangle no row keeps reaches no lane.
         uint32_t tmask;
         {
-            const uint32_t jl = lane_id();
+            const uint32_t jl = lane_id() & (uint32_t)(kLeaf - 1);
+            static_assert(kLeaf == 16, "four rows of one triangle per lane");
             bool keep = false;
-            T thr_min = F::inf();
-            if (P.cull) thr_min = wave_min_t<T>(act ? F::max_(S.zb, zc - dz) : F::inf()) - P.CUT;
+            T thr_row = F::inf();
+            if (P.cull) thr_row = row_min_t<T>(act ? F::max_(S.zb, zc - dz) : F::inf()) - P.CUT;
             if (jl < nt) {
                 const T* r = my + rec_len<T>() * jl;
                 const T bx0 = r[R_BX0], bx1 = r[R_BX1], by0 = r[R_BY0], by1 = r[R_BY1];
-                keep = !((bx1 < lx0) | (bx0 > lx1) | (by1 < ly0) | (by0 > ly1));
+                const T hx0 = (T)(row_x0 - bo.ox), hx1 = (T)(row_x1 - bo.ox);
+                const T hy0 = (T)(row_y0 - bo.oy), hy1 = (T)(row_y1 - bo.oy);
+                keep = !((bx1 < hx0 - P.R) | (bx0 > hx1 + P.R) | (by1 < hy0 - P.R) | (by0 > hy1 + P.R));
                 if (P.cull) {
-                    const T hx0 = (T)(hull_x0 - bo.ox), hx1 = (T)(hull_x1 - bo.ox);
-                    const T hy0 = (T)(hull_y0 - bo.oy), hy1 = (T)(hull_y1 - bo.oy);
                     const T ddx = F::max_(F::max_(bx0 - hx1, hx0 - bx1), (T)0);
                     const T ddy = F::max_(F::max_(by0 - hy1, hy0 - by1), (T)0);
-                    keep = keep & (r[R_ZCAP] - prof_lower<T, K>(F::fma_(ddx, ddx, ddy * ddy), P) >= thr_min);
+                    T cap = r[R_ZCAP] - prof_lower<T, K>(F::fma_(ddx, ddx, ddy * ddy), P);
+                    if (K != K_CONE) {
+                        // the facet plane's highest value over the hull (at
+                        // the corner its gradient points to)
+                        const T gx = r[R_GX], gy = r[R_GY];
+                        const T pl = F::fma_(gy, gy >= (T)0 ? hy1 : hy0, F::fma_(gx, gx >= (T)0 ? hx1 : hx0, r[R_F0]));
+                        cap = F::min_(cap, pl + r[R_PMAR]);
+                    }
+                    keep = keep & (cap >= thr_row);
                 }
             }
-            tmask = (uint32_t)__ballot(keep);
+            const uint64_t rowkeep = __ballot(keep);
+            tmask = (uint32_t)((rowkeep | (rowkeep >> 16) | (rowkeep >> 32) | (rowkeep >> 48)) & 0xffffu);
         }
         // near-tie set: triangles reaching the band below the best height
         // the lane is known to reach (running best or floor)
@@ -1068,7 +1082,10 @@ __device__ __attribute__((always_inline)) inline void select_points(const SelArg
                 const uint32_t ku = uniform_u(t0 + j);
                 const T* r = my + rec_len<T>() * j;
                 const T bx0 = r[R_BX0], bx1 = r[R_BX1], by0 = r[R_BY0], by1 = r[R_BY1];
-                bool on = act & !((bx1 < xl - P.R) | (bx0 > xl + P.R) | (by1 < yl - P.R) | (by0 > yl + P.R));
+                // (the contact code skips a lane whose running best is at or
+                // above the triangle's top: the reference's below() test)
+                bool on = act & (S.zb < r[R_BZ1]) &
+                          !((bx1 < xl - P.R) | (bx0 > xl + P.R) | (by1 < yl - P.R) | (by0 > yl + P.R));
                 if (P.cull) {
                     const T d2 = box_d2<T>(xl, yl, bx0, bx1, by0, by1);
                     on = on & (tri_cap<T, K>(r, xl, yl, d2, P) >= F::max_(S.zb, zc - dz) - P.CUT);

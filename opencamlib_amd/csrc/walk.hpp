@@ -29,6 +29,17 @@ __device__ inline T wave_min_t(T v) {
     for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
     return v;
 }
+// Min/max over the lane's row: lanes 16r .. 16r + 15 of the wavefront.
+template <class T>
+__device__ inline T row_min_t(T v) {
+    for (int o = 8; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+template <class T>
+__device__ inline T row_max_t(T v) {
+    for (int o = 8; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
 __device__ inline uint32_t wave_max_u(uint32_t v) {
     for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
     return v;
