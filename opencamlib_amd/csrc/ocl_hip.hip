@@ -790,10 +790,11 @@ struct SelArgs {
 //   ptab: the wavefront's table of the (owner lane, triangle) pairs of the
 //         batch being evaluated (64 entries)
 // Level-0 boxes a wavefront sorts before evaluating them (a power of two:
-// the bitonic sort runs over the list rounded up to one).  On the 1M-point
-// Ball workload 64 beat 32, 128 and 256 (select 1.53 against 1.63, 1.59 and
-// 1.73 ms): shorter lists are sorted sooner and more often.
-constexpr uint32_t kBoxList = 64;
+// the bitonic sort runs over the list rounded up to one).  With the walk
+// pruned by the hierarchy caps, 32 beat 64 and 128 on the 1M-point Ball
+// workload (select 1.355 against 1.395 and 1.395 ms; BullCutter unchanged);
+// before the pruning 64 had beaten 32, 128 and 256.
+constexpr uint32_t kBoxList = 32;
 
 // Pair packing: the (point, triangle) pairs of a level-0 box are evaluated
 // 64 to a wavefront step, each lane for the pair's owner, and folded back by
